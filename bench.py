@@ -1,0 +1,141 @@
+#!/usr/bin/env python3
+"""Plain-ADMM throughput of the parADMM engine on one MI355X.
+
+A step is one full ADMM iteration (X, M, Z, U, N over every edge) of the
+unit-square circle packing, N = 5000 circles by default: 12,507,500 pair
+factors, 5000 box and 5000 radius factors, E = 25,005,000 edges of dim 3,
+fp32 messages, fixed rho = 1, alpha = 1, seeded synthetic init.  Inputs are
+resident in HBM before the timed region; the timed region is K iterations
+launched through the engine's captured hipGraph and bracketed by HIP events
+on the engine's stream (the library synchronises before and after).
+
+Prints one JSON line: value = edge-updates/s, plus the roofline of the
+dominant kernel (algorithmic bytes / mean launch time from HIP events) and a
+bounded OpenMP CPU baseline (the FP64 restatement of the reference's OpenMP
+path, timed on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per edge of each kernel (fp32): what the kernel must read
+# and write for one edge, excluding index streams and per-factor parameters.
+#   generic X pass (per kind): read U (12) + rho (4), write X (12)       = 28
+#   generic Z pass (fused M, U update): read X, U, rho, alpha; write U = 44
+#   dense fused pair-tile pass: read X, U, rho, alpha; write X, U      = 56
+BYTES_PER_EDGE = {
+    "disk_collision_3d": 28, "box_3d": 28, "max_radius_3d": 28,
+    "zpass_fused": 44,
+    "pairtile_fused": 56, "unary_fused": 56, "zreduce": 0,
+}
+EDGES_PER_LAUNCH = {}
+
+
+def edges_per_launch(name, n):
+    pairs = n * (n - 1) // 2
+    return {
+        "disk_collision_3d": 2 * pairs, "box_3d": n, "max_radius_3d": n,
+        "zpass_fused": n * (n + 1), "pairtile_fused": 2 * pairs, "unary_fused": 2 * n,
+    }.get(name, 0)
+
+
+def cpu_baseline(n, iters, threads):
+    """The reference's OpenMP path (five parallel passes per iteration,
+    packing/host.c:237-244) as the oracle's -Ofast restatement."""
+    from oracle import pyoracle as po
+    g = po.Graph.square(n, 0, 1.0, 1.0, 1.0, fast=True)
+    dt = g.iterate_omp(iters, threads)
+    e = g.nedge
+    g.close()
+    return {"value": e * iters / dt, "unit": "edge-updates/s", "cores": threads, "kind": "port",
+            "sample": f"N={n} unit-square packing, {iters} OpenMP iterations (FP64, -Ofast), {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=5000, help="circles")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--profile-iters", type=int, default=10)
+    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the single-instance solve does not shard across GPUs here).", file=sys.stderr)
+        sys.exit(2)
+
+    from engine_amd import paradmm as P
+
+    t0 = time.time()
+    g = P.packing_square(args.n, args.seed, rho=1.0, alpha=1.0, a_exp=1.0)
+    g.prepare()
+    t_setup = time.time() - t0
+    E = g.nedge
+    layout = {1: "generic", 2: "dense"}.get(g.layout(), "?")
+    print(f"# setup {t_setup:.1f} s: N={args.n} E={E} layout={g.layout()}", file=sys.stderr, flush=True)
+
+    g.time_iterations(args.warmup)
+    ms = g.time_iterations(args.steps)
+    ms_per_step = ms / args.steps
+    value = E * args.steps / (ms * 1e-3)
+    print(f"# timed {args.steps} iterations: {ms_per_step:.4f} ms/iter, {value:.4e} edge-updates/s",
+          file=sys.stderr, flush=True)
+
+    prof = g.profile_kernels(args.profile_iters)
+    per_iter = {k: v[0] * v[1] / args.profile_iters for k, v in prof.items()}
+    dom = max(per_iter, key=per_iter.get)
+    dom_ms = prof[dom][0]
+    dom_bytes = BYTES_PER_EDGE.get(dom, 0) * edges_per_launch(dom, args.n)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    print(f"# kernels (mean ms per launch): " + ", ".join(f"{k} {v[0]:.4f}x{v[1]}" for k, v in prof.items()),
+          file=sys.stderr, flush=True)
+
+    p2, d2 = g.residuals()
+    g.close()
+
+    line = {
+        "metric": "edge-updates/s (%HBM peak) + time to residual <=1e-4, learned-rho vs fixed-rho",
+        "value": value,
+        "unit": "edge-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain ADMM unit-square circle packing, N={args.n} circles, E={E} edges of dim 3, "
+                        f"fixed rho=1, alpha=1, radius exponent 1, seed {args.seed}",
+            "circles": args.n, "edges": E, "layout": layout,
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_launch": dom_bytes, "mean_launch_ms": dom_ms,
+        },
+        "kernels_ms": {k: v[0] for k, v in prof.items()},
+        "residuals_last": {"primal2": p2, "dual2": d2},
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_iters, threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

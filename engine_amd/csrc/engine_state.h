@@ -1,0 +1,108 @@
+// engine_state.h -- device state of one graph, shared by engine.hip and the
+// layout-specific files.  Internal.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "paradmm.h"
+
+namespace pdm {
+
+struct KindInfo {
+    int deg;      // edges per factor; 0 = any degree, each edge independent
+    int dims;     // num_dims the operator is defined for
+    int nparams;  // doubles read from the factor's parameter pointer
+    const char* name;
+};
+const KindInfo& kind_info(int kind);
+
+struct KindTable {
+    int kind = 0;
+    int nitems = 0;
+    int* d_items = nullptr;     // [deg][nitems] device positions
+    float* d_params = nullptr;  // [nitems][nparams]
+};
+
+struct DenseState;  // dense.hip
+
+// One timed kernel launch: events recorded on the engine stream around it.
+struct ProfRec {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+struct DevGraph {
+    graph* G = nullptr;
+    int E = 0, V = 0, F = 0, D = 0;
+    int layout = 0;   // 1 generic, 2 dense pair tiles
+    long stride = 0;  // elements per component of the edge SoA buffers
+    long npos = 0;
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void*> allocs;
+
+    // maps
+    int* d_pos = nullptr;           // api edge -> device position
+    int* d_edge_var_api = nullptr;  // api edge -> variable
+    int* d_edge_var_dev = nullptr;  // device position -> variable (generic)
+    int* d_var_off = nullptr;       // [V+1] CSR offsets (generic)
+    std::vector<KindTable> kinds;   // generic X pass tables
+
+    // state
+    float *d_X = nullptr, *d_U = nullptr, *d_N = nullptr, *d_M = nullptr;
+    float *d_rho = nullptr, *d_alpha = nullptr;
+    float *d_Z = nullptr, *d_Zprev = nullptr;
+    double* d_res_part = nullptr;  // residual partials
+    double* d_res = nullptr;       // [2] primal^2, dual^2
+    double* d_stage = nullptr;     // host transfer staging (fp64)
+
+    // coherence
+    bool host_newer = true;    // host arrays hold data the device has not seen
+    bool dev_newer = false;    // device holds iterations the host has not seen
+    bool n_from_buffer = true; // the next X pass must read N from the N buffer
+    bool m_valid = false;      // the M buffer holds the M of the last iteration
+    bool rho_changed = true;   // rho/alpha uploaded since the last per-variable sums
+    bool rho_device_written = false;
+    bool iters_done = false;
+    bool dense_unary_stale = false;
+
+    // captured iterations
+    int graph_iters = 16;
+    hipGraph_t ghandle = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    DenseState* dense = nullptr;
+
+    // kernel timing (paradmm_gpu_profile_kernels): eager launches only
+    bool prof = false;
+    std::vector<ProfRec> prof_recs;
+    std::vector<hipEvent_t> prof_pool;
+    size_t prof_used = 0;
+};
+
+// Brackets one kernel launch with events when profiling (no-op otherwise).
+void prof_begin(DevGraph* s, const char* name);
+void prof_end(DevGraph* s);
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+int dmalloc(DevGraph* s, void** p, size_t bytes);
+int ensure_aux(DevGraph* s);
+int launch_mrecon(DevGraph* s);
+int launch_npass(DevGraph* s);
+int residual_reduce(DevGraph* s);
+int warm_start_device(DevGraph* s, int maxiter);
+void free_dev(DevGraph* s);
+
+// dense.hip: returns -1 if the graph is not an all-pairs dim-3 packing,
+// 0 when the dense layout was built, >0 on error.
+int dense_try_finalize(DevGraph* s, graph* G, const std::vector<int>& fkind);
+int dense_alloc_state(DevGraph* s);
+int dense_iteration(DevGraph* s, int from_n);
+int dense_legacy_xpass(DevGraph* s);
+int dense_legacy_zpass(DevGraph* s);
+int dense_residual_reduce(DevGraph* s);
+void dense_free(DevGraph* s);
+
+}  // namespace pdm

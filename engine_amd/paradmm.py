@@ -1,0 +1,257 @@
+"""Python mirror of the parADMM C API (paradmm.h + paradmm_gpu.h) over ctypes.
+
+The product is libparadmm.so (C ABI, HIP kernels for gfx950).  This module
+only binds it: the `graph` struct of core/paradmm.h:33-63, the legacy entry
+points (startG, addNode, initialize_*, update*) and the MI355X extension
+(paradmm_gpu_*).  It raises at import time if the library is not built; there
+is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libparadmm.so")
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+
+PROX_FN = C.CFUNCTYPE(None, _DP, _DP, _DP, C.c_int, C.c_int, C.c_void_p)
+
+KIND_PAIR_2D, KIND_WALL_2D, KIND_RADIUS_2D, KIND_PAIR_3D, KIND_BOX_3D, KIND_RADIUS_3D = 1, 2, 3, 4, 5, 6
+
+
+class GraphStruct(C.Structure):
+    """`graph` of core/paradmm.h:33-63, field for field."""
+    _fields_ = [
+        ("num_opts", C.c_int),
+        ("num_vars", C.c_int),
+        ("num_dims", C.c_int),
+        ("num_edges", C.c_int),
+        ("list_of_operators_functions_pointers", C.POINTER(C.c_void_p)),
+        ("list_of_operators_param_pointers", C.POINTER(C.c_void_p)),
+        ("list_of_operators_param_sizes", _IP),
+        ("operatorix_to_numedges", _IP),
+        ("operatorix_to_edgelistix", C.POINTER(_IP)),
+        ("varix_to_numedges", _IP),
+        ("varix_to_edgelistix", C.POINTER(_IP)),
+        ("edgeix_to_varlistix", _IP),
+        ("X", _DP), ("M", _DP), ("N", _DP), ("U", _DP),
+        ("Z", _DP),
+        ("RHOS", _DP), ("ALPHAS", _DP),
+    ]
+
+
+class ParadmmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Loads libparadmm.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run `make -C engine_amd` (or __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    G = C.POINTER(GraphStruct)
+    V = C.c_void_p
+    sig = {
+        "abs_d": (C.c_double, [C.c_double]),
+        "quicksort": (None, [_IP, C.c_int]),
+        "get_random": (C.c_double, [C.c_double, C.c_double]),
+        "startG": (None, [G, C.c_int]),
+        "closeG": (None, [G]),
+        "printG": (None, [G]),
+        "initialize_RHOS_APHAS": (None, [G, C.c_double, C.c_double]),
+        "initialize_X_N_Z_M_U_zeros": (None, [G]),
+        "initialize_X_N_Z_M_U_rand": (None, [G] + [C.c_double] * 10),
+        "initialize_X_N_Z_M_U_optimal": (None, [G, _DP, _DP, C.c_int]),
+        "addNode": (None, [G, V, V, C.c_int, _IP]),
+        "addNodeWithParamSize": (None, [G, V, V, C.c_int, C.c_int, _IP]),
+        "update_outgoing_rhos": (None, [G]),
+        "updateXM": (None, [G]), "updateX": (None, [G]), "updateM": (None, [G]),
+        "updateZ": (None, [G]), "updateUN": (None, [G]), "updateU": (None, [G]),
+        "updateN": (None, [G]), "updateG": (None, [G]),
+        "updateXOpenMP": (None, [G]), "updateMOpenMP": (None, [G]), "updateZOpenMP": (None, [G]),
+        "updateUOpenMP": (None, [G]), "updateNOpenMP": (None, [G]),
+        "updateGOpenMP": (None, [G, C.c_int, C.c_int]),
+        "paradmm_last_error": (C.c_char_p, []),
+        "paradmm_register_prox_kind": (C.c_int, [V, C.c_int]),
+        "paradmm_prox_kind_of": (C.c_int, [V]),
+        "paradmm_packing_square": (C.c_int, [G, C.c_int, C.c_double]),
+        "paradmm_packing_init_square": (C.c_int, [G, C.c_uint64]),
+        "paradmm_gpu_prepare": (C.c_int, [G]),
+        "paradmm_gpu_iterate": (C.c_int, [G, C.c_int]),
+        "paradmm_gpu_sync_to_host": (C.c_int, [G]),
+        "paradmm_gpu_sync_from_host": (C.c_int, [G]),
+        "paradmm_gpu_release": (C.c_int, [G]),
+        "paradmm_gpu_layout": (C.c_int, [G]),
+        "paradmm_gpu_residuals": (C.c_int, [G, _DP, _DP]),
+        "paradmm_gpu_time_iterations": (C.c_int, [G, C.c_int, _DP]),
+        "paradmm_gpu_profile_kernels": (C.c_int, [G, C.c_int, C.c_int, C.c_char_p, _DP, _IP, _IP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names the C headers declare (checked against the library's exports by the tests)."""
+    return [
+        "abs_d", "quicksort", "get_random", "realloc2", "startG", "closeG", "printG",
+        "initialize_RHOS_APHAS", "initialize_X_N_Z_M_U_zeros", "initialize_X_N_Z_M_U_rand",
+        "initialize_X_N_Z_M_U_optimal", "addNode", "addNodeWithParamSize", "update_outgoing_rhos",
+        "updateXM", "updateX", "updateM", "updateZ", "updateUN", "updateU", "updateN", "updateG",
+        "updateXOpenMP", "updateMOpenMP", "updateZOpenMP", "updateUOpenMP", "updateNOpenMP", "updateGOpenMP",
+    ]
+
+
+def prox_address(name):
+    """Address of one of the library's built-in prox operators (a device kind token)."""
+    lib = load()
+    return C.cast(getattr(lib, name), C.c_void_p).value
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = load().paradmm_last_error().decode(errors="replace")
+        raise ParadmmError(f"{what}: {msg}" if what else msg)
+
+
+class Graph:
+    """Owns one `graph` struct of libparadmm.  Arrays are numpy views on the
+    library-owned host arrays (edge-major AoS, as in the reference)."""
+
+    def __init__(self, ndims):
+        self.lib = load()
+        self.g = GraphStruct()
+        self.ptr = C.pointer(self.g)
+        self.lib.startG(self.ptr, ndims)
+        self._keep = []  # parameter blocks referenced by factors
+
+    # ---- construction
+    def add_factor(self, fn_addr, variables, params=None):
+        vars_arr = (C.c_int * len(variables))(*variables)
+        p = None
+        if params is not None:
+            buf = (C.c_double * len(params))(*params)
+            self._keep.append(buf)
+            p = C.cast(buf, C.c_void_p)
+        self.lib.addNode(self.ptr, C.c_void_p(fn_addr), p, len(variables), vars_arr)
+
+    def packing_square(self, ncircles, a_exp=1.0):
+        check(self.lib.paradmm_packing_square(self.ptr, ncircles, a_exp), "paradmm_packing_square")
+
+    def init_rho_alpha(self, rho, alpha):
+        self.lib.initialize_RHOS_APHAS(self.ptr, rho, alpha)
+
+    def init_zeros(self):
+        self.lib.initialize_X_N_Z_M_U_zeros(self.ptr)
+
+    def init_square(self, seed):
+        check(self.lib.paradmm_packing_init_square(self.ptr, seed), "paradmm_packing_init_square")
+
+    # ---- views
+    @property
+    def nedge(self):
+        return self.g.num_edges
+
+    @property
+    def nvar(self):
+        return self.g.num_vars
+
+    @property
+    def nfac(self):
+        return self.g.num_opts
+
+    @property
+    def ndims(self):
+        return self.g.num_dims
+
+    def array(self, name):
+        p = getattr(self.g, name)
+        if not p:
+            raise ParadmmError(f"{name} is not allocated")
+        if name == "Z":
+            n = self.nvar * self.ndims
+        elif name in ("RHOS", "ALPHAS"):
+            n = self.nedge
+        else:
+            n = self.nedge * self.ndims
+        return np.ctypeslib.as_array(p, shape=(n,))
+
+    def edge_var(self):
+        return np.ctypeslib.as_array(self.g.edgeix_to_varlistix, shape=(self.nedge,)).copy()
+
+    # ---- legacy passes
+    def legacy(self, name, *args):
+        getattr(self.lib, name)(self.ptr, *args)
+
+    # ---- device
+    def prepare(self):
+        check(self.lib.paradmm_gpu_prepare(self.ptr), "paradmm_gpu_prepare")
+
+    def iterate(self, iters):
+        check(self.lib.paradmm_gpu_iterate(self.ptr, iters), "paradmm_gpu_iterate")
+
+    def sync_to_host(self):
+        check(self.lib.paradmm_gpu_sync_to_host(self.ptr), "paradmm_gpu_sync_to_host")
+
+    def sync_from_host(self):
+        check(self.lib.paradmm_gpu_sync_from_host(self.ptr), "paradmm_gpu_sync_from_host")
+
+    def layout(self):
+        return self.lib.paradmm_gpu_layout(self.ptr)
+
+    def residuals(self):
+        p, d = C.c_double(), C.c_double()
+        check(self.lib.paradmm_gpu_residuals(self.ptr, C.byref(p), C.byref(d)), "paradmm_gpu_residuals")
+        return p.value, d.value
+
+    def time_iterations(self, iters):
+        ms = C.c_double()
+        check(self.lib.paradmm_gpu_time_iterations(self.ptr, iters, C.byref(ms)), "paradmm_gpu_time_iterations")
+        return ms.value
+
+    def profile_kernels(self, iters, max_kernels=16):
+        """{kernel name: (mean ms, launches)} over iters eagerly launched iterations."""
+        names = C.create_string_buffer(64 * max_kernels)
+        ms = (C.c_double * max_kernels)()
+        cnt = (C.c_int * max_kernels)()
+        nk = C.c_int()
+        check(self.lib.paradmm_gpu_profile_kernels(self.ptr, iters, max_kernels, names, ms, cnt, C.byref(nk)),
+              "paradmm_gpu_profile_kernels")
+        raw = names.raw
+        out = {}
+        for j in range(nk.value):
+            nm = raw[64 * j:64 * (j + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = (ms[j], cnt[j])
+        return out
+
+    def close(self):
+        if self.ptr is not None:
+            self.lib.closeG(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def packing_square(ncircles, seed, rho=1.0, alpha=1.0, a_exp=1.0):
+    """Builds and initialises the unit-square packing problem (dim 3)."""
+    g = Graph(3)
+    g.packing_square(ncircles, a_exp)
+    g.init_rho_alpha(rho, alpha)
+    g.init_square(seed)
+    return g

@@ -1,0 +1,89 @@
+"""Writes the golden fixtures of tests/golden/ from the reference compiled from
+its own sources (oracle/_ref, built by `make -C oracle` where the reference
+is present).  Run once in a container that has the reference; the fixtures are
+committed and travel, the reference does not.
+
+Fixtures (numpy .npz, float64):
+  square20_seed0.npz   unit-square packing N=20 (dim 3), rho=1, alpha=1, a=1:
+                       reference engine (core/paradmm.c updateXM/updateZ/updateUN)
+                       with the oracle's dim-3 operators; Z after iterations 1..200
+                       and after 5000.
+  square20_seed0_rho100.npz   same at rho=100 (the converging setting).
+  triangle6_seed1.npz  the reference demo's triangle packing (packing/host.c
+                       factor order and operators) N=6, seeded init in [-1,2];
+                       Z after iterations 1..200.
+  prox2_kat.npz        the reference's own dim-2 operators (packing/host.c) on
+                       seeded random inputs: inputs and outputs.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", ".."))
+
+from oracle import pyoracle as po  # noqa: E402
+
+
+def square(ref, nc, seed, rho, alpha, a, iters):
+    nv = nc * 3
+    hist = np.zeros((iters, nv))
+    ref.ref_square_run(nc, seed, rho, alpha, a, iters, po.dptr(hist))
+    return hist
+
+
+def main():
+    ref = po.load_ref()
+    if ref is None:
+        raise SystemExit("oracle/_ref is not built (needs the reference sources): make -C oracle")
+    h = square(ref, 20, 0, 1.0, 1.0, 1.0, 5000)
+    np.savez_compressed(os.path.join(HERE, "square20_seed0.npz"), z=h[:200], z_final=h[-1],
+                        n=20, seed=0, rho=1.0, alpha=1.0, a=1.0, iters_final=5000)
+    h = square(ref, 20, 0, 100.0, 1.0, 1.0, 5000)
+    np.savez_compressed(os.path.join(HERE, "square20_seed0_rho100.npz"), z=h[:200], z_final=h[-1],
+                        n=20, seed=0, rho=100.0, alpha=1.0, a=1.0, iters_final=5000)
+
+    nc, iters = 6, 200
+    hist = np.zeros((iters, 2 * nc * 2))
+    ref.ref_triangle_run(nc, 1, -1.0, 2.0, iters, po.dptr(hist))
+    np.savez_compressed(os.path.join(HERE, "triangle6_seed1.npz"), z=hist, n=nc, seed=1, lo=-1.0, hi=2.0)
+
+    # known-answer vectors of the reference's dim-2 operators
+    rng = np.random.default_rng(12345)
+    m = 400
+    pair_n = rng.uniform(-1, 2, size=(m, 8))
+    pair_n[:, 2:4] = pair_n[:, 0:2] + rng.uniform(-0.5, 0.5, size=(m, 2))  # many overlaps
+    pair_rho = rng.uniform(0.5, 20, size=(m, 4))
+    pair_x = np.zeros((m, 8))
+    for i in range(m):
+        x = np.zeros(8)
+        po.call_prox(ref.ref_prox_pair2, x, pair_n[i].copy(), pair_rho[i].copy(), 4, 2)
+        pair_x[i] = x
+    s = np.sin(np.pi / 3)
+    walls = np.array([[0., -1., 0., 0.], [-s, 0.5, 0., 0.], [s, 0.5, 1., 0.]])
+    wall_n = rng.uniform(-1, 2, size=(m, 4))
+    wall_rho = rng.uniform(0.5, 20, size=(m, 2))
+    wall_k = rng.integers(0, 3, size=m)
+    wall_x = np.zeros((m, 4))
+    for i in range(m):
+        x = np.zeros(4)
+        po.call_prox(ref.ref_prox_wall2, x, wall_n[i].copy(), wall_rho[i].copy(), 2, 2, walls[wall_k[i]].copy())
+        wall_x[i] = x
+    rad_n = rng.uniform(-1, 2, size=(m, 2))
+    rad_rho = rng.uniform(2.5, 20, size=(m, 1))
+    rad_a = np.where(rng.uniform(size=m) < 0.5, 1.0, 2.0)
+    rad_x = np.zeros((m, 2))
+    for i in range(m):
+        x = np.zeros(2)
+        po.call_prox(ref.ref_prox_radius2, x, rad_n[i].copy(), rad_rho[i].copy(), 1, 2, np.array([rad_a[i]]))
+        rad_x[i] = x
+    np.savez_compressed(os.path.join(HERE, "prox2_kat.npz"), pair_n=pair_n, pair_rho=pair_rho, pair_x=pair_x,
+                        walls=walls, wall_n=wall_n, wall_rho=wall_rho, wall_k=wall_k, wall_x=wall_x,
+                        rad_n=rad_n, rad_rho=rad_rho, rad_a=rad_a, rad_x=rad_x)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,130 @@
+"""The C ABI of libparadmm without a GPU: the library loads, exports every
+symbol the headers declare, builds graphs exactly as the reference does, and
+its error paths return codes before any device work."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def header_symbols():
+    names = set()
+    for h in ("paradmm.h", "paradmm_gpu.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt):
+            names.add(m.group(1))
+    drop = {"if", "defined", "sizeof", "op", "Operator", "paradmm_prox_fn", "void"}
+    return sorted(n for n in names if n not in drop and not n.startswith("__"))
+
+
+def test_library_exports_every_declared_symbol(pdm):
+    out = subprocess.run(["nm", "-D", "--defined-only", pdm.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    declared = header_symbols()
+    assert len(declared) > 40
+    missing = [n for n in declared if n not in exported]
+    assert not missing, missing
+    for n in pdm.exported_symbols():
+        assert n in exported
+
+
+def test_builder_matches_reference_structure(pdm, oracle):
+    """Edge numbering, edge->var and per-variable lists equal the reference builder's."""
+    g = pdm.packing_square(9, 1)
+    o = oracle.Graph.square(9, 1)
+    assert (g.nedge, g.nvar, g.nfac) == (o.nedge, o.nvar, o.nfac) == (90, 9, 9 + 9 + 36)
+    assert np.array_equal(g.edge_var(), o.edge_var())
+    lib = o.lib
+    for v in range(g.nvar):
+        deg = g.g.varix_to_numedges[v]
+        assert deg == lib.or_var_deg(o.h, v) == 10
+        mine = [g.g.varix_to_edgelistix[v][i] for i in range(deg)]
+        theirs = [lib.or_var_edges(o.h, v)[i] for i in range(deg)]
+        assert mine == theirs
+    first = np.ctypeslib.as_array(lib.or_fac_first(o.h), shape=(o.nfac,))
+    for f in range(g.nfac):
+        assert g.g.operatorix_to_edgelistix[f][0] == first[f]
+    for name, oname in [("X", "X"), ("N", "N"), ("M", "M"), ("U", "U"), ("Z", "Z"), ("RHOS", "RHO")]:
+        assert np.array_equal(g.array(name), o.array(oname)), name
+
+
+def test_addnode_sorts_a_copy_and_handles_growth(pdm):
+    g = pdm.Graph(2)
+    addr = pdm.prox_address("paradmm_prox_disk_collision_2d")
+    rng = np.random.default_rng(0)
+    lists = []
+    for _ in range(3000):  # crosses several capacity doublings
+        v = rng.choice(200, size=4, replace=False)
+        arr = (C.c_int * 4)(*v.tolist())
+        g.lib.addNode(g.ptr, C.c_void_p(addr), None, 4, arr)
+        assert list(arr) == v.tolist()  # caller's list untouched
+        lists.append(sorted(v.tolist()))
+    assert g.nfac == 3000 and g.nedge == 12000
+    ev = g.edge_var()
+    for f in (0, 1, 777, 2999):
+        s = g.g.operatorix_to_edgelistix[f][0]
+        assert s == 4 * f
+        assert [g.g.operatorix_to_edgelistix[f][i] for i in range(4)] == [s, s + 1, s + 2, s + 3]
+        assert ev[s:s + 4].tolist() == lists[f]
+    total = sum(g.g.varix_to_numedges[v] for v in range(g.nvar))
+    assert total == 12000
+
+
+def test_unknown_callback_is_reported(pdm):
+    """A host callback the engine cannot run on the GPU: an error code, no device work."""
+    cb = pdm.PROX_FN(lambda *a: None)
+    g = pdm.Graph(3)
+    vv = (C.c_int * 1)(0)
+    g.lib.addNode(g.ptr, C.cast(cb, C.c_void_p), None, 1, vv)
+    g.init_rho_alpha(1.0, 1.0)
+    g.init_zeros()
+    rc = g.lib.paradmm_gpu_prepare(g.ptr)
+    assert rc == 2
+    assert "not a device prox kind" in g.lib.paradmm_last_error().decode()
+    # registering it as a kind makes it acceptable (checked again on the GPU)
+    assert g.lib.paradmm_register_prox_kind(C.cast(cb, C.c_void_p), 5) == 0
+    assert g.lib.paradmm_prox_kind_of(C.cast(cb, C.c_void_p)) == 5
+    g.close()
+
+
+def test_shape_mismatch_is_reported(pdm):
+    g = pdm.Graph(2)  # dim-3 operator on a dim-2 graph
+    vv = (C.c_int * 2)(0, 1)
+    g.lib.addNode(g.ptr, C.c_void_p(pdm.prox_address("paradmm_prox_disk_collision_3d")), None, 2, vv)
+    g.init_rho_alpha(1.0, 1.0)
+    g.init_zeros()
+    assert g.lib.paradmm_gpu_prepare(g.ptr) == 3
+    assert "do not fit" in g.lib.paradmm_last_error().decode()
+
+
+def test_builtin_kinds_registered(pdm):
+    lib = pdm.load()
+    for name, k in [("paradmm_prox_disk_collision_2d", 1), ("paradmm_prox_disk_plane_2d", 2),
+                    ("paradmm_prox_max_radius_2d", 3), ("paradmm_prox_disk_collision_3d", 4),
+                    ("paradmm_prox_box_3d", 5), ("paradmm_prox_max_radius_3d", 6)]:
+        assert lib.paradmm_prox_kind_of(C.c_void_p(pdm.prox_address(name))) == k
+
+
+def test_utilities(pdm):
+    lib = pdm.load()
+    assert lib.abs_d(-2.5) == 2.5 and lib.abs_d(3.0) == 3.0
+    a = np.random.default_rng(2).integers(-50, 50, size=300).astype(np.int32)
+    arr = (C.c_int * len(a))(*a.tolist())
+    lib.quicksort(arr, len(a))
+    assert list(arr) == sorted(a.tolist())
+    r = [lib.get_random(-1.0, 2.0) for _ in range(100)]
+    assert min(r) >= -1.0 and max(r) <= 2.0
+
+
+def test_seeded_init_equals_oracle(pdm, oracle):
+    g = pdm.packing_square(30, 77, 2.0, 0.5)
+    o = oracle.Graph.square(30, 77, 2.0, 0.5)
+    assert np.array_equal(g.array("Z"), o.array("Z"))
+    assert np.array_equal(g.array("X"), o.array("X"))
+    assert np.array_equal(g.array("ALPHAS"), o.array("ALPHA"))
