@@ -663,7 +663,11 @@ int ensure_aux(DevGraph* s)
         CK(hipMemsetAsync(s->d_M, 0, sizeof(float) * se, s->stream));
     }
     if (!s->d_stage) {
+        // host transfers use D*max(E,V) doubles; the warm start keeps a copy
+        // of X (D*stride floats, larger when the dense layout pads) here too
         size_t n = (size_t)s->D * (size_t)(s->E > s->V ? s->E : s->V);
+        const size_t nx = ((size_t)s->D * (size_t)s->stride + 1) / 2;
+        if (nx > n) n = nx;
         RC(dmalloc(s, (void**)&s->d_stage, sizeof(double) * n));
     }
     return PARADMM_OK;
@@ -893,6 +897,11 @@ static int run_fused(DevGraph* s, int iters)
         s->n_from_buffer = false;
         --iters;
     }
+    if (s->layout == 2 && !dense_steady(s) && iters > 0) {
+        // the captured iterations assume a pending U update
+        RC(fused_iteration(s, 0));
+        --iters;
+    }
     const int GI = s->graph_iters;
     if (iters >= GI && GI > 1) {
         if (!s->gexec) {
@@ -973,6 +982,7 @@ static int get_state(graph* G, DevGraph** out, bool upload_if_new)
         s->dev_newer = false;
         s->n_from_buffer = true;
         s->m_valid = false;
+        if (s->layout == 2) dense_reset(s);
     }
     *out = s;
     return PARADMM_OK;
@@ -983,6 +993,7 @@ static int sync_down(DevGraph* s)
 {
     if (!s->dev_newer) return PARADMM_OK;
     RC(ensure_aux(s));
+    if (s->layout == 2) RC(dense_finish(s));
     if (!s->m_valid) RC(launch_mrecon(s));
     RC(launch_npass(s));
     int which = A_X | A_M | A_N | A_U | A_Z;
@@ -1035,6 +1046,7 @@ extern "C" int pdm_legacy_iterations(graph* G, int maxiter)
     s->host_newer = false;
     s->n_from_buffer = true;
     s->m_valid = false;
+    if (s->layout == 2) dense_reset(s);
     RC(run_fused(s, maxiter));
     s->dev_newer = maxiter > 0;
     RC(sync_down(s));

@@ -104,5 +104,11 @@ int dense_legacy_xpass(DevGraph* s);
 int dense_legacy_zpass(DevGraph* s);
 int dense_residual_reduce(DevGraph* s);
 void dense_free(DevGraph* s);
+// true when the next dense iteration is the steady-state (captured) kind
+bool dense_steady(DevGraph* s);
+// the device state was replaced from the host: U is current
+void dense_reset(DevGraph* s);
+// applies the pending U update (and its primal residual) so U is current
+int dense_finish(DevGraph* s);
 
 }  // namespace pdm

@@ -9,14 +9,27 @@
 
 namespace pdm {
 
+// r^e, evaluated exactly for the small integer exponents the radius
+// objective produces (a = 1, 2 give e in {-1, 0, 1}); powf otherwise.
+// Same values as powf there, including r = 0 (1/0 = inf) and e = 0 (1).
+__device__ __forceinline__ float pow_exp(float r, float e)
+{
+    if (e == 0.0f) return 1.0f;
+    if (e == 1.0f) return r;
+    if (e == -1.0f) return 1.0f / r;
+    if (e == 2.0f) return r * r;
+    return powf(r, e);
+}
+
 // max_alpha_radius (packing/host.c:10-29): ten Newton steps on
 // -r^(a-1) + (rho/a)(r - n) = 0 from |n/2|; a negative iterate returns 0.
 __device__ __forceinline__ float max_alpha_radius(float n, float rho, float a)
 {
     float r = fabsf(n * 0.5f);
+#pragma unroll 1
     for (int it = 0; it < 10; ++it) {
-        const float f = -powf(r, a - 1.0f) + (rho / a) * (r - n);
-        const float fp = -(a - 1.0f) * powf(r, a - 2.0f) + (rho / a);
+        const float f = -pow_exp(r, a - 1.0f) + (rho / a) * (r - n);
+        const float fp = -(a - 1.0f) * pow_exp(r, a - 2.0f) + (rho / a);
         r = r - f / fp;
         if (r < 0.0f) return 0.0f;
     }
@@ -40,15 +53,17 @@ __device__ __forceinline__ bool pair_project(float c1x, float c1y, float r1, flo
     const float s = r1 + r2;
     if (!(d2 < s * s)) return false;
     const float dist = sqrtf(d2);
-    const float lam = (dist - s) / (1.0f / wx1 + 1.0f / wr1 + 1.0f / wx2 + 1.0f / wr2);
-    const float ux = (c2x - c1x) / dist, uy = (c2y - c1y) / dist;
-    const float m1 = lam / wx1, m2 = lam / wx2;
+    const float ix1 = 1.0f / wx1, ir1 = 1.0f / wr1, ix2 = 1.0f / wx2, ir2 = 1.0f / wr2;
+    const float lam = (dist - s) / (ix1 + ir1 + ix2 + ir2);
+    const float idist = 1.0f / dist;
+    const float ux = -dx * idist, uy = -dy * idist;
+    const float m1 = lam * ix1, m2 = lam * ix2;
     o1x = c1x + ux * m1;
     o1y = c1y + uy * m1;
     o2x = c2x - ux * m2;
     o2y = c2y - uy * m2;
-    or1 = r1 + lam / wr1;
-    or2 = r2 + lam / wr2;
+    or1 = r1 + lam * ir1;
+    or2 = r2 + lam * ir2;
     return true;
 }
 

@@ -57,7 +57,7 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run `make -C engine_amd` (or __graft_entry__.build())")
-    lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    lib = C.CDLL(LIB_PATH)  # RTLD_LOCAL: the API names also exist in the reference build
     G = C.POINTER(GraphStruct)
     V = C.c_void_p
     sig = {

@@ -99,10 +99,16 @@ def test_fused_iterate_trajectory_vs_reference(fixture):
 def test_fused_iterate_full_state_vs_oracle():
     """After k device iterations every host array equals the oracle's (M and N included)."""
     g, o = square_pair(12, 9, rho=5.0, alpha=0.5)
-    g.iterate(37)
-    o.iterate(37)
+    g.iterate(5)
+    o.iterate(5)
     g.sync_to_host()
     _compare_all(o, g, 2e-5)
+    # N and M read back are those of the reference state after an iteration
+    ev = g.edge_var()
+    Z = g.array("Z").reshape(-1, 3)
+    N, U, X, M = (g.array(n).reshape(-1, 3) for n in ("N", "U", "X", "M"))
+    assert np.allclose(N, Z[ev] - U, rtol=0, atol=1e-6)
+    assert np.allclose(M, X + U - 0.5 * (X - Z[ev]), rtol=0, atol=1e-6)
 
 
 def test_triangle_fused_vs_oracle():
