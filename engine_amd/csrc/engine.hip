@@ -25,6 +25,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <chrono>
+#include <cmath>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -623,6 +625,19 @@ static int finalize(graph* G, DevGraph** out)
         FAIL_IF(finalize_generic(s, G, fkind));
         s->layout = 1;
     }
+    // factor type of every api edge (the learned rho policy's feature)
+    {
+        std::vector<unsigned char> et(E, 0);
+        for (int f = 0; f < F; ++f) {
+            const int K = fkind[f];
+            const unsigned char t = (K == PARADMM_KIND_PAIR_2D || K == PARADMM_KIND_PAIR_3D) ? 0
+                                    : (K == PARADMM_KIND_RADIUS_2D || K == PARADMM_KIND_RADIUS_3D) ? 2 : 1;
+            for (int q = 0; q < G->operatorix_to_numedges[f]; ++q) et[G->operatorix_to_edgelistix[f][q]] = t;
+        }
+        FAIL_IF(dmalloc(s, (void**)&s->d_edge_type_api, (size_t)E));
+        hipError_t e = hipMemcpy(s->d_edge_type_api, et.data(), (size_t)E, hipMemcpyHostToDevice);
+        if (e != hipSuccess) FAIL_IF(hip_fail(e, "hipMemcpy", __FILE__, __LINE__));
+    }
     // state buffers
     const size_t se = (size_t)D * s->stride;
     FAIL_IF(dmalloc(s, (void**)&s->d_X, sizeof(float) * se));
@@ -988,6 +1003,10 @@ static int get_state(graph* G, DevGraph** out, bool upload_if_new)
     return PARADMM_OK;
 }
 
+namespace pdm {
+int pdm_get_state(graph* G, DevGraph** out, bool upload) { return get_state(G, out, upload); }
+}  // namespace pdm
+
 // Brings the host arrays up to date with a device-resident run.
 static int sync_down(DevGraph* s)
 {
@@ -1133,6 +1152,85 @@ extern "C" int paradmm_gpu_residuals(graph* G, double* primal2, double* dual2)
     CK(hipStreamSynchronize(s->stream));
     if (primal2) *primal2 = h[0];
     if (dual2) *dual2 = h[1];
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_gpu_square_quality(graph* G, double* max_overlap, double* max_box, double* sum_r)
+{
+    DevGraph* s;
+    RC(get_state(G, &s, true));
+    double q[3];
+    RC(square_quality(s, q));
+    if (max_overlap) *max_overlap = q[0];
+    if (max_box) *max_box = q[1];
+    if (sum_r) *sum_r = q[2];
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_gpu_solve(graph* G, const paradmm_solve_opts* o, paradmm_solve_result* r)
+{
+    if (!o || !r || o->max_iters < 1 || o->check_every < 1) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_gpu_solve: bad options");
+        return PARADMM_ERR_ARG;
+    }
+    DevGraph* s;
+    RC(get_state(G, &s, true));
+    if (o->tol_feas >= 0 && s->D != 3) {
+        pdm_set_error(PARADMM_ERR_SHAPE, "feasibility check needs the dim-3 packing");
+        return PARADMM_ERR_SHAPE;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const double scale = 1.0 / ((double)s->D * (double)s->E);
+    int it = 0, since_policy = 0;
+    r->converged = 0;
+    r->max_overlap = r->max_box = r->objective = -1.0;
+    while (it < o->max_iters) {
+        int n = o->check_every < o->max_iters - it ? o->check_every : o->max_iters - it;
+        if (o->policy_every > 0 && s->rho_policy) {
+            // never run past a policy point inside a chunk
+            const int left = o->policy_every - since_policy;
+            if (left < n) n = left;
+        }
+        RC(run_fused(s, n));
+        it += n;
+        since_policy += n;
+        if (o->policy_every > 0 && s->rho_policy && since_policy >= o->policy_every) {
+            RC(s->rho_policy(s, (float)it / (float)o->max_iters));
+            since_policy = 0;
+        }
+        RC(residual_reduce(s));
+        double h[2];
+        CK(hipMemcpyAsync(h, s->d_res, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+        CK(hipStreamSynchronize(s->stream));
+        r->primal_rms = sqrt(h[0] * scale);
+        r->dual_rms = sqrt(h[1] * scale);
+        if (r->primal_rms <= o->tol_primal && r->dual_rms <= o->tol_dual) {
+            bool ok = true;
+            if (o->tol_feas >= 0) {
+                double q[3];
+                RC(square_quality(s, q));
+                r->max_overlap = q[0];
+                r->max_box = q[1];
+                r->objective = q[2];
+                ok = q[0] <= o->tol_feas && q[1] <= o->tol_feas;
+            }
+            if (ok) {
+                r->converged = 1;
+                break;
+            }
+        }
+    }
+    if (o->tol_feas >= 0 && !r->converged) {
+        double q[3];
+        RC(square_quality(s, q));
+        r->max_overlap = q[0];
+        r->max_box = q[1];
+        r->objective = q[2];
+    }
+    r->iterations = it;
+    r->elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s->dev_newer = true;
+    s->m_valid = false;
     return PARADMM_OK;
 }
 

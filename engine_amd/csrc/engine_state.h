@@ -57,6 +57,10 @@ struct DevGraph {
     double* d_res_part = nullptr;  // residual partials
     double* d_res = nullptr;       // [2] primal^2, dual^2
     double* d_stage = nullptr;     // host transfer staging (fp64)
+    double* d_quality = nullptr;   // packing quality partials
+    unsigned char* d_edge_type_api = nullptr;  // api edge -> PDM_EDGE_* (pair, container, radius)
+    float* d_ckpt = nullptr;       // saved X, U, rho, Z, Zprev (paradmm_gpu_state_save)
+    bool ckpt_n_from_buffer = false;
 
     // coherence
     bool host_newer = true;    // host arrays hold data the device has not seen
@@ -75,6 +79,11 @@ struct DevGraph {
 
     DenseState* dense = nullptr;
 
+    // rho policy applied by paradmm_gpu_solve every policy_every iterations;
+    // iter_frac = iterations done / max_iters.  Set by the learned-rho module.
+    int (*rho_policy)(DevGraph* s, float iter_frac) = nullptr;
+    void* policy_ctx = nullptr;
+
     // kernel timing (paradmm_gpu_profile_kernels): eager launches only
     bool prof = false;
     std::vector<ProfRec> prof_recs;
@@ -87,11 +96,15 @@ void prof_begin(DevGraph* s, const char* name);
 void prof_end(DevGraph* s);
 
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
+// device state of G, finalised; with upload, host writes since are uploaded
+int pdm_get_state(graph* G, DevGraph** out, bool upload);
 int dmalloc(DevGraph* s, void** p, size_t bytes);
 int ensure_aux(DevGraph* s);
 int launch_mrecon(DevGraph* s);
 int launch_npass(DevGraph* s);
 int residual_reduce(DevGraph* s);
+// unit-square packing read from Z: out = {max overlap, max container violation, sum r}
+int square_quality(DevGraph* s, double out[3]);
 int warm_start_device(DevGraph* s, int maxiter);
 void free_dev(DevGraph* s);
 

@@ -1,0 +1,916 @@
+// mlp.hip -- the learned rho policy: a 6 -> 32 -> 32 -> 1 ReLU MLP whose
+// forward and backward passes run on bf16 MFMA (v_mfma_f32_16x16x16_bf16)
+// with fp32 accumulation, trained with Adam on fp32 master weights.
+//
+// No reference counterpart: the reference's rho hook (core/paradmm.c:564-566)
+// is empty.  Parameter vector (1313 fp32): W1[32][6], b1[32], W2[32][32],
+// b2[32], W3[32], b3.
+//
+// MFMA 16x16x16 operand layout (wave64, 4 elements per lane):
+//   A[i][k]: lane i + 16*(k/4), element k%4      (i row, k reduction)
+//   B[k][j]: lane j + 16*(k/4), element k%4      (j column)
+//   D[i][j]: lane j + 16*(i/4), element i%4
+// A wave pushes tiles of 16 rows through the network; activations move from
+// the D layout of one layer to the A layout of the next through LDS.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <string.h>
+
+#include <cmath>
+#include <algorithm>
+#include <random>
+#include <vector>
+
+#include "engine_internal.h"
+#include "engine_state.h"
+#include "mlp.h"
+#include "paradmm_gpu.h"
+
+#define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hip_fail(e_, #x, __FILE__, __LINE__); } while (0)
+#define RC(x) do { int r_ = (x); if (r_) return r_; } while (0)
+
+namespace pdm {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// round-to-nearest-even fp32 -> bf16 (NaN stays NaN)
+__device__ __forceinline__ short f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (short)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
+
+__device__ __forceinline__ v4f mfma16(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+// Weights staged in LDS by every block: bf16 operands and fp32 biases.
+struct MlpShared {
+    short w1[MLP_NH][16];  // K padded 6 -> 16
+    short w2[MLP_NH][MLP_NH];
+    short w3[MLP_NH];
+    float b1[MLP_NH], b2[MLP_NH], b3;
+};
+
+__device__ void stage_weights(const float* __restrict__ p, MlpShared& w)
+{
+    for (int q = threadIdx.x; q < MLP_NH * 16; q += blockDim.x) {
+        const int u = q / 16, k = q % 16;
+        w.w1[u][k] = k < MLP_NIN ? f2bf(p[MLP_OFF_W1 + u * MLP_NIN + k]) : (short)0;
+    }
+    for (int q = threadIdx.x; q < MLP_NH * MLP_NH; q += blockDim.x) w.w2[q / MLP_NH][q % MLP_NH] = f2bf(p[MLP_OFF_W2 + q]);
+    for (int q = threadIdx.x; q < MLP_NH; q += blockDim.x) {
+        w.w3[q] = f2bf(p[MLP_OFF_W3 + q]);
+        w.b1[q] = p[MLP_OFF_B1 + q];
+        w.b2[q] = p[MLP_OFF_B2 + q];
+    }
+    if (threadIdx.x == 0) w.b3 = p[MLP_OFF_B3];
+}
+
+// Per-wave scratch: one 16-row tile.
+struct TileScratch {
+    short f[16][16];     // features (bf16), K padded
+    short h1[16][MLP_NH];
+    short h2[MLP_NH > 16 ? 16 : 16][MLP_NH];
+    float y[16];
+    short dz[16][MLP_NH];  // back-propagated pre-activation gradient (bf16)
+};
+
+// Forward of one 16-row tile whose features are in ts.f.  Returns in z1, z2
+// (D layout, two 16-column tiles each) the pre-activations; ts.h1, ts.h2 get
+// the bf16 activations; ts.y the outputs (written by lanes with l%16 == 0).
+__device__ __forceinline__ void tile_forward(const MlpShared& w, TileScratch& ts, v4f (&z1)[2], v4f (&z2)[2])
+{
+    const int l = threadIdx.x & 63;
+    const int li = l & 15, lq = l >> 4;
+    // layer 1: A = features [16 x 16], B = W1^T
+    v4s a;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) a[e] = ts.f[li][4 * lq + e];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        v4s b;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b[e] = w.w1[nt * 16 + li][4 * lq + e];
+        z1[nt] = mfma16(a, b, v4f{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            z1[nt][e] += w.b1[nt * 16 + li];
+            ts.h1[4 * lq + e][nt * 16 + li] = f2bf(fmaxf(z1[nt][e], 0.0f));
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // layer 2: A = h1 [16 x 32] (two K steps), B = W2^T
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            v4s aa, bb;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                aa[e] = ts.h1[li][ks * 16 + 4 * lq + e];
+                bb[e] = w.w2[nt * 16 + li][ks * 16 + 4 * lq + e];
+            }
+            acc = mfma16(aa, bb, acc);
+        }
+        z2[nt] = acc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            z2[nt][e] += w.b2[nt * 16 + li];
+            ts.h2[4 * lq + e][nt * 16 + li] = f2bf(fmaxf(z2[nt][e], 0.0f));
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // layer 3: A = h2, B = W3^T in column 0
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        v4s aa, bb;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            aa[e] = ts.h2[li][ks * 16 + 4 * lq + e];
+            bb[e] = li == 0 ? w.w3[ks * 16 + 4 * lq + e] : (short)0;
+        }
+        acc = mfma16(aa, bb, acc);
+    }
+    if (li == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ts.y[4 * lq + e] = acc[e] + w.b3;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// y = MLP(features) for n rows of 6 fp32 features.
+__global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ feat, int n,
+                                                     const float* __restrict__ params, float* __restrict__ out)
+{
+    __shared__ MlpShared w;
+    __shared__ TileScratch tsw[4];
+    stage_weights(params, w);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    TileScratch& ts = tsw[wv];
+    const long ntiles = ((long)n + 15) / 16;
+    for (long tile = (long)blockIdx.x * 4 + wv; tile < ntiles; tile += (long)gridDim.x * 4) {
+        const long r0 = tile * 16;
+        for (int q = l; q < 256; q += 64) {
+            const int i = q / 16, k = q % 16;
+            const long row = r0 + i;
+            ts.f[i][k] = (k < MLP_NIN && row < n) ? f2bf(feat[row * MLP_NIN + k]) : (short)0;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        v4f z1[2], z2[2];
+        tile_forward(w, ts, z1, z2);
+        if (l < 16 && r0 + l < n) out[r0 + l] = ts.y[l];
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Partial gradient of 0.5 * sum((y - t)^2) / inv_b over this block's rows.
+// grad_part[blockIdx.x][MLP_NPARAMS].
+__global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ feat, const float* __restrict__ target,
+                                                  int n, float inv_b, const float* __restrict__ params,
+                                                  float* __restrict__ grad_part, double* __restrict__ loss_part)
+{
+    __shared__ MlpShared w;
+    __shared__ TileScratch tsw[4];
+    __shared__ float red[4][MLP_NPARAMS_PAD];
+    __shared__ double lred[4];
+    stage_weights(params, w);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int li = l & 15, lq = l >> 4;
+    TileScratch& ts = tsw[wv];
+    v4f gW2[2][2], gW1[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        gW1[a] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int b = 0; b < 2; ++b) gW2[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    float gb1[2] = {0.0f, 0.0f}, gb2[2] = {0.0f, 0.0f}, gW3[2] = {0.0f, 0.0f}, gb3 = 0.0f;
+    double loss = 0.0;
+    const long ntiles = ((long)n + 15) / 16;
+    for (long tile = (long)blockIdx.x * 4 + wv; tile < ntiles; tile += (long)gridDim.x * 4) {
+        const long r0 = tile * 16;
+        for (int q = l; q < 256; q += 64) {
+            const int i = q / 16, k = q % 16;
+            const long row = r0 + i;
+            ts.f[i][k] = (k < MLP_NIN && row < n) ? f2bf(feat[row * MLP_NIN + k]) : (short)0;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        v4f z1[2], z2[2];
+        tile_forward(w, ts, z1, z2);
+        // dy per row (zero for padding rows), broadcast through LDS
+        if (l < 16) {
+            const long row = r0 + l;
+            float dy = 0.0f;
+            if (row < n) {
+                const float d = ts.y[l] - target[row];
+                loss += 0.5 * (double)d * (double)d;
+                dy = d * inv_b;
+            }
+            ts.y[l] = dy;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        // dW3[u] += sum_rows dy h2[u]; db3 += sum dy.  Lane li owns u = nt*16 + li over
+        // rows 4*lq .. 4*lq+3.
+        if (l == 0)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) gb3 += ts.y[i];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int u = nt * 16 + li;
+            const float w3u = bf2f(w.w3[u]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = 4 * lq + e;
+                const float dy = ts.y[i];
+                gW3[nt] += dy * bf2f(ts.h2[i][u]);
+                const float dz = z2[nt][e] > 0.0f ? dy * w3u : 0.0f;
+                gb2[nt] += dz;
+                ts.dz[i][u] = f2bf(dz);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        // dW2[mt*16+i][nt*16+j] += sum_rows dz2[row][.] h1[row][.]  (K = the 16 rows)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            v4s aa;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) aa[e] = ts.dz[4 * lq + e][mt * 16 + li];
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                v4s bb;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) bb[e] = ts.h1[4 * lq + e][nt * 16 + li];
+                gW2[mt][nt] = mfma16(aa, bb, gW2[mt][nt]);
+            }
+        }
+        // dh1 = dz2 (rows x 32 out) * W2 (32 out x 32 in), then dz1
+        v4f dh1[2];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                v4s aa, bb;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    aa[e] = ts.dz[li][ks * 16 + 4 * lq + e];
+                    bb[e] = w.w2[ks * 16 + 4 * lq + e][nt * 16 + li];
+                }
+                acc = mfma16(aa, bb, acc);
+            }
+            dh1[nt] = acc;
+        }
+        __builtin_amdgcn_wave_barrier();  // all lanes have read dz2 before it is overwritten
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int u = nt * 16 + li;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float dz = z1[nt][e] > 0.0f ? dh1[nt][e] : 0.0f;
+                gb1[nt] += dz;
+                ts.dz[4 * lq + e][u] = f2bf(dz);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        // dW1[mt*16+i][k] += sum_rows dz1[row][.] f[row][k]
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            v4s aa, bb;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                aa[e] = ts.dz[4 * lq + e][mt * 16 + li];
+                bb[e] = ts.f[4 * lq + e][li];
+            }
+            gW1[mt] = mfma16(aa, bb, gW1[mt]);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // gather this wave's partials into red[wv][...] (parameter order)
+    float* g = red[wv];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int u = mt * 16 + 4 * lq + e, k = li;
+            if (k < MLP_NIN) g[MLP_OFF_W1 + u * MLP_NIN + k] = gW1[mt][e];
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) g[MLP_OFF_W2 + u * MLP_NH + nt * 16 + li] = gW2[mt][nt][e];
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        // sum the four lane groups (rows) of each column
+        float b1v = gb1[nt], b2v = gb2[nt], w3v = gW3[nt];
+        b1v += __shfl_xor(b1v, 16, 64); b1v += __shfl_xor(b1v, 32, 64);
+        b2v += __shfl_xor(b2v, 16, 64); b2v += __shfl_xor(b2v, 32, 64);
+        w3v += __shfl_xor(w3v, 16, 64); w3v += __shfl_xor(w3v, 32, 64);
+        if (lq == 0) {
+            g[MLP_OFF_B1 + nt * 16 + li] = b1v;
+            g[MLP_OFF_B2 + nt * 16 + li] = b2v;
+            g[MLP_OFF_W3 + nt * 16 + li] = w3v;
+        }
+    }
+    if (l == 0) g[MLP_OFF_B3] = gb3;
+    // loss: lanes 0..15 hold per-row terms
+    double lw = loss;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) lw += __shfl_xor(lw, o, 64);
+    if (l == 0) lred[wv] = lw;
+    __syncthreads();
+    for (int q = threadIdx.x; q < MLP_NPARAMS; q += blockDim.x)
+        grad_part[(long)blockIdx.x * MLP_NPARAMS + q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = (lred[0] + lred[1]) + (lred[2] + lred[3]);
+}
+
+// grad = sum over blocks (fixed order); loss likewise
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ part, int nblk, float* __restrict__ grad,
+                                                    const double* __restrict__ lpart, double* __restrict__ loss)
+{
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < MLP_NPARAMS; q += gridDim.x * 256) {
+        float s = 0.0f;
+        for (int b = 0; b < nblk; ++b) s += part[(long)b * MLP_NPARAMS + q];
+        grad[q] = s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += lpart[b];
+        *loss = s;
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the fp32 master weights
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                                                  float* __restrict__ v, float lr, float c1, float c2)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= MLP_NPARAMS) return;
+    const float gq = g[q];
+    const float mq = 0.9f * m[q] + 0.1f * gq;
+    const float vq = 0.999f * v[q] + 0.001f * gq * gq;
+    m[q] = mq;
+    v[q] = vq;
+    p[q] -= lr * (mq * c1) / (sqrtf(vq * c2) + 1e-8f);
+}
+
+// rows of a minibatch gathered by index
+__global__ __launch_bounds__(256) void k_mlp_gather(const float* __restrict__ feat, const float* __restrict__ target,
+                                                    const int* __restrict__ idx, int n, float* __restrict__ fo,
+                                                    float* __restrict__ to)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    const long s = idx[r];
+#pragma unroll
+    for (int k = 0; k < MLP_NIN; ++k) fo[(long)r * MLP_NIN + k] = feat[s * MLP_NIN + k];
+    to[r] = target[s];
+}
+
+// One MFMA on known operands: D = A x B with A[i][k] = bf16(i - k/4),
+// B[k][j] = bf16((k + 2 j) / 8); written row-major [16][16].
+__global__ void k_mfma_layout_probe(float* __restrict__ out)
+{
+    const int l = threadIdx.x & 63, li = l & 15, lq = l >> 4;
+    v4s a, b;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int k = 4 * lq + e;
+        a[e] = f2bf((float)li - (float)k * 0.25f);
+        b[e] = f2bf((float)(k + 2 * li) * 0.125f);
+    }
+    const v4f d = mfma16(a, b, v4f{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+    for (int e = 0; e < 4; ++e) out[(4 * lq + e) * 16 + li] = d[e];
+}
+
+}  // namespace pdm
+
+using namespace pdm;
+
+struct paradmm_rho_mlp {
+    float* d_p = nullptr;   // fp32 master parameters
+    float* d_m = nullptr;   // Adam moments
+    float* d_v = nullptr;
+    float* d_g = nullptr;   // reduced gradient
+    float* d_gpart = nullptr;
+    double* d_lpart = nullptr;
+    double* d_loss = nullptr;
+    int gblocks = 0;        // capacity of the partial buffers
+    int step = 0;
+    hipStream_t stream = nullptr;
+};
+
+static int mlp_fail(hipError_t e, const char* what)
+{
+    pdm_set_error(PARADMM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    return PARADMM_ERR_HIP;
+}
+
+#define MCK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return mlp_fail(e_, #x); } while (0)
+
+static int mlp_alloc(paradmm_rho_mlp* m)
+{
+    MCK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    MCK(hipMalloc(&m->d_p, sizeof(float) * MLP_NPARAMS));
+    MCK(hipMalloc(&m->d_m, sizeof(float) * MLP_NPARAMS));
+    MCK(hipMalloc(&m->d_v, sizeof(float) * MLP_NPARAMS));
+    MCK(hipMalloc(&m->d_g, sizeof(float) * MLP_NPARAMS));
+    MCK(hipMalloc(&m->d_loss, sizeof(double)));
+    MCK(hipMemset(m->d_m, 0, sizeof(float) * MLP_NPARAMS));
+    MCK(hipMemset(m->d_v, 0, sizeof(float) * MLP_NPARAMS));
+    return PARADMM_OK;
+}
+
+static int mlp_ensure_blocks(paradmm_rho_mlp* m, int nblk)
+{
+    if (nblk <= m->gblocks) return PARADMM_OK;
+    if (m->d_gpart) MCK(hipFree(m->d_gpart));
+    if (m->d_lpart) MCK(hipFree(m->d_lpart));
+    MCK(hipMalloc(&m->d_gpart, sizeof(float) * MLP_NPARAMS * (size_t)nblk));
+    MCK(hipMalloc(&m->d_lpart, sizeof(double) * (size_t)nblk));
+    m->gblocks = nblk;
+    return PARADMM_OK;
+}
+
+extern "C" void paradmm_rho_mlp_destroy(paradmm_rho_mlp* m)
+{
+    if (!m) return;
+    (void)hipFree(m->d_p); (void)hipFree(m->d_m); (void)hipFree(m->d_v); (void)hipFree(m->d_g);
+    (void)hipFree(m->d_gpart); (void)hipFree(m->d_lpart); (void)hipFree(m->d_loss);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+extern "C" int paradmm_rho_mlp_create(paradmm_rho_mlp** out, const float* params, int nparams, uint64_t seed)
+{
+    if (!out || (params && nparams != MLP_NPARAMS)) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_rho_mlp_create: need %d parameters", MLP_NPARAMS);
+        return PARADMM_ERR_ARG;
+    }
+    paradmm_rho_mlp* m = new paradmm_rho_mlp();
+    int rc = mlp_alloc(m);
+    if (rc) { paradmm_rho_mlp_destroy(m); return rc; }
+    std::vector<float> p(MLP_NPARAMS, 0.0f);
+    if (params) {
+        memcpy(p.data(), params, sizeof(float) * MLP_NPARAMS);
+    } else {
+        // He-normal weights, zero biases
+        std::mt19937_64 gen(seed);
+        std::normal_distribution<float> n1(0.0f, std::sqrt(2.0f / MLP_NIN)), n2(0.0f, std::sqrt(2.0f / MLP_NH)),
+            n3(0.0f, std::sqrt(1.0f / MLP_NH));
+        for (int q = 0; q < MLP_NH * MLP_NIN; ++q) p[MLP_OFF_W1 + q] = n1(gen);
+        for (int q = 0; q < MLP_NH * MLP_NH; ++q) p[MLP_OFF_W2 + q] = n2(gen);
+        for (int q = 0; q < MLP_NH; ++q) p[MLP_OFF_W3 + q] = n3(gen);
+    }
+    hipError_t e = hipMemcpy(m->d_p, p.data(), sizeof(float) * MLP_NPARAMS, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { paradmm_rho_mlp_destroy(m); return mlp_fail(e, "hipMemcpy"); }
+    *out = m;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_rho_mlp_get_params(paradmm_rho_mlp* m, float* params, int nparams)
+{
+    if (!m || !params || nparams != MLP_NPARAMS) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_rho_mlp_get_params: need %d floats", MLP_NPARAMS);
+        return PARADMM_ERR_ARG;
+    }
+    MCK(hipMemcpy(params, m->d_p, sizeof(float) * MLP_NPARAMS, hipMemcpyDeviceToHost));
+    return PARADMM_OK;
+}
+
+namespace pdm {
+const float* mlp_device_params(paradmm_rho_mlp* m) { return m ? m->d_p : nullptr; }
+}
+
+extern "C" int paradmm_rho_mlp_forward(paradmm_rho_mlp* m, const float* feat, int n, float* out)
+{
+    if (!m || !feat || !out || n < 1) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_rho_mlp_forward: bad arguments");
+        return PARADMM_ERR_ARG;
+    }
+    float *df = nullptr, *dy = nullptr;
+    MCK(hipMalloc(&df, sizeof(float) * MLP_NIN * (size_t)n));
+    hipError_t e = hipMalloc(&dy, sizeof(float) * (size_t)n);
+    if (e == hipSuccess) e = hipMemcpyAsync(df, feat, sizeof(float) * MLP_NIN * (size_t)n, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) {
+        const long ntiles = ((long)n + 15) / 16;
+        const unsigned grid = (unsigned)std::min<long>((ntiles + 3) / 4, 4096);
+        k_mlp_forward<<<grid, 256, 0, m->stream>>>(df, n, m->d_p, dy);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dy, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(df);
+    (void)hipFree(dy);
+    if (e != hipSuccess) return mlp_fail(e, "paradmm_rho_mlp_forward");
+    return PARADMM_OK;
+}
+
+// gradient of 0.5 * mean((y - t)^2) over rows already on the device
+static int device_gradient(paradmm_rho_mlp* m, const float* dfeat, const float* dtarget, int n)
+{
+    const long ntiles = ((long)n + 15) / 16;
+    const int nblk = (int)std::min<long>((ntiles + 3) / 4, 1024);
+    RC(mlp_ensure_blocks(m, nblk));
+    k_mlp_grad<<<nblk, 256, 0, m->stream>>>(dfeat, dtarget, n, 1.0f / (float)n, m->d_p, m->d_gpart, m->d_lpart);
+    MCK(hipGetLastError());
+    k_mlp_reduce<<<(MLP_NPARAMS + 255) / 256, 256, 0, m->stream>>>(m->d_gpart, nblk, m->d_g, m->d_lpart, m->d_loss);
+    MCK(hipGetLastError());
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_rho_mlp_gradient(paradmm_rho_mlp* m, const float* feat, const float* target, int n,
+                                        float* grad, double* loss)
+{
+    if (!m || !feat || !target || !grad || n < 1) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_rho_mlp_gradient: bad arguments");
+        return PARADMM_ERR_ARG;
+    }
+    float *df = nullptr, *dt = nullptr;
+    MCK(hipMalloc(&df, sizeof(float) * MLP_NIN * (size_t)n));
+    MCK(hipMalloc(&dt, sizeof(float) * (size_t)n));
+    int rc = PARADMM_OK;
+    hipError_t e = hipMemcpyAsync(df, feat, sizeof(float) * MLP_NIN * (size_t)n, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, target, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, m->stream);
+    if (e != hipSuccess) rc = mlp_fail(e, "hipMemcpy");
+    if (!rc) rc = device_gradient(m, df, dt, n);
+    double l = 0.0;
+    if (!rc) {
+        e = hipMemcpyAsync(grad, m->d_g, sizeof(float) * MLP_NPARAMS, hipMemcpyDeviceToHost, m->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&l, m->d_loss, sizeof(double), hipMemcpyDeviceToHost, m->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+        if (e != hipSuccess) rc = mlp_fail(e, "hipMemcpy");
+    }
+    if (loss) *loss = l / n;
+    (void)hipFree(df);
+    (void)hipFree(dt);
+    return rc;
+}
+
+extern "C" int paradmm_rho_mlp_train(paradmm_rho_mlp* m, const float* feat, const float* target, int n, int epochs,
+                                     int batch, float lr, uint64_t seed, double* final_loss)
+{
+    if (!m || !feat || !target || n < 1 || epochs < 1 || batch < 16) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_rho_mlp_train: bad arguments");
+        return PARADMM_ERR_ARG;
+    }
+    float *df = nullptr, *dt = nullptr, *bf = nullptr, *bt = nullptr;
+    int* didx = nullptr;
+    MCK(hipMalloc(&df, sizeof(float) * MLP_NIN * (size_t)n));
+    MCK(hipMalloc(&dt, sizeof(float) * (size_t)n));
+    MCK(hipMalloc(&bf, sizeof(float) * MLP_NIN * (size_t)batch));
+    MCK(hipMalloc(&bt, sizeof(float) * (size_t)batch));
+    MCK(hipMalloc(&didx, sizeof(int) * (size_t)n));
+    int rc = PARADMM_OK;
+    hipError_t e = hipMemcpyAsync(df, feat, sizeof(float) * MLP_NIN * (size_t)n, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, target, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, m->stream);
+    if (e != hipSuccess) rc = mlp_fail(e, "hipMemcpy");
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::mt19937_64 gen(seed);
+    double epoch_loss = 0.0;
+    for (int ep = 0; ep < epochs && !rc; ++ep) {
+        std::shuffle(idx.begin(), idx.end(), gen);
+        e = hipMemcpyAsync(didx, idx.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, m->stream);
+        if (e != hipSuccess) { rc = mlp_fail(e, "hipMemcpy"); break; }
+        e = hipStreamSynchronize(m->stream);  // idx is reused next epoch
+        if (e != hipSuccess) { rc = mlp_fail(e, "sync"); break; }
+        epoch_loss = 0.0;
+        for (int s0 = 0; s0 < n && !rc; s0 += batch) {
+            const int nb = std::min(batch, n - s0);
+            k_mlp_gather<<<(nb + 255) / 256, 256, 0, m->stream>>>(df, dt, didx + s0, nb, bf, bt);
+            if ((e = hipGetLastError()) != hipSuccess) { rc = mlp_fail(e, "gather"); break; }
+            rc = device_gradient(m, bf, bt, nb);
+            if (rc) break;
+            m->step += 1;
+            const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)m->step));
+            const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)m->step));
+            k_mlp_adam<<<(MLP_NPARAMS + 255) / 256, 256, 0, m->stream>>>(m->d_p, m->d_g, m->d_m, m->d_v, lr, c1, c2);
+            if ((e = hipGetLastError()) != hipSuccess) { rc = mlp_fail(e, "adam"); break; }
+            if (ep == epochs - 1) {
+                double l = 0.0;
+                e = hipMemcpyAsync(&l, m->d_loss, sizeof(double), hipMemcpyDeviceToHost, m->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+                if (e != hipSuccess) { rc = mlp_fail(e, "loss"); break; }
+                epoch_loss += l;
+            }
+        }
+    }
+    if (!rc) {
+        e = hipStreamSynchronize(m->stream);
+        if (e != hipSuccess) rc = mlp_fail(e, "sync");
+    }
+    if (final_loss) *final_loss = epoch_loss / n;
+    (void)hipFree(df); (void)hipFree(dt); (void)hipFree(bf); (void)hipFree(bt); (void)hipFree(didx);
+    return rc;
+}
+
+extern "C" int paradmm_rho_mlp_layout_probe(float* out256)
+{
+    if (!out256) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_rho_mlp_layout_probe: NULL");
+        return PARADMM_ERR_ARG;
+    }
+    float* d = nullptr;
+    MCK(hipMalloc(&d, sizeof(float) * 256));
+    k_mfma_layout_probe<<<1, 64>>>(d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(out256, d, sizeof(float) * 256, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return mlp_fail(e, "layout probe");
+    return PARADMM_OK;
+}
+
+// ------------------------------------------------------------------ rho policy on a graph
+
+namespace pdm {
+
+// Features of one edge (shared by the training-data extraction and the
+// in-solver policy): log10 of the primal residual |X_e - Z_v|, of the
+// rho-weighted dual residual rho_e |Z_v - Zprev_v| and of rho_e, scaled to
+// O(1); the factor type as two indicator slots (pair, radius; the container
+// is 0,0); the fraction of the iteration budget done.
+__device__ __forceinline__ void edge_features(float r, float s, float rho, int type, float frac, float (&f)[MLP_NIN])
+{
+    f[0] = log10f(fmaxf(r, 1e-10f)) * 0.2f;
+    f[1] = log10f(fmaxf(s, 1e-10f)) * 0.2f;
+    f[2] = log10f(fmaxf(rho, 1e-10f)) * 0.5f;
+    f[3] = type == PDM_EDGE_PAIR ? 1.0f : 0.0f;
+    f[4] = type == PDM_EDGE_RADIUS ? 1.0f : 0.0f;
+    f[5] = frac;
+}
+
+struct PolicyArgs {
+    int E, D;
+    long stride;
+    const int* pos;
+    const int* edge_var;
+    const unsigned char* type;
+    const float* X;
+    float* U;
+    float* rho;
+    const float* Z;
+    const float* Zprev;
+    float frac;
+    float* feat_out;  // features only (training data) when non-null
+};
+
+__device__ __forceinline__ void edge_residuals(const PolicyArgs& a, int e, float& r, float& s, float& rho, int& type,
+                                               int& p)
+{
+    p = a.pos[e];
+    const int v = a.edge_var[e];
+    float r2 = 0.0f, dz2 = 0.0f;
+    for (int k = 0; k < a.D; ++k) {
+        const float z = a.Z[v * a.D + k];
+        const float d = a.X[k * a.stride + p] - z;
+        const float q = z - a.Zprev[v * a.D + k];
+        r2 += d * d;
+        dz2 += q * q;
+    }
+    rho = a.rho[p];
+    r = sqrtf(r2);
+    s = rho * sqrtf(dz2);
+    type = a.type[e];
+}
+
+__global__ __launch_bounds__(256) void k_rho_features(PolicyArgs a)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.E) return;
+    float r, s, rho, f[MLP_NIN];
+    int type, p;
+    edge_residuals(a, e, r, s, rho, type, p);
+    edge_features(r, s, rho, type, a.frac, f);
+#pragma unroll
+    for (int k = 0; k < MLP_NIN; ++k) a.feat_out[(long)e * MLP_NIN + k] = f[k];
+}
+
+// rho_e <- 10^clamp(MLP(features_e)), U_e <- U_e rho_old / rho_new.  Each
+// wave takes 64 edges per step (one per lane) and pushes them through the
+// network as four 16-row MFMA tiles.
+__global__ __launch_bounds__(256) void k_rho_policy(PolicyArgs a, const float* __restrict__ params)
+{
+    __shared__ MlpShared w;
+    __shared__ TileScratch tsw[4];
+    __shared__ float yv[4][64];
+    stage_weights(params, w);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    TileScratch& ts = tsw[wv];
+    const long nchunks = ((long)a.E + 63) / 64;
+    for (long ch = (long)blockIdx.x * 4 + wv; ch < nchunks; ch += (long)gridDim.x * 4) {
+        const long e = ch * 64 + l;
+        const bool valid = e < a.E;
+        float r = 0.0f, s = 0.0f, rho = 1.0f, f[MLP_NIN];
+        int type = 0, p = 0;
+        if (valid) edge_residuals(a, (int)e, r, s, rho, type, p);
+        edge_features(r, s, rho, type, a.frac, f);
+        for (int t = 0; t < 4; ++t) {
+            // rows of tile t are lanes 16t .. 16t+15
+            if ((l >> 4) == t) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) ts.f[l & 15][k] = k < MLP_NIN ? f2bf(f[k]) : (short)0;
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            v4f z1[2], z2[2];
+            tile_forward(w, ts, z1, z2);
+            if (l < 16) yv[wv][16 * t + l] = ts.y[l];
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (valid) {
+            const float y = fminf(fmaxf(yv[wv][l], MLP_LOG10_RHO_MIN), MLP_LOG10_RHO_MAX);
+            const float rn = exp10f(y);
+            const float scale = rho / rn;
+            for (int k = 0; k < a.D; ++k) a.U[k * a.stride + p] *= scale;
+            a.rho[p] = rn;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// uniform rho multiplier with the scaled dual rescaled: rho *= f, U /= f
+__global__ __launch_bounds__(256) void k_scale_rho(long n, int D, long stride, float f, float* __restrict__ rho,
+                                                   float* __restrict__ U)
+{
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    rho[p] *= f;
+    const float inv = 1.0f / f;
+    for (int k = 0; k < D; ++k) U[k * stride + p] *= inv;
+}
+
+static PolicyArgs policy_args(DevGraph* s, float frac)
+{
+    PolicyArgs a;
+    a.E = s->E;
+    a.D = s->D;
+    a.stride = s->stride;
+    a.pos = s->d_pos;
+    a.edge_var = s->d_edge_var_api;
+    a.type = s->d_edge_type_api;
+    a.X = s->d_X;
+    a.U = s->d_U;
+    a.rho = s->d_rho;
+    a.Z = s->d_Z;
+    a.Zprev = s->d_Zprev;
+    a.frac = frac;
+    a.feat_out = nullptr;
+    return a;
+}
+
+// makes U current (dense layout) so it can be rescaled
+static int current_u(DevGraph* s)
+{
+    if (s->layout == 2) RC(dense_finish(s));
+    return PARADMM_OK;
+}
+
+static int apply_policy(DevGraph* s, float frac)
+{
+    paradmm_rho_mlp* m = (paradmm_rho_mlp*)s->policy_ctx;
+    if (!m) return PARADMM_OK;
+    RC(current_u(s));
+    PolicyArgs a = policy_args(s, frac);
+    const long nchunks = ((long)s->E + 63) / 64;
+    const unsigned grid = (unsigned)std::min<long>((nchunks + 3) / 4, 8192);
+    prof_begin(s, "rho_policy");
+    k_rho_policy<<<grid, 256, 0, s->stream>>>(a, mlp_device_params(m));
+    prof_end(s);
+    CK(hipGetLastError());
+    s->rho_device_written = true;
+    return PARADMM_OK;
+}
+
+}  // namespace pdm
+
+extern "C" int paradmm_gpu_set_rho_policy(graph* G, paradmm_rho_mlp* m)
+{
+    DevGraph* s;
+    RC(pdm_get_state(G, &s, true));
+    s->policy_ctx = m;
+    s->rho_policy = m ? &apply_policy : nullptr;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_gpu_apply_rho_policy(graph* G, paradmm_rho_mlp* m, float iter_frac)
+{
+    if (!m) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_gpu_apply_rho_policy: no network");
+        return PARADMM_ERR_ARG;
+    }
+    DevGraph* s;
+    RC(pdm_get_state(G, &s, true));
+    void* keep = s->policy_ctx;
+    s->policy_ctx = m;
+    int rc = apply_policy(s, iter_frac);
+    s->policy_ctx = keep;
+    RC(rc);
+    CK(hipStreamSynchronize(s->stream));
+    s->dev_newer = true;
+    s->m_valid = false;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_gpu_rho_features(graph* G, float iter_frac, float* feat)
+{
+    if (!feat) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_gpu_rho_features: NULL output");
+        return PARADMM_ERR_ARG;
+    }
+    DevGraph* s;
+    RC(pdm_get_state(G, &s, true));
+    const size_t n = (size_t)s->E * MLP_NIN;
+    float* d = nullptr;
+    CK(hipMalloc(&d, sizeof(float) * n));
+    PolicyArgs a = policy_args(s, iter_frac);
+    a.feat_out = d;
+    k_rho_features<<<(unsigned)((s->E + 255) / 256), 256, 0, s->stream>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(feat, d, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "paradmm_gpu_rho_features", __FILE__, __LINE__);
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_gpu_scale_rho(graph* G, double factor)
+{
+    if (!(factor > 0.0)) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_gpu_scale_rho: factor must be > 0");
+        return PARADMM_ERR_ARG;
+    }
+    DevGraph* s;
+    RC(pdm_get_state(G, &s, true));
+    RC(current_u(s));
+    k_scale_rho<<<(unsigned)((s->stride + 255) / 256), 256, 0, s->stream>>>(s->stride, s->D, s->stride, (float)factor,
+                                                                             s->d_rho, s->d_U);
+    CK(hipGetLastError());
+    CK(hipStreamSynchronize(s->stream));
+    s->rho_device_written = true;
+    s->dev_newer = true;
+    s->m_valid = false;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_gpu_state_save(graph* G)
+{
+    DevGraph* s;
+    RC(pdm_get_state(G, &s, true));
+    RC(current_u(s));
+    const size_t se = (size_t)s->D * s->stride, sv = (size_t)s->D * s->V;
+    const size_t total = 2 * se + s->stride + 2 * sv;
+    if (!s->d_ckpt) RC(dmalloc(s, (void**)&s->d_ckpt, sizeof(float) * total));
+    float* c = s->d_ckpt;
+    CK(hipMemcpyAsync(c, s->d_X, sizeof(float) * se, hipMemcpyDeviceToDevice, s->stream));
+    CK(hipMemcpyAsync(c + se, s->d_U, sizeof(float) * se, hipMemcpyDeviceToDevice, s->stream));
+    CK(hipMemcpyAsync(c + 2 * se, s->d_rho, sizeof(float) * s->stride, hipMemcpyDeviceToDevice, s->stream));
+    CK(hipMemcpyAsync(c + 2 * se + s->stride, s->d_Z, sizeof(float) * sv, hipMemcpyDeviceToDevice, s->stream));
+    CK(hipMemcpyAsync(c + 2 * se + s->stride + sv, s->d_Zprev, sizeof(float) * sv, hipMemcpyDeviceToDevice, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    s->ckpt_n_from_buffer = s->n_from_buffer;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_gpu_state_restore(graph* G)
+{
+    DevGraph* s;
+    RC(pdm_get_state(G, &s, false));
+    if (!s->d_ckpt) {
+        pdm_set_error(PARADMM_ERR_STATE, "no saved state");
+        return PARADMM_ERR_STATE;
+    }
+    const size_t se = (size_t)s->D * s->stride, sv = (size_t)s->D * s->V;
+    const float* c = s->d_ckpt;
+    CK(hipMemcpyAsync(s->d_X, c, sizeof(float) * se, hipMemcpyDeviceToDevice, s->stream));
+    CK(hipMemcpyAsync(s->d_U, c + se, sizeof(float) * se, hipMemcpyDeviceToDevice, s->stream));
+    CK(hipMemcpyAsync(s->d_rho, c + 2 * se, sizeof(float) * s->stride, hipMemcpyDeviceToDevice, s->stream));
+    CK(hipMemcpyAsync(s->d_Z, c + 2 * se + s->stride, sizeof(float) * sv, hipMemcpyDeviceToDevice, s->stream));
+    CK(hipMemcpyAsync(s->d_Zprev, c + 2 * se + s->stride + sv, sizeof(float) * sv, hipMemcpyDeviceToDevice, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    if (s->layout == 2) dense_reset(s);  // the saved U was current
+    s->n_from_buffer = s->ckpt_n_from_buffer;
+    s->host_newer = false;
+    s->dev_newer = true;
+    s->m_valid = false;
+    s->rho_device_written = true;
+    return PARADMM_OK;
+}

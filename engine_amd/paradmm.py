@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libparadmm.so")
 
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int)
+_FP = C.POINTER(C.c_float)
 
 PROX_FN = C.CFUNCTYPE(None, _DP, _DP, _DP, C.c_int, C.c_int, C.c_void_p)
 
@@ -41,6 +42,22 @@ class GraphStruct(C.Structure):
         ("Z", _DP),
         ("RHOS", _DP), ("ALPHAS", _DP),
     ]
+
+
+class SolveOpts(C.Structure):
+    """paradmm_solve_opts of paradmm_gpu.h"""
+    _fields_ = [("max_iters", C.c_int), ("check_every", C.c_int), ("tol_primal", C.c_double),
+                ("tol_dual", C.c_double), ("tol_feas", C.c_double), ("policy_every", C.c_int)]
+
+
+class SolveResult(C.Structure):
+    """paradmm_solve_result of paradmm_gpu.h"""
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("primal_rms", C.c_double),
+                ("dual_rms", C.c_double), ("max_overlap", C.c_double), ("max_box", C.c_double),
+                ("objective", C.c_double), ("elapsed_ms", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class ParadmmError(RuntimeError):
@@ -94,6 +111,22 @@ def load():
         "paradmm_gpu_residuals": (C.c_int, [G, _DP, _DP]),
         "paradmm_gpu_time_iterations": (C.c_int, [G, C.c_int, _DP]),
         "paradmm_gpu_profile_kernels": (C.c_int, [G, C.c_int, C.c_int, C.c_char_p, _DP, _IP, _IP]),
+        "paradmm_gpu_copy_bandwidth": (C.c_int, [C.c_size_t, C.c_int, _DP]),
+        "paradmm_gpu_solve": (C.c_int, [G, C.POINTER(SolveOpts), C.POINTER(SolveResult)]),
+        "paradmm_gpu_square_quality": (C.c_int, [G, _DP, _DP, _DP]),
+        "paradmm_rho_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), _FP, C.c_int, C.c_uint64]),
+        "paradmm_rho_mlp_destroy": (None, [V]),
+        "paradmm_rho_mlp_get_params": (C.c_int, [V, _FP, C.c_int]),
+        "paradmm_rho_mlp_forward": (C.c_int, [V, _FP, C.c_int, _FP]),
+        "paradmm_rho_mlp_gradient": (C.c_int, [V, _FP, _FP, C.c_int, _FP, _DP]),
+        "paradmm_rho_mlp_train": (C.c_int, [V, _FP, _FP, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, _DP]),
+        "paradmm_rho_mlp_layout_probe": (C.c_int, [_FP]),
+        "paradmm_gpu_rho_features": (C.c_int, [G, C.c_float, _FP]),
+        "paradmm_gpu_set_rho_policy": (C.c_int, [G, V]),
+        "paradmm_gpu_apply_rho_policy": (C.c_int, [G, V, C.c_float]),
+        "paradmm_gpu_scale_rho": (C.c_int, [G, C.c_double]),
+        "paradmm_gpu_state_save": (C.c_int, [G]),
+        "paradmm_gpu_state_restore": (C.c_int, [G]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -118,6 +151,78 @@ def prox_address(name):
     """Address of one of the library's built-in prox operators (a device kind token)."""
     lib = load()
     return C.cast(getattr(lib, name), C.c_void_p).value
+
+
+def copy_bandwidth(nbytes=1 << 30, iters=20):
+    """GB/s of a plain device copy (read + write counted)."""
+    gbs = C.c_double()
+    check(load().paradmm_gpu_copy_bandwidth(nbytes, iters, C.byref(gbs)), "paradmm_gpu_copy_bandwidth")
+    return gbs.value
+
+
+def _fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_FP)
+
+
+class RhoMLP:
+    """The learned rho policy network (paradmm_rho_mlp of paradmm_gpu.h)."""
+    NPARAMS = 1313
+
+    def __init__(self, params=None, seed=0):
+        self.lib = load()
+        h = C.c_void_p()
+        if params is None:
+            check(self.lib.paradmm_rho_mlp_create(C.byref(h), None, 0, seed), "paradmm_rho_mlp_create")
+        else:
+            p = np.ascontiguousarray(params, dtype=np.float32)
+            check(self.lib.paradmm_rho_mlp_create(C.byref(h), _fptr(p), p.size, seed), "paradmm_rho_mlp_create")
+        self.h = h
+
+    def params(self):
+        p = np.zeros(self.NPARAMS, dtype=np.float32)
+        check(self.lib.paradmm_rho_mlp_get_params(self.h, _fptr(p), p.size), "get_params")
+        return p
+
+    def forward(self, feat):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        y = np.zeros(f.shape[0], dtype=np.float32)
+        check(self.lib.paradmm_rho_mlp_forward(self.h, _fptr(f), f.shape[0], _fptr(y)), "forward")
+        return y
+
+    def gradient(self, feat, target):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(self.NPARAMS, dtype=np.float32)
+        loss = C.c_double()
+        check(self.lib.paradmm_rho_mlp_gradient(self.h, _fptr(f), _fptr(t), f.shape[0], _fptr(g), C.byref(loss)),
+              "gradient")
+        return loss.value, g
+
+    def train(self, feat, target, epochs, batch=4096, lr=1e-3, seed=0):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        loss = C.c_double()
+        check(self.lib.paradmm_rho_mlp_train(self.h, _fptr(f), _fptr(t), f.shape[0], epochs, batch, lr, seed,
+                                             C.byref(loss)), "train")
+        return loss.value
+
+    def close(self):
+        if self.h:
+            self.lib.paradmm_rho_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mfma_layout_probe():
+    out = np.zeros(256, dtype=np.float32)
+    check(load().paradmm_rho_mlp_layout_probe(_fptr(out)), "layout probe")
+    return out.reshape(16, 16)
 
 
 def check(rc, what=""):
@@ -220,6 +325,39 @@ class Graph:
         ms = C.c_double()
         check(self.lib.paradmm_gpu_time_iterations(self.ptr, iters, C.byref(ms)), "paradmm_gpu_time_iterations")
         return ms.value
+
+    def solve(self, max_iters, tol=1e-4, tol_feas=1e-4, check_every=50, policy_every=0):
+        """Iterates to tolerance on the GPU; returns the paradmm_solve_result as a dict."""
+        o = SolveOpts(max_iters, check_every, tol, tol, tol_feas, policy_every)
+        r = SolveResult()
+        check(self.lib.paradmm_gpu_solve(self.ptr, C.byref(o), C.byref(r)), "paradmm_gpu_solve")
+        return r.as_dict()
+
+    def rho_features(self, iter_frac=0.0):
+        f = np.zeros((self.nedge, 6), dtype=np.float32)
+        check(self.lib.paradmm_gpu_rho_features(self.ptr, iter_frac, _fptr(f)), "rho_features")
+        return f
+
+    def set_rho_policy(self, mlp):
+        check(self.lib.paradmm_gpu_set_rho_policy(self.ptr, mlp.h if mlp is not None else None), "set_rho_policy")
+
+    def apply_rho_policy(self, mlp, iter_frac):
+        check(self.lib.paradmm_gpu_apply_rho_policy(self.ptr, mlp.h, iter_frac), "apply_rho_policy")
+
+    def scale_rho(self, factor):
+        check(self.lib.paradmm_gpu_scale_rho(self.ptr, factor), "scale_rho")
+
+    def state_save(self):
+        check(self.lib.paradmm_gpu_state_save(self.ptr), "state_save")
+
+    def state_restore(self):
+        check(self.lib.paradmm_gpu_state_restore(self.ptr), "state_restore")
+
+    def square_quality(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(self.lib.paradmm_gpu_square_quality(self.ptr, C.byref(a), C.byref(b), C.byref(c)),
+              "paradmm_gpu_square_quality")
+        return a.value, b.value, c.value
 
     def profile_kernels(self, iters, max_kernels=16):
         """{kernel name: (mean ms, launches)} over iters eagerly launched iterations."""

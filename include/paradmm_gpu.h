@@ -105,6 +105,82 @@ int paradmm_gpu_time_iterations(graph *G, int iters, double *ms);
  * max_kernels NUL-terminated names of 64 bytes each. */
 int paradmm_gpu_profile_kernels(graph *G, int iters, int max_kernels, char *names, double *avg_ms,
                                 int *count, int *nkernels);
+/* ---- solving to tolerance (NEW: the reference runs a fixed iteration count,
+ * packing/host.c:237-244, and has no stopping test) ---------------------------
+ * Iterates in chunks of check_every, after each chunk computes the FP64
+ * residuals as RMS values, sqrt(primal2 / (num_dims * num_edges)) and
+ * sqrt(dual2 / (num_dims * num_edges)), and stops when both are <= their
+ * tolerance and, if tol_feas >= 0, the unit-square packing read from Z has
+ * max pair overlap and max container violation <= tol_feas.  A rho policy
+ * attached with paradmm_gpu_set_rho_policy runs every policy_every
+ * iterations (0: fixed rho).  elapsed_ms is host wall time of the whole solve
+ * including the checks. */
+typedef struct paradmm_solve_opts {
+    int max_iters;
+    int check_every;
+    double tol_primal;
+    double tol_dual;
+    double tol_feas;
+    int policy_every;
+} paradmm_solve_opts;
+
+typedef struct paradmm_solve_result {
+    int iterations;
+    int converged;
+    double primal_rms;
+    double dual_rms;
+    double max_overlap;   /* < 0 when not evaluated */
+    double max_box;
+    double objective;     /* sum of radii */
+    double elapsed_ms;
+} paradmm_solve_result;
+
+int paradmm_gpu_solve(graph *G, const paradmm_solve_opts *opts, paradmm_solve_result *res);
+/* Feasibility and objective of a unit-square packing (num_dims 3, Z rows are
+ * circles): max pair overlap, max container violation, sum of radii. */
+int paradmm_gpu_square_quality(graph *G, double *max_overlap, double *max_box, double *sum_r);
+
+/* ---- learned rho (NEW: the reference's hook update_outgoing_rhos,
+ * core/paradmm.c:564-566, is empty) ---------------------------------------
+ * A 6 -> 32 -> 32 -> 1 ReLU MLP maps per-edge features to log10(rho_e);
+ * forward and backward run on bf16 MFMA with fp32 accumulation, training is
+ * Adam on fp32 master weights.  Features of edge e (variable v):
+ *   log10 |X_e - Z_v| / 5, log10(rho_e |Z_v - Zprev_v|) / 5, log10(rho_e) / 2,
+ *   [factor is a pair], [factor is a radius], iteration fraction.
+ * Applying the policy sets rho_e = 10^clamp(y, -1, 4) and rescales the scaled
+ * dual U_e by rho_old / rho_new.  Parameters: 1313 fp32 in the order
+ * W1[32][6], b1[32], W2[32][32], b2[32], W3[32], b3. */
+typedef struct paradmm_rho_mlp paradmm_rho_mlp;
+/* params NULL: He-normal init from seed; else nparams must be 1313 */
+int paradmm_rho_mlp_create(paradmm_rho_mlp **out, const float *params, int nparams, uint64_t seed);
+void paradmm_rho_mlp_destroy(paradmm_rho_mlp *m);
+int paradmm_rho_mlp_get_params(paradmm_rho_mlp *m, float *params, int nparams);
+/* y[i] = MLP(feat[i][0..5]) for n rows */
+int paradmm_rho_mlp_forward(paradmm_rho_mlp *m, const float *feat, int n, float *y);
+/* gradient of 0.5 mean((y - target)^2) over the n rows (one batch) and the loss */
+int paradmm_rho_mlp_gradient(paradmm_rho_mlp *m, const float *feat, const float *target, int n, float *grad,
+                             double *loss);
+/* epochs of shuffled minibatch Adam; final_loss = mean loss of the last epoch */
+int paradmm_rho_mlp_train(paradmm_rho_mlp *m, const float *feat, const float *target, int n, int epochs, int batch,
+                          float lr, uint64_t seed, double *final_loss);
+/* one 16x16x16 bf16 MFMA on fixed operands, D written row-major (layout check) */
+int paradmm_rho_mlp_layout_probe(float *out256);
+/* per-edge features of the current device state, num_edges x 6 in api edge order */
+int paradmm_gpu_rho_features(graph *G, float iter_frac, float *feat);
+/* attach (m != NULL) or detach the policy used by paradmm_gpu_solve */
+int paradmm_gpu_set_rho_policy(graph *G, paradmm_rho_mlp *m);
+/* apply the policy once now */
+int paradmm_gpu_apply_rho_policy(graph *G, paradmm_rho_mlp *m, float iter_frac);
+/* rho_e *= factor on every edge, U_e /= factor (uniform change of rho) */
+int paradmm_gpu_scale_rho(graph *G, double factor);
+/* save / restore the device iterate (X, U, rho, Z, Zprev): counterfactual rollouts */
+int paradmm_gpu_state_save(graph *G);
+int paradmm_gpu_state_restore(graph *G);
+
+/* NEW: bandwidth of a plain device-to-device copy of `bytes` (read + write
+ * counted), iters launches timed with HIP events: the measured HBM ceiling
+ * reported next to the nominal peak. */
+int paradmm_gpu_copy_bandwidth(size_t bytes, int iters, double *gbs);
 
 #ifdef __cplusplus
 }
