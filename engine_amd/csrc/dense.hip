@@ -81,8 +81,30 @@ struct DArgs {
     const int2* tiles;
 };
 
-__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
-__device__ __forceinline__ void st4(float* p, const float4& v) { *reinterpret_cast<float4*>(p) = v; }
+// Edge streams are read and written once per iteration; PDM_DENSE_NT=1 marks
+// those accesses non-temporal.
+#ifndef PDM_DENSE_NT
+#define PDM_DENSE_NT 0
+#endif
+typedef float nf4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld4(const float* p)
+{
+#if PDM_DENSE_NT
+    const nf4 v = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+#else
+    return *reinterpret_cast<const float4*>(p);
+#endif
+}
+__device__ __forceinline__ void st4(float* p, const float4& v)
+{
+#if PDM_DENSE_NT
+    const nf4 w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<nf4*>(p));
+#else
+    *reinterpret_cast<float4*>(p) = v;
+#endif
+}
 __device__ __forceinline__ float& cmp(float4& v, int c) { return (&v.x)[c]; }
 
 __device__ __forceinline__ float xsum16(float v)
@@ -103,11 +125,14 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
     constexpr bool NBUF = (MODE == DM_FUSED_NBUF || MODE == DM_LEGACY_X);
     constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z);
     constexpr bool PART_M = (MODE == DM_LEGACY_Z);
-    constexpr bool NEED_X = APPLY_U;
-    constexpr bool NEED_U = APPLY_U || (PROX && !NBUF) || (PART && !PART_M);
-    constexpr bool NEED_A = APPLY_U;
     constexpr bool WRITE_U = APPLY_U;
     constexpr bool WRITE_X = PROX;
+    // a diagonal tile's vector stores also cover its unary slots (processed
+    // afterwards by diag_unary, which reads their old X) and its padding:
+    // X is loaded there so masked slots are written back unchanged
+    constexpr bool NEED_X = APPLY_U || (VAR == 2 && WRITE_X);
+    constexpr bool NEED_U = APPLY_U || (PROX && !NBUF) || (PART && !PART_M);
+    constexpr bool NEED_A = APPLY_U;
 
     const int N = a.N;
     const long S = a.stride;
@@ -212,7 +237,15 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                if constexpr (WRITE_X) { cmp(xa[k], c) = xA[k]; cmp(xb[k], c) = xB[k]; }
+                if constexpr (WRITE_X) {
+                    if constexpr (VAR == 2) {  // masked slots keep their X
+                        cmp(xa[k], c) = vm ? xA[k] : cmp(xa[k], c);
+                        cmp(xb[k], c) = vm ? xB[k] : cmp(xb[k], c);
+                    } else {
+                        cmp(xa[k], c) = xA[k];
+                        cmp(xb[k], c) = xB[k];
+                    }
+                }
                 if constexpr (WRITE_U) { cmp(ua[k], c) = uA[k]; cmp(ub[k], c) = uB[k]; }
             }
         }

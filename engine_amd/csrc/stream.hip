@@ -8,10 +8,25 @@
 
 namespace {
 
-__global__ __launch_bounds__(256) void k_copy_stream(const float4* __restrict__ src, float4* __restrict__ dst, long n4)
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// four 16-byte loads in flight per thread, non-temporal (streamed once)
+__global__ __launch_bounds__(256) void k_copy_stream(const f4* __restrict__ src, f4* __restrict__ dst, long n4)
 {
-    const long stride = (long)gridDim.x * 256;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+    const long stride = (long)gridDim.x * 1024;
+    for (long i = (long)blockIdx.x * 1024 + threadIdx.x; i < n4; i += stride) {
+        f4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long j = i + 256 * u;
+            if (j < n4) v[u] = __builtin_nontemporal_load(src + j);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long j = i + 256 * u;
+            if (j < n4) __builtin_nontemporal_store(v[u], dst + j);
+        }
+    }
 }
 
 int fail(hipError_t e, const char* what)
@@ -29,7 +44,7 @@ extern "C" int paradmm_gpu_copy_bandwidth(size_t bytes, int iters, double* gbs)
         return PARADMM_ERR_ARG;
     }
     const long n4 = (long)(bytes / 16);
-    float4 *a = nullptr, *b = nullptr;
+    f4 *a = nullptr, *b = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = PARADMM_OK;
@@ -40,7 +55,7 @@ extern "C" int paradmm_gpu_copy_bandwidth(size_t bytes, int iters, double* gbs)
         if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) { rc = fail(e, "stream"); break; }
         if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) { rc = fail(e, "event"); break; }
         if ((e = hipMemsetAsync(a, 0, n4 * 16, st)) != hipSuccess) { rc = fail(e, "memset"); break; }
-        const unsigned grid = 256 * 32;  // 32 workgroups per CU, grid-stride
+        const unsigned grid = 256 * 8;  // 8 workgroups per CU, grid-stride
         k_copy_stream<<<grid, 256, 0, st>>>(a, b, n4);  // warm-up
         if ((e = hipEventRecord(e0, st)) != hipSuccess) { rc = fail(e, "record"); break; }
         for (int i = 0; i < iters; ++i) k_copy_stream<<<grid, 256, 0, st>>>(a, b, n4);

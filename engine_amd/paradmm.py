@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libparadmm.so")
+LIB_PATH = os.environ.get("PARADMM_LIB") or os.path.join(HERE, "libparadmm.so")
 
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int)
