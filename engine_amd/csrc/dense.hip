@@ -43,7 +43,7 @@ constexpr int DTT = DT * DT;  // edge slots per sub-block
 
 // minimum waves per SIMD requested for the tile kernel (register budget)
 #ifndef PDM_PAIRTILE_WAVES
-#define PDM_PAIRTILE_WAVES 3
+#define PDM_PAIRTILE_WAVES 2
 #endif
 
 enum {
@@ -84,7 +84,7 @@ struct DArgs {
 // Edge streams are read and written once per iteration; PDM_DENSE_NT=1 marks
 // those accesses non-temporal.
 #ifndef PDM_DENSE_NT
-#define PDM_DENSE_NT 0
+#define PDM_DENSE_NT 1
 #endif
 typedef float nf4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 ld4(const float* p)
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDM_PAIRTIL
     constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z);
     __shared__ float srow[5][DT];
     __shared__ float scol[4][5][DT];
-    __shared__ double sred[256];
+    __shared__ double sred[4];
 
     const int t = blockIdx.x;
     const int2 ij = a.tiles[t];
@@ -372,7 +372,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDM_PAIRTIL
         pair_tile<MODE, 0>(a, t, I, J, srow, scol, pr);
     }
 
-    if constexpr (APPLY_U) sred[tid] = pr;
+    if constexpr (APPLY_U) {
+        // FP64 primal partial: wave butterfly, then the four waves in order
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) pr += __shfl_xor(pr, o, 64);
+        if ((tid & 63) == 0) sred[tid >> 6] = pr;
+    }
     __syncthreads();
     if constexpr (PART) {
         const int N = a.N, nb = a.nb;
@@ -399,11 +404,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDM_PAIRTIL
         }
     }
     if constexpr (APPLY_U) {
-        for (int s = 128; s > 0; s >>= 1) {
-            if (tid < s) sred[tid] += sred[tid + s];
-            __syncthreads();
-        }
-        if (tid == 0) a.Rp[t] = sred[0];
+        if (tid == 0) a.Rp[t] = (sred[0] + sred[1]) + (sred[2] + sred[3]);
     }
 }
 
