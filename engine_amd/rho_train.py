@@ -1,21 +1,23 @@
 """Training and evaluation of the learned rho policy.
 
-The data comes from the plain solver itself.  Along a solver trajectory the
-state is checkpointed every `window` iterations; from each checkpoint the
-solver is rolled forward `window` iterations once per candidate uniform rho
-multiplier (0.25x .. 4x, with the scaled dual rescaled) and the multiplier
-whose rollout ends with the lowest score
+Training data come from the plain solver itself:
 
-    score = mean over the window's two checks of log10(max(primal, dual))
-            + 0.5 * log10(max(max pair overlap, 1e-6))
+1. Teacher.  For each training instance (unit-square packing, N circles,
+   seed) the plain fixed-rho solver is run to tolerance (primal and dual RMS
+   residual <= tol, max overlap and container violation <= tol) for every rho
+   of a grid; the teacher's rho* is the fastest to converge (ties and
+   near-ties within 10% broken by the larger objective sum r); if none
+   converges, the rho with the smallest final max(primal, dual, overlap).
+2. Rows.  From several starting rho (1, 10, rho* and a large one) the
+   solver runs `horizon` iterations; every `every` iterations the per-edge
+   features of paradmm_gpu_rho_features are sampled, each with target
+   log10(rho*).  The network thus learns which rho to move to from the
+   dynamics it observes, wherever it starts.
+3. Training with the library's own bf16-MFMA forward/backward and Adam.
 
-is the label: the target of every edge at that checkpoint is
-log10(rho_e * best multiplier).  The trajectory then continues from the
-checkpoint with the best multiplier (a greedy lookahead teacher).  The
-network (6 -> 32 -> 32 -> 1, bf16 MFMA) is trained on (edge features,
-target) rows with the library's own GPU Adam kernels.
-
-Everything runs through libparadmm's C ABI (paradmm_gpu.h).
+Applied inside paradmm_gpu_solve every `policy_every` iterations, the
+network sets per-edge rho; the plain fixed-rho solver stays the fallback and
+its feasibility and objective judge every learned result.
 """
 import json
 import os
@@ -27,54 +29,60 @@ from . import paradmm as P
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "models", "rho_policy.f32")
-MULTIPLIERS = (0.25, 0.5, 1.0, 2.0, 4.0)
+RHO_GRID = (20.0, 35.0, 50.0, 70.0, 100.0, 150.0, 200.0, 300.0)
 
 
-def _score(g, iters):
-    half = iters // 2
-    logs = []
-    for n in (half, iters - half):
-        r = g.solve(n, tol=0.0, tol_feas=-1.0, check_every=n)
-        logs.append(np.log10(max(r["primal_rms"], r["dual_rms"], 1e-12)))
-    ov, _, _ = g.square_quality()
-    return float(np.mean(logs)) + 0.5 * float(np.log10(max(ov, 1e-6)))
+def solve_once(n, seed, rho, policy=None, max_iters=20000, tol=1e-4, check_every=50, policy_every=50):
+    """One solve to tolerance; policy None is the plain fixed-rho method."""
+    g = P.packing_square(n, seed, rho=rho, alpha=1.0)
+    g.prepare()
+    if policy is not None:
+        g.set_rho_policy(policy)
+    r = g.solve(max_iters, tol=tol, tol_feas=tol, check_every=check_every,
+                policy_every=policy_every if policy is not None else 0)
+    g.close()
+    return r
 
 
-def generate(sizes, seeds, rho0s, snapshots=16, window=250, rows_per_snapshot=2000, rng_seed=0, log=print):
-    """Rows of (features, target) from greedy counterfactual rollouts."""
+def best_fixed_rho(n, seed, grid=RHO_GRID, max_iters=20000, tol=1e-4):
+    """The teacher: fastest converging fixed rho of the grid for this instance."""
+    runs = {rho: solve_once(n, seed, rho, None, max_iters=max_iters, tol=tol) for rho in grid}
+    conv = {rho: r for rho, r in runs.items() if r["converged"]}
+    if conv:
+        fastest = min(r["iterations"] for r in conv.values())
+        near = {rho: r for rho, r in conv.items() if r["iterations"] <= 1.1 * fastest}
+        best = max(near, key=lambda rho: near[rho]["objective"])
+    else:
+        best = min(runs, key=lambda rho: max(runs[rho]["primal_rms"], runs[rho]["dual_rms"],
+                                              max(runs[rho]["max_overlap"], 0.0)))
+    return best, runs
+
+
+def generate(instances, starts=(1.0, 10.0, None, 400.0), horizon=1000, every=50, max_iters=20000,
+             rows_per_snapshot=1000, rng_seed=0, log=print):
+    """Rows of (features, target) for the given (n, seed) instances."""
     rng = np.random.default_rng(rng_seed)
-    feats, targets = [], []
-    total = snapshots * window
-    for n in sizes:
-        for seed in seeds:
-            rho0 = float(rho0s[rng.integers(len(rho0s))])
+    feats, targets, teacher = [], [], {}
+    for n, seed in instances:
+        t0 = time.time()
+        rho_star, runs = best_fixed_rho(n, seed, max_iters=max_iters)
+        teacher[f"{n}/{seed}"] = {"rho_star": rho_star,
+                                  "iterations": {str(k): (v["iterations"] if v["converged"] else None)
+                                                 for k, v in runs.items()}}
+        for start in starts:
+            rho0 = rho_star if start is None else start
             g = P.packing_square(int(n), int(seed), rho=rho0, alpha=1.0)
             g.prepare()
-            rho = rho0
-            t0 = time.time()
-            for s in range(snapshots):
-                it = s * window
-                f = g.rho_features(it / total)
-                g.state_save()
-                scores = []
-                for m in MULTIPLIERS:
-                    g.state_restore()
-                    if m != 1.0:
-                        g.scale_rho(m)
-                    scores.append(_score(g, window))
-                best = MULTIPLIERS[int(np.argmin(scores))]
+            for it in range(every, horizon + 1, every):
+                g.iterate(every)
+                f = g.rho_features(it / max_iters)
                 take = rng.choice(f.shape[0], size=min(rows_per_snapshot, f.shape[0]), replace=False)
                 feats.append(f[take])
-                # rho is uniform along these trajectories: feature 2 is log10(rho)/2
-                targets.append((f[take, 2] * 2.0 + np.log10(best)).astype(np.float32))
-                g.state_restore()
-                if best != 1.0:
-                    g.scale_rho(best)
-                rho *= best
-                g.iterate(window)
-            log(f"# data N={n} seed={seed} rho0={rho0:g}: final rho {rho:g}, {time.time() - t0:.1f} s")
+                targets.append(np.full(take.size, np.log10(rho_star), dtype=np.float32))
             g.close()
-    return np.concatenate(feats).astype(np.float32), np.concatenate(targets).astype(np.float32)
+        log(f"# teacher N={n} seed={seed}: rho*={rho_star:g} "
+            f"({teacher[f'{n}/{seed}']['iterations']}), {time.time() - t0:.1f} s")
+    return np.concatenate(feats).astype(np.float32), np.concatenate(targets).astype(np.float32), teacher
 
 
 def train(feat, target, epochs=30, batch=4096, lr=2e-3, seed=0):
@@ -98,13 +106,16 @@ def load_policy(path=DEFAULT_WEIGHTS):
     return P.RhoMLP(params=p)
 
 
-def solve_once(n, seed, rho, policy=None, max_iters=20000, tol=1e-4, check_every=50, policy_every=50):
-    """One solve to tolerance; policy None is the plain fixed-rho method."""
-    g = P.packing_square(n, seed, rho=rho, alpha=1.0)
-    g.prepare()
-    if policy is not None:
-        g.set_rho_policy(policy)
-    r = g.solve(max_iters, tol=tol, tol_feas=tol, check_every=check_every,
-                policy_every=policy_every if policy is not None else 0)
-    g.close()
-    return r
+def compare(n, seeds, policy, fixed=(1.0, 50.0, 100.0), max_iters=20000, log=print):
+    """Time to tolerance of fixed rho values and of the learned policy (from rho = 1)."""
+    out = []
+    for seed in seeds:
+        row = {"n": n, "seed": seed}
+        for rho in fixed:
+            row[f"fixed{rho:g}"] = solve_once(n, seed, rho, None, max_iters=max_iters)
+        row["learned"] = solve_once(n, seed, 1.0, policy, max_iters=max_iters)
+        out.append(row)
+        log(f"# N={n} seed={seed}: " + ", ".join(
+            f"{k} {'%d it %.0f ms' % (v['iterations'], v['elapsed_ms']) if v['converged'] else 'not converged'}"
+            f" obj {v['objective']:.4f}" for k, v in row.items() if isinstance(v, dict)))
+    return out

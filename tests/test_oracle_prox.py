@@ -226,3 +226,15 @@ def test_splitmix64_known_values(oracle):
     s = C.c_uint64(0)
     out = [lib.or_splitmix64(C.byref(s)) for _ in range(3)]
     assert out == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_pair_coincident_centres_separate_along_x(oracle):
+    """Coincident centres (the reference divides 0/0, packing/host.c:123): the
+    pair separates along +x and ends touching, weights as usual."""
+    n = np.array([0.3, 0.4, 0.1, 0.3, 0.4, 0.2])
+    x = prox(oracle, "or_prox_pair3", n, [2.0, 6.0], 2, 3)
+    assert np.all(np.isfinite(x))
+    assert x[1] == 0.4 and x[4] == 0.4               # no y motion
+    assert x[0] < 0.3 < x[3]                          # circle 1 to -x, circle 2 to +x
+    assert x[3] - x[0] == pytest.approx(x[2] + x[5], abs=1e-12)
+    assert (0.3 - x[0]) * 2.0 == pytest.approx((x[3] - 0.3) * 6.0, rel=1e-12)
