@@ -29,7 +29,9 @@ __device__ __forceinline__ float max_alpha_radius(float n, float rho, float a)
 #pragma unroll 1
     for (int it = 0; it < 10; ++it) {
         const float f = -pow_exp(r, a - 1.0f) + (rho / a) * (r - n);
-        const float fp = -(a - 1.0f) * pow_exp(r, a - 2.0f) + (rho / a);
+        // the curvature term vanishes identically for a = 1 (the reference
+        // evaluates 0 * pow(0, -1) = NaN at r = 0, packing/host.c:18)
+        const float fp = (a == 1.0f ? 0.0f : -(a - 1.0f) * pow_exp(r, a - 2.0f)) + (rho / a);
         r = r - f / fp;
         if (r < 0.0f) return 0.0f;
     }

@@ -315,7 +315,9 @@ double or_max_alpha_radius(double n, double rho, double a)
     double r = fabs(n * 0.5);
     for (int it = 0; it < 10; it++) {
         double f = -pow(r, a - 1.0) + (rho / a) * (r - n);
-        double fp = -(a - 1.0) * pow(r, a - 2.0) + (rho / a);
+        /* the curvature term vanishes identically for a = 1; the reference
+         * evaluates 0 * pow(0, -1) = NaN at r = 0 (packing/host.c:18) */
+        double fp = (a == 1.0 ? 0.0 : -(a - 1.0) * pow(r, a - 2.0)) + (rho / a);
         r = r - f / fp;
         if (r < 0) return 0.0;
     }

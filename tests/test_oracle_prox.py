@@ -238,3 +238,9 @@ def test_pair_coincident_centres_separate_along_x(oracle):
     assert x[0] < 0.3 < x[3]                          # circle 1 to -x, circle 2 to +x
     assert x[3] - x[0] == pytest.approx(x[2] + x[5], abs=1e-12)
     assert (0.3 - x[0]) * 2.0 == pytest.approx((x[3] - 0.3) * 6.0, rel=1e-12)
+
+
+def test_radius_a1_at_zero_is_finite(oracle):
+    """n = 0 with a = 1: the curvature term is identically zero; the result is 1/rho."""
+    lib = oracle.load()
+    assert lib.or_max_alpha_radius(0.0, 4.0, 1.0) == pytest.approx(0.25)
