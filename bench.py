@@ -25,17 +25,26 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-# Algorithmic bytes per edge of each kernel (fp32): what the kernel must read
-# and write for one edge, excluding index streams and per-factor parameters.
-#   generic X pass (per kind): read U (12) + rho (4), write X (12)       = 28
+# Algorithmic bytes of one edge-update (one edge through a full X/M/Z/U/N
+# iteration, fp32, d = 3): the X pass reads U (12) and rho (4) and writes X
+# (12); the Z pass reads X (12), U (12), rho (4), alpha (4) and writes U (12):
+# 72 B.  N and M are transients and Z (N x 12 B) stays in cache.
+EDGE_UPDATE_BYTES = 72
+# What each kernel itself must move per edge it processes (index streams and
+# per-factor parameters excluded):
+#   generic X pass (per kind): read U, rho; write X                    = 28
 #   generic Z pass (fused M, U update): read X, U, rho, alpha; write U = 44
-#   dense fused pair-tile pass: read X, U, rho, alpha; write X, U      = 56
+#   dense fused pair-tile pass (X and Z work of a whole iteration in one
+#   pass): read X, U, rho, alpha; write X, U                           = 56
 BYTES_PER_EDGE = {
     "disk_collision_3d": 28, "box_3d": 28, "max_radius_3d": 28,
     "zpass_fused": 44,
     "pairtile_fused": 56, "unary_fused": 56, "zreduce": 0,
 }
-EDGES_PER_LAUNCH = {}
+# kernels that carry a whole iteration's edge work (their launches count
+# E edge-updates each)
+FULL_ITERATION_KERNELS = ("pairtile_fused",)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pairtile_fused_n5000_hbm_traffic.md")
 
 
 def edges_per_launch(name, n):
@@ -44,6 +53,18 @@ def edges_per_launch(name, n):
         "disk_collision_3d": 2 * pairs, "box_3d": n, "max_radius_3d": n,
         "zpass_fused": n * (n + 1), "pairtile_fused": 2 * pairs, "unary_fused": 2 * n,
     }.get(name, 0)
+
+
+def measured_traffic(kernel, n, seed):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (copy-kernel calibrated), when it covers this kernel and workload."""
+    if kernel != "pairtile_fused" or n != 5000 or seed != 0 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(n, iters, threads):
@@ -66,9 +87,11 @@ def main():
     ap.add_argument("--n", type=int, default=5000, help="circles")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profile-iters", type=int, default=10)
-    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-loop-iters", type=int, default=0,
+                    help="also time this many iterations of the legacy per-pass host loop")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -97,12 +120,37 @@ def main():
     per_iter = {k: v[0] * v[1] / args.profile_iters for k, v in prof.items()}
     dom = max(per_iter, key=per_iter.get)
     dom_ms = prof[dom][0]
-    dom_bytes = BYTES_PER_EDGE.get(dom, 0) * edges_per_launch(dom, args.n)
+    dom_edges = edges_per_launch(dom, args.n)
+    if dom in FULL_ITERATION_KERNELS:
+        dom_edges = E  # unary edges of the diagonal tiles included
+        dom_bytes = EDGE_UPDATE_BYTES * dom_edges
+    else:
+        dom_bytes = BYTES_PER_EDGE.get(dom, 0) * dom_edges
+    kernel_bytes = BYTES_PER_EDGE.get(dom, 0) * dom_edges
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom, args.n, args.seed)
     print(f"# kernels (mean ms per launch): " + ", ".join(f"{k} {v[0]:.4f}x{v[1]}" for k, v in prof.items()),
           file=sys.stderr, flush=True)
 
     p2, d2 = g.residuals()
+    host_loop = None
+    if args.host_loop_iters > 0:
+        # the reference's own loop shape (packing/host.c:355-360): updateXM,
+        # updateZ, updateUN through the legacy C API, which keeps the FP64
+        # host arrays coherent on every call (PCIe both ways, host-side
+        # AoS <-> SoA conversion included)
+        g.sync_to_host()
+        g.legacy("updateXM")
+        t0 = time.time()
+        for _ in range(args.host_loop_iters):
+            g.legacy("updateXM")
+            g.legacy("updateZ")
+            g.legacy("updateUN")
+        dt = time.time() - t0
+        host_loop = {"iterations": args.host_loop_iters, "s_per_iter": dt / args.host_loop_iters,
+                     "edge_updates_per_s": E * args.host_loop_iters / dt}
+        print(f"# legacy host loop (PCIe-inclusive): {dt / args.host_loop_iters * 1e3:.1f} ms/iter, "
+              f"{host_loop['edge_updates_per_s']:.3e} edge-updates/s", file=sys.stderr, flush=True)
     g.close()
     copy_gbs = P.copy_bandwidth(1 << 30, 20)
     print(f"# plain copy kernel (1 GiB): {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
@@ -127,13 +175,18 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "bytes_per_launch": dom_bytes, "mean_launch_ms": dom_ms,
-            "copy_kernel_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "bytes_per_launch": dom_bytes, "bytes_per_edge_update": EDGE_UPDATE_BYTES,
+            "kernel_min_bytes_per_launch": kernel_bytes,
+            "kernel_min_gbs": kernel_bytes / (dom_ms * 1e-3) / 1e9,
+            "traffic_gbs": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None,
+            "mean_launch_ms": dom_ms, "copy_kernel_gbs": copy_gbs,
         },
         "kernels_ms": {k: v[0] for k, v in prof.items()},
         "residuals_last": {"primal2": p2, "dual2": d2},
     }
+    if host_loop:
+        line["host_loop_pcie_inclusive"] = host_loop
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_iters, threads)

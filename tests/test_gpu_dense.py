@@ -231,3 +231,24 @@ def test_coincident_centres_stay_finite(layout, monkeypatch):
     o.iterate(3)
     assert np.all(np.isfinite(g.array("Z")))
     assert max_rel(g.array("Z"), o.array("Z")) < 1e-5
+
+
+@pytest.mark.parametrize("layout", ["dense", "generic"])
+def test_final_radii_vs_reference(layout, monkeypatch):
+    """5000 iterations of N=20 at rho=100 (the converging setting): final
+    radii against the reference engine compiled from source (golden fixture)
+    within 1e-4; the error against the FP64 oracle is printed beside it."""
+    if layout == "generic":
+        monkeypatch.setenv("PARADMM_LAYOUT", "generic")
+    f = np.load(os.path.join(GOLDEN, "square20_seed0_rho100.npz"), allow_pickle=False)
+    n, iters = int(f["n"]), int(f["iters_final"])
+    g, o = square_pair(n, int(f["seed"]), float(f["rho"]), float(f["alpha"]), float(f["a"]))
+    g.iterate(iters)
+    g.sync_to_host()
+    o.iterate(iters)
+    r = g.array("Z").reshape(-1, 3)[:, 2]
+    err_ref = float(np.max(np.abs(r - f["z_final"].reshape(-1, 3)[:, 2])))
+    err_or = float(np.max(np.abs(r - o.array("Z").reshape(-1, 3)[:, 2])))
+    print(f"{layout}: final radii after {iters} iterations: max |dr| vs reference {err_ref:.2e}, vs FP64 oracle {err_or:.2e}")
+    assert g.layout() == (2 if layout == "dense" else 1)
+    assert err_ref < 1e-4 and err_or < 1e-4
