@@ -1,0 +1,707 @@
+// batch.hip -- many independent unit-square packings solved in one
+// persistent launch.
+//
+// The reference solves one graph per process (core/paradmm.c:736-741; its
+// single-block GPU variant d_updateG_XMZUN_single_block,
+// core/paradmm-gpu.cu:177-281, is the closest relative).  Here B instances
+// of uneven size share one buffer set, each in the pair-tile layout of
+// pairtile.h with tile side T (32 or 64, chosen by the autotuner):
+//   slots  [s0_b, s0_b + nt_b * 2T^2)   X[3][S], U[3][S], rho[S], alpha[S]
+//   circles[c0_b, c0_b + N_b)           Z, Zprev, radius exponents
+//   tiles  [t0_b, t0_b + nt_b)          (instance, I, J), FP64 primal and
+//                                       quality partials
+//   partials P + p0_b: [5][N_b][nb_b]   per-block sums for the Z consensus
+//
+// k_batch_solve is a cooperative (co-resident) grid that loops over
+// iterations with grid-wide barriers between phases:
+//   A  every tile of every active instance, strided over the workgroups:
+//      the fused tile pass (U update of the previous iteration, prox,
+//      partials); at check iterations also the quality partials of Z;
+//   B  every circle of every active instance: Z = sum rho M / sum rho over its
+//      block partials in fixed order, FP64 dual partial.
+// Every `check_every` iterations, after phase A, one wavefront per active
+// instance reduces its residuals (primal of this iteration from phase A, dual
+// from the previous phase B) and its quality, and retires the instance when
+// primal, dual, overlap and container violation are all within tolerance;
+// every workgroup then rebuilds the list of active instances in LDS, so the
+// tiles and circles of retired instances drop out of both phases.  All
+// workgroups take the same decisions from the same global state after a grid
+// barrier, so they leave the loop together.
+#include <hip/hip_cooperative_groups.h>
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <string.h>
+
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "engine_internal.h"
+#include "engine_state.h"
+#include "pairtile.h"
+#include "paradmm_gpu.h"
+
+#define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hip_fail(e_, #x, __FILE__, __LINE__); } while (0)
+#define RC(x) do { int r_ = (x); if (r_) return r_; } while (0)
+
+namespace cg = cooperative_groups;
+
+namespace pdm {
+
+constexpr int BATCH_MAX = 1024;  // instances per batch (LDS lists)
+
+struct BInst {
+    int N, nb, nt;  // circles, blocks, tiles
+    int c0, t0;     // first circle, first tile
+    long s0;        // first slot
+    long p0;        // first float of the partial region
+    double E;       // edges: N(N-1) pair edges + 2N unary
+};
+
+struct BState {
+    int active;
+    int iterations;
+    int converged;
+    int pad;
+    double primal_rms, dual_rms, max_overlap, max_box, objective;
+    long long t_end;
+};
+
+struct BArgs {
+    int ninst;
+    int max_iters, check_every;
+    int first_nbuf;  // the first pass reads N from Nb
+    double tol_p, tol_d, tol_f;
+    long stride;
+    float *X, *U;
+    const float *rho, *alpha, *Nb;
+    float *Z, *Zprev, *P;
+    const float* aexp;
+    const BInst* inst;
+    const int4* tiles;  // {instance, I, J, local tile}
+    double* Rp;         // [tiles]
+    double* Rd;         // [circles]
+    double* Q;          // [tiles][3]
+    BState* st;
+    long long* t_start;
+};
+
+struct BLists {
+    int n;
+    int inst[BATCH_MAX];
+    int tpre[BATCH_MAX + 1];
+    int cpre[BATCH_MAX + 1];
+    int scan[3][256];
+};
+
+__device__ __forceinline__ DArgs inst_view(const BArgs& a, const BInst& in)
+{
+    DArgs v;
+    v.N = in.N;
+    v.nb = in.nb;
+    v.stride = a.stride;
+    v.X = a.X + in.s0;
+    v.U = a.U + in.s0;
+    v.rho = a.rho + in.s0;
+    v.alpha = a.alpha + in.s0;
+    v.Nb = a.Nb ? a.Nb + in.s0 : nullptr;
+    v.Mb = nullptr;
+    v.Z = a.Z + 3L * in.c0;
+    v.P = a.P + in.p0;
+    v.Rp = nullptr;
+    v.aexp = a.aexp + in.c0;
+    v.tiles = nullptr;
+    return v;
+}
+
+// Active instances in index order with prefix sums of their tile and circle
+// counts; every workgroup builds the same lists from st[].active.
+__device__ void rebuild_lists(const BArgs& a, BLists& L)
+{
+    const int tid = threadIdx.x;
+    const int per = (a.ninst + 255) / 256;
+    const int lo = min(tid * per, a.ninst), hi = min(lo + per, a.ninst);
+    int na = 0, nt = 0, nc = 0;
+    for (int i = lo; i < hi; ++i)
+        if (__hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            ++na;
+            nt += a.inst[i].nt;
+            nc += a.inst[i].N;
+        }
+    L.scan[0][tid] = na;
+    L.scan[1][tid] = nt;
+    L.scan[2][tid] = nc;
+    __syncthreads();
+    if (tid < 3) {  // exclusive scans, one thread per quantity
+        int acc = 0;
+        for (int k = 0; k < 256; ++k) {
+            const int v = L.scan[tid][k];
+            L.scan[tid][k] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    int k = L.scan[0][tid], tp = L.scan[1][tid], cp = L.scan[2][tid];
+    for (int i = lo; i < hi; ++i)
+        if (__hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            L.inst[k] = i;
+            L.tpre[k] = tp;
+            L.cpre[k] = cp;
+            ++k;
+            tp += a.inst[i].nt;
+            cp += a.inst[i].N;
+        }
+    if (tid == 255) {
+        L.n = k;
+        L.tpre[k] = tp;
+        L.cpre[k] = cp;
+    }
+    __syncthreads();
+}
+
+// position of w in a prefix array: pre[k] <= w < pre[k+1]
+__device__ __forceinline__ int find_prefix(const int* pre, int n, int w)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int T, int MODE, bool QUAL>
+__device__ void phase_tiles(const BArgs& a, const BLists& L, TileShared<T>& sh)
+{
+    const int ntiles = L.tpre[L.n];
+    for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+        const int k = find_prefix(L.tpre, L.n, w);
+        const BInst in = a.inst[L.inst[k]];
+        const int lt = w - L.tpre[k];
+        const int t = in.t0 + lt;
+        const int4 td = a.tiles[t];
+        const DArgs v = inst_view(a, in);
+        tile_pass<T, MODE, QUAL>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
+        __syncthreads();  // the tile's shared scratch is reused by the next tile
+    }
+}
+
+// Z consensus of every active circle, one thread per circle, blocks summed
+// in fixed order.
+__device__ void phase_circles(const BArgs& a, const BLists& L)
+{
+    const int ncirc = L.cpre[L.n];
+    for (int w = blockIdx.x * 256 + threadIdx.x; w < ncirc; w += gridDim.x * 256) {
+        const int k = find_prefix(L.cpre, L.n, w);
+        const BInst in = a.inst[L.inst[k]];
+        const int c = w - L.cpre[k];
+        const float* P = a.P + in.p0;
+        const long N = in.N;
+        float s[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (int K = 0; K < in.nb; ++K)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) s[q] += P[(q * N + c) * in.nb + K];
+        float* Z = a.Z + 3L * (in.c0 + c);
+        float* Zp = a.Zprev + 3L * (in.c0 + c);
+        double dz2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float z = s[q] / s[3];
+            const float zo = Z[q];
+            const double d = (double)(z - zo);
+            dz2 += d * d;
+            Zp[q] = zo;
+            Z[q] = z;
+        }
+        a.Rd[in.c0 + c] = (double)s[4] * dz2;
+    }
+}
+
+// One wavefront per active instance: residuals, quality, retirement.
+__device__ void phase_check(const BArgs& a, const BLists& L, int it, bool final)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    for (int k = wave; k < L.n; k += gridDim.x * 4) {
+        const int i = L.inst[k];
+        const BInst in = a.inst[i];
+        double p = 0.0, d = 0.0, ov = 0.0, bx = 0.0, sr = 0.0;
+        for (int t = lane; t < in.nt; t += 64) {
+            const long g = in.t0 + t;
+            p += a.Rp[g];
+            ov = fmax(ov, a.Q[3 * g]);
+            bx = fmax(bx, a.Q[3 * g + 1]);
+            sr += a.Q[3 * g + 2];
+        }
+        for (int c = lane; c < in.N; c += 64) d += a.Rd[in.c0 + c];
+        p = wsum64d(p);
+        d = wsum64d(d);
+        sr = wsum64d(sr);
+        ov = wmax64d(ov);
+        bx = wmax64d(bx);
+        if (lane == 0) {
+            const double pr = sqrt(p / (3.0 * in.E)), du = sqrt(d / (3.0 * in.E));
+            bool conv = pr <= a.tol_p && du <= a.tol_d;
+            if (a.tol_f >= 0.0) conv = conv && ov <= a.tol_f && bx <= a.tol_f;
+            BState& s = a.st[i];
+            s.iterations = it;
+            s.primal_rms = pr;
+            s.dual_rms = du;
+            s.max_overlap = ov;
+            s.max_box = bx;
+            s.objective = sr;
+            if (conv || final) {
+                s.converged = conv ? 1 : 0;
+                s.t_end = wall_clock64();
+                s.active = 0;
+            }
+        }
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_solve(BArgs a)
+{
+    cg::grid_group grid = cg::this_grid();
+    __shared__ BLists L;
+    __shared__ TileShared<T> sh;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.t_start = wall_clock64();
+    rebuild_lists(a, L);
+    for (int it = 0;; ++it) {
+        const bool final = it >= a.max_iters;
+        const bool check = it > 0 && (final || it % a.check_every == 0);
+        // phase A: the tiles (first pass: N from the buffer or Z - U)
+        if (final) phase_tiles<T, DM_FINISH, true>(a, L, sh);
+        else if (it == 0 && a.first_nbuf) phase_tiles<T, DM_FUSED_NBUF, false>(a, L, sh);
+        else if (it == 0) phase_tiles<T, DM_FUSED_CURRENT, false>(a, L, sh);
+        else if (check) phase_tiles<T, DM_FUSED, true>(a, L, sh);
+        else phase_tiles<T, DM_FUSED, false>(a, L, sh);
+        grid.sync();
+        if (check) {
+            phase_check(a, L, it, final);
+            grid.sync();
+            rebuild_lists(a, L);
+            if (L.n == 0) break;
+        }
+        // phase B: the Z consensus of the instances still active
+        phase_circles(a, L);
+        grid.sync();
+    }
+}
+
+}  // namespace pdm
+
+using namespace pdm;
+
+struct paradmm_batch {
+    std::vector<graph*> graphs;
+    int ninst = 0;
+    int T = 0;              // configured tile side (0: not configured)
+    int blocks_per_cu = 0;  // persistent grid
+    int grid = 0;
+    int cus = 0;
+    double clock_khz = 0.0;
+    std::vector<BInst> inst;
+    std::vector<std::vector<int>> pos;  // per graph: edge -> instance slot
+    std::vector<float> aexp;
+    long slots = 0;
+    int ntiles = 0, ncirc = 0;
+    long pfloats = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void*> allocs;
+    // device
+    float *X = nullptr, *U = nullptr, *rho = nullptr, *alpha = nullptr, *Nb = nullptr;
+    float *Z = nullptr, *Zprev = nullptr, *P = nullptr, *aexp_d = nullptr;
+    BInst* inst_d = nullptr;
+    int4* tiles_d = nullptr;
+    double *Rp = nullptr, *Rd = nullptr, *Q = nullptr;
+    BState* st = nullptr;
+    long long* t_start = nullptr;
+    bool solved = false;
+};
+
+static void batch_free_dev(paradmm_batch* b)
+{
+    for (void* p : b->allocs) (void)hipFree(p);
+    b->allocs.clear();
+}
+
+static int bmalloc(paradmm_batch* b, void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+    if (e != hipSuccess) {
+        pdm_set_error(PARADMM_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        *p = nullptr;
+        return PARADMM_ERR_NOMEM;
+    }
+    b->allocs.push_back(*p);
+    return PARADMM_OK;
+}
+
+static int grid_for(paradmm_batch* b, int T, int bpc, int* grid)
+{
+    int per_cu = 0;
+    hipError_t e = T == 32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_solve<32>, 256, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_solve<64>, 256, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor", __FILE__, __LINE__);
+    if (per_cu < 1) {
+        pdm_set_error(PARADMM_ERR_HIP, "batch kernel does not fit a compute unit");
+        return PARADMM_ERR_HIP;
+    }
+    if (bpc < 1 || bpc > per_cu) {
+        pdm_set_error(PARADMM_ERR_ARG, "blocks_per_cu %d outside 1..%d for tile %d", bpc, per_cu, T);
+        return PARADMM_ERR_ARG;
+    }
+    *grid = bpc * b->cus;
+    return PARADMM_OK;
+}
+
+// Builds the layout for tile side T and allocates the device buffers.
+static int batch_layout(paradmm_batch* b, int T)
+{
+    batch_free_dev(b);
+    b->inst.assign(b->ninst, BInst{});
+    b->pos.assign(b->ninst, std::vector<int>());
+    b->aexp.clear();
+    long s0 = 0, p0 = 0;
+    int t0 = 0, c0 = 0;
+    std::vector<int4> tiles;
+    for (int i = 0; i < b->ninst; ++i) {
+        graph* G = b->graphs[i];
+        std::vector<int> fk;
+        RC(pdm_factor_kinds(G, fk));
+        std::vector<float> ae;
+        int nb = 0, nt = 0;
+        const int rc = dense_structure(G, fk, T, b->pos[i], ae, nb, nt);
+        if (rc < 0) {
+            pdm_set_error(PARADMM_ERR_SHAPE, "batch instance %d is not an all-pairs unit-square packing (dim 3)", i);
+            return PARADMM_ERR_SHAPE;
+        }
+        RC(rc);
+        BInst& in = b->inst[i];
+        in.N = G->num_vars;
+        in.nb = nb;
+        in.nt = nt;
+        in.c0 = c0;
+        in.t0 = t0;
+        in.s0 = s0;
+        in.p0 = p0;
+        in.E = (double)G->num_edges;
+        for (int I = 0, t = 0; I < nb; ++I)
+            for (int J = I; J < nb; ++J, ++t) tiles.push_back(make_int4(i, I, J, t));
+        b->aexp.insert(b->aexp.end(), ae.begin(), ae.end());
+        s0 += (long)nt * 2 * T * T;
+        p0 += 5L * in.N * nb;
+        t0 += nt;
+        c0 += in.N;
+    }
+    b->slots = s0;
+    b->ntiles = t0;
+    b->ncirc = c0;
+    b->pfloats = p0;
+    const size_t S = (size_t)s0;
+    RC(bmalloc(b, (void**)&b->X, sizeof(float) * 3 * S));
+    RC(bmalloc(b, (void**)&b->U, sizeof(float) * 3 * S));
+    RC(bmalloc(b, (void**)&b->Nb, sizeof(float) * 3 * S));
+    RC(bmalloc(b, (void**)&b->rho, sizeof(float) * S));
+    RC(bmalloc(b, (void**)&b->alpha, sizeof(float) * S));
+    RC(bmalloc(b, (void**)&b->Z, sizeof(float) * 3 * (size_t)c0));
+    RC(bmalloc(b, (void**)&b->Zprev, sizeof(float) * 3 * (size_t)c0));
+    RC(bmalloc(b, (void**)&b->aexp_d, sizeof(float) * (size_t)c0));
+    RC(bmalloc(b, (void**)&b->P, sizeof(float) * (size_t)p0));
+    RC(bmalloc(b, (void**)&b->inst_d, sizeof(BInst) * (size_t)b->ninst));
+    RC(bmalloc(b, (void**)&b->tiles_d, sizeof(int4) * (size_t)t0));
+    RC(bmalloc(b, (void**)&b->Rp, sizeof(double) * (size_t)t0));
+    RC(bmalloc(b, (void**)&b->Q, sizeof(double) * 3 * (size_t)t0));
+    RC(bmalloc(b, (void**)&b->Rd, sizeof(double) * (size_t)c0));
+    RC(bmalloc(b, (void**)&b->st, sizeof(BState) * (size_t)b->ninst));
+    RC(bmalloc(b, (void**)&b->t_start, sizeof(long long)));
+    CK(hipMemcpy(b->inst_d, b->inst.data(), sizeof(BInst) * (size_t)b->ninst, hipMemcpyHostToDevice));
+    CK(hipMemcpy(b->tiles_d, tiles.data(), sizeof(int4) * (size_t)t0, hipMemcpyHostToDevice));
+    CK(hipMemcpy(b->aexp_d, b->aexp.data(), sizeof(float) * (size_t)c0, hipMemcpyHostToDevice));
+    // partials of retired or padded circles are never read, but keep them finite
+    CK(hipMemset(b->P, 0, sizeof(float) * (size_t)p0));
+    CK(hipMemset(b->Rp, 0, sizeof(double) * (size_t)t0));
+    CK(hipMemset(b->Q, 0, sizeof(double) * 3 * (size_t)t0));
+    CK(hipMemset(b->Rd, 0, sizeof(double) * (size_t)c0));
+    b->T = T;
+    b->solved = false;
+    return PARADMM_OK;
+}
+
+// Host state of every graph -> device (fp32 SoA); padding slots zero.
+static int batch_upload(paradmm_batch* b)
+{
+    const size_t S = (size_t)b->slots;
+    std::vector<float> x(3 * S, 0.0f), u(3 * S, 0.0f), n(3 * S, 0.0f), r(S, 0.0f), al(S, 0.0f);
+    std::vector<float> z(3 * (size_t)b->ncirc);
+    for (int i = 0; i < b->ninst; ++i) {
+        graph* G = b->graphs[i];
+        const BInst& in = b->inst[i];
+        if (!G->X || !G->U || !G->N || !G->Z || !G->RHOS || !G->ALPHAS) {
+            pdm_set_error(PARADMM_ERR_STATE, "batch instance %d: graph arrays not initialised", i);
+            return PARADMM_ERR_STATE;
+        }
+        const std::vector<int>& pos = b->pos[i];
+        for (int e = 0; e < G->num_edges; ++e) {
+            const size_t p = (size_t)in.s0 + (size_t)pos[e];
+            for (int k = 0; k < 3; ++k) {
+                x[k * S + p] = (float)G->X[3L * e + k];
+                u[k * S + p] = (float)G->U[3L * e + k];
+                n[k * S + p] = (float)G->N[3L * e + k];
+            }
+            r[p] = (float)G->RHOS[e];
+            al[p] = (float)G->ALPHAS[e];
+        }
+        for (int v = 0; v < in.N; ++v)
+            for (int k = 0; k < 3; ++k) z[3L * (in.c0 + v) + k] = (float)G->Z[3L * v + k];
+    }
+    CK(hipMemcpyAsync(b->X, x.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice, b->stream));
+    CK(hipMemcpyAsync(b->U, u.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice, b->stream));
+    CK(hipMemcpyAsync(b->Nb, n.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice, b->stream));
+    CK(hipMemcpyAsync(b->rho, r.data(), sizeof(float) * S, hipMemcpyHostToDevice, b->stream));
+    CK(hipMemcpyAsync(b->alpha, al.data(), sizeof(float) * S, hipMemcpyHostToDevice, b->stream));
+    CK(hipMemcpyAsync(b->Z, z.data(), sizeof(float) * z.size(), hipMemcpyHostToDevice, b->stream));
+    CK(hipMemcpyAsync(b->Zprev, z.data(), sizeof(float) * z.size(), hipMemcpyHostToDevice, b->stream));
+    std::vector<BState> st(b->ninst);
+    for (BState& s : st) {
+        memset(&s, 0, sizeof s);
+        s.active = 1;
+        s.max_overlap = s.max_box = s.objective = -1.0;
+    }
+    CK(hipMemcpyAsync(b->st, st.data(), sizeof(BState) * st.size(), hipMemcpyHostToDevice, b->stream));
+    CK(hipStreamSynchronize(b->stream));
+    return PARADMM_OK;
+}
+
+static int batch_launch(paradmm_batch* b, const paradmm_solve_opts* o, float* ms)
+{
+    BArgs a;
+    a.ninst = b->ninst;
+    a.max_iters = o->max_iters;
+    a.check_every = o->check_every;
+    a.first_nbuf = 1;
+    a.tol_p = o->tol_primal;
+    a.tol_d = o->tol_dual;
+    a.tol_f = o->tol_feas;
+    a.stride = b->slots;
+    a.X = b->X;
+    a.U = b->U;
+    a.rho = b->rho;
+    a.alpha = b->alpha;
+    a.Nb = b->Nb;
+    a.Z = b->Z;
+    a.Zprev = b->Zprev;
+    a.P = b->P;
+    a.aexp = b->aexp_d;
+    a.inst = b->inst_d;
+    a.tiles = b->tiles_d;
+    a.Rp = b->Rp;
+    a.Rd = b->Rd;
+    a.Q = b->Q;
+    a.st = b->st;
+    a.t_start = b->t_start;
+    void* args[] = {&a};
+    const void* fn = b->T == 32 ? (const void*)k_batch_solve<32> : (const void*)k_batch_solve<64>;
+    CK(hipEventRecord(b->ev0, b->stream));
+    CK(hipLaunchCooperativeKernel(fn, dim3(b->grid), dim3(256), args, 0, b->stream));
+    CK(hipEventRecord(b->ev1, b->stream));
+    CK(hipEventSynchronize(b->ev1));
+    CK(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    return PARADMM_OK;
+}
+
+static int check_opts(const paradmm_solve_opts* o)
+{
+    if (!o || o->max_iters < 1 || o->check_every < 1) {
+        pdm_set_error(PARADMM_ERR_ARG, "batch: bad solve options");
+        return PARADMM_ERR_ARG;
+    }
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_batch_create(paradmm_batch** out, graph* const* graphs, int count)
+{
+    if (!out || !graphs || count < 1 || count > BATCH_MAX) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_create: need 1..%d graphs", BATCH_MAX);
+        return PARADMM_ERR_ARG;
+    }
+    for (int i = 0; i < count; ++i)
+        if (!graphs[i]) {
+            pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_create: graph %d is NULL", i);
+            return PARADMM_ERR_ARG;
+        }
+    paradmm_batch* b = new paradmm_batch();
+    b->graphs.assign(graphs, graphs + count);
+    b->ninst = count;
+    int dev = 0, khz = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        const int rc = hip_fail(e, "paradmm_batch_create", __FILE__, __LINE__);
+        paradmm_batch_destroy(b);
+        return rc;
+    }
+    b->clock_khz = khz;
+    *out = b;
+    return PARADMM_OK;
+}
+
+extern "C" void paradmm_batch_destroy(paradmm_batch* b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_free_dev(b);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int paradmm_batch_configure(paradmm_batch* b, int tile, int blocks_per_cu)
+{
+    if (!b || (tile != 32 && tile != 64)) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_configure: tile must be 32 or 64");
+        return PARADMM_ERR_ARG;
+    }
+    int grid = 0;
+    RC(grid_for(b, tile, blocks_per_cu, &grid));
+    if (b->T != tile) RC(batch_layout(b, tile));
+    b->blocks_per_cu = blocks_per_cu;
+    b->grid = grid;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_batch_autotune(paradmm_batch* b, int iters, paradmm_batch_config* chosen,
+                                      paradmm_batch_config* tried, int max_tried, int* ntried)
+{
+    if (!b || iters < 1) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_autotune: bad arguments");
+        return PARADMM_ERR_ARG;
+    }
+    // every candidate runs the same fixed number of iterations from the
+    // graphs' state, without convergence checks
+    paradmm_solve_opts o;
+    memset(&o, 0, sizeof o);
+    o.max_iters = iters;
+    o.check_every = iters;
+    o.tol_primal = o.tol_dual = -1.0;
+    o.tol_feas = -1.0;
+    paradmm_batch_config best{0, 0, 0.0};
+    int n = 0;
+    const int tiles[2] = {32, 64};
+    for (int ti = 0; ti < 2; ++ti) {
+        const int T = tiles[ti];
+        int per_cu = 0;
+        hipError_t e = T == 32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_solve<32>, 256, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_solve<64>, 256, 0);
+        if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor", __FILE__, __LINE__);
+        for (int bpc = 1; bpc <= per_cu && bpc <= 4; ++bpc) {
+            RC(paradmm_batch_configure(b, T, bpc));
+            RC(batch_upload(b));
+            float ms = 0.0f;
+            RC(batch_launch(b, &o, &ms));  // warm
+            RC(batch_upload(b));
+            RC(batch_launch(b, &o, &ms));
+            const double per = ms / iters;
+            if (tried && n < max_tried) tried[n] = paradmm_batch_config{T, bpc, per};
+            ++n;
+            if (best.tile == 0 || per < best.ms_per_iter) best = paradmm_batch_config{T, bpc, per};
+        }
+    }
+    if (ntried) *ntried = n;
+    RC(paradmm_batch_configure(b, best.tile, best.blocks_per_cu));
+    if (chosen) *chosen = best;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_batch_solve(paradmm_batch* b, const paradmm_solve_opts* o, paradmm_solve_result* res,
+                                   double* kernel_ms)
+{
+    if (!b || !res) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_solve: bad arguments");
+        return PARADMM_ERR_ARG;
+    }
+    RC(check_opts(o));
+    if (o->policy_every > 0) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_solve: the batch runs the plain fixed-rho method (policy_every 0)");
+        return PARADMM_ERR_ARG;
+    }
+    if (!b->T) RC(paradmm_batch_configure(b, 32, 1));
+    RC(batch_upload(b));
+    float ms = 0.0f;
+    RC(batch_launch(b, o, &ms));
+    std::vector<BState> st(b->ninst);
+    long long t0 = 0;
+    CK(hipMemcpy(st.data(), b->st, sizeof(BState) * st.size(), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(&t0, b->t_start, sizeof t0, hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->ninst; ++i) {
+        paradmm_solve_result& r = res[i];
+        r.iterations = st[i].iterations;
+        r.converged = st[i].converged;
+        r.primal_rms = st[i].primal_rms;
+        r.dual_rms = st[i].dual_rms;
+        r.max_overlap = st[i].max_overlap;
+        r.max_box = st[i].max_box;
+        r.objective = st[i].objective;
+        r.elapsed_ms = (double)(st[i].t_end - t0) / b->clock_khz;
+    }
+    if (kernel_ms) *kernel_ms = ms;
+    b->solved = true;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_batch_sync_to_host(paradmm_batch* b)
+{
+    if (!b || !b->solved) {
+        pdm_set_error(PARADMM_ERR_STATE, "paradmm_batch_sync_to_host: nothing solved");
+        return PARADMM_ERR_STATE;
+    }
+    const size_t S = (size_t)b->slots;
+    std::vector<float> x(3 * S), u(3 * S), r(S), z(3 * (size_t)b->ncirc);
+    CK(hipMemcpy(x.data(), b->X, sizeof(float) * 3 * S, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(u.data(), b->U, sizeof(float) * 3 * S, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(r.data(), b->rho, sizeof(float) * S, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(z.data(), b->Z, sizeof(float) * z.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->ninst; ++i) {
+        graph* G = b->graphs[i];
+        const BInst& in = b->inst[i];
+        const std::vector<int>& pos = b->pos[i];
+        for (int v = 0; v < in.N; ++v)
+            for (int k = 0; k < 3; ++k) G->Z[3L * v + k] = z[3L * (in.c0 + v) + k];
+        for (int e = 0; e < G->num_edges; ++e) {
+            const size_t p = (size_t)in.s0 + (size_t)pos[e];
+            const int v = G->edgeix_to_varlistix[e];
+            for (int k = 0; k < 3; ++k) {
+                const double xe = x[k * S + p], ue = u[k * S + p];
+                G->X[3L * e + k] = xe;
+                G->U[3L * e + k] = ue;
+                G->M[3L * e + k] = xe + ue;
+                G->N[3L * e + k] = G->Z[3L * v + k] - ue;
+            }
+            G->RHOS[e] = r[p];
+        }
+    }
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_batch_info(paradmm_batch* b, int* tile, int* blocks_per_cu, int* grid, long* slots, int* tiles)
+{
+    if (!b) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_info: NULL batch");
+        return PARADMM_ERR_ARG;
+    }
+    if (tile) *tile = b->T;
+    if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu;
+    if (grid) *grid = b->grid;
+    if (slots) *slots = b->slots;
+    if (tiles) *tiles = b->ntiles;
+    return PARADMM_OK;
+}

@@ -566,18 +566,12 @@ static int finalize_generic(DevGraph* s, graph* G, const std::vector<int>& fkind
     return PARADMM_OK;
 }
 
-static int finalize(graph* G, DevGraph** out)
+namespace pdm {
+// device operator kind of every factor, with degree / dimension checks
+int pdm_factor_kinds(graph* G, std::vector<int>& fkind)
 {
-    const int E = G->num_edges, V = G->num_vars, F = G->num_opts, D = G->num_dims;
-    if (D < 1 || D > 4) {
-        pdm_set_error(PARADMM_ERR_SHAPE, "num_dims %d not supported (1..4)", D);
-        return PARADMM_ERR_SHAPE;
-    }
-    if (E <= 0 || V <= 0 || F <= 0) {
-        pdm_set_error(PARADMM_ERR_STATE, "graph is empty");
-        return PARADMM_ERR_STATE;
-    }
-    std::vector<int> fkind(F);
+    const int F = G->num_opts, D = G->num_dims;
+    fkind.assign(F, 0);
     for (int f = 0; f < F; ++f) {
         paradmm_prox_fn fn = (paradmm_prox_fn)G->list_of_operators_functions_pointers[f];
         const int K = paradmm_prox_kind_of(fn);
@@ -599,6 +593,24 @@ static int finalize(graph* G, DevGraph** out)
         }
         fkind[f] = K;
     }
+
+    return PARADMM_OK;
+}
+}  // namespace pdm
+
+static int finalize(graph* G, DevGraph** out)
+{
+    const int E = G->num_edges, V = G->num_vars, F = G->num_opts, D = G->num_dims;
+    if (D < 1 || D > 4) {
+        pdm_set_error(PARADMM_ERR_SHAPE, "num_dims %d not supported (1..4)", D);
+        return PARADMM_ERR_SHAPE;
+    }
+    if (E <= 0 || V <= 0 || F <= 0) {
+        pdm_set_error(PARADMM_ERR_STATE, "graph is empty");
+        return PARADMM_ERR_STATE;
+    }
+    std::vector<int> fkind;
+    RC(pdm_factor_kinds(G, fkind));
 
     DevGraph* s = new DevGraph();
     s->G = G;

@@ -98,6 +98,11 @@ void prof_end(DevGraph* s);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 // device state of G, finalised; with upload, host writes since are uploaded
 int pdm_get_state(graph* G, DevGraph** out, bool upload);
+int pdm_factor_kinds(graph* G, std::vector<int>& fkind);
+// the dense pair-tile slot of every edge for tile side T; -1 when the graph
+// is not the all-pairs unit-square packing
+int dense_structure(graph* G, const std::vector<int>& fkind, int T, std::vector<int>& pos, std::vector<float>& aexp,
+                    int& nb, int& nt);
 int dmalloc(DevGraph* s, void** p, size_t bytes);
 int ensure_aux(DevGraph* s);
 int launch_mrecon(DevGraph* s);

@@ -60,6 +60,14 @@ class SolveResult(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchConfig(C.Structure):
+    """paradmm_batch_config of paradmm_gpu.h"""
+    _fields_ = [("tile", C.c_int), ("blocks_per_cu", C.c_int), ("ms_per_iter", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class ParadmmError(RuntimeError):
     pass
 
@@ -116,6 +124,14 @@ def load():
         "paradmm_gpu_square_quality": (C.c_int, [G, _DP, _DP, _DP]),
         "paradmm_rho_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), _FP, C.c_int, C.c_uint64]),
         "paradmm_rho_mlp_destroy": (None, [V]),
+        "paradmm_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(G), C.c_int]),
+        "paradmm_batch_destroy": (None, [V]),
+        "paradmm_batch_configure": (C.c_int, [V, C.c_int, C.c_int]),
+        "paradmm_batch_autotune": (C.c_int, [V, C.c_int, C.POINTER(BatchConfig), C.POINTER(BatchConfig), C.c_int,
+                                             _IP]),
+        "paradmm_batch_solve": (C.c_int, [V, C.POINTER(SolveOpts), C.POINTER(SolveResult), _DP]),
+        "paradmm_batch_sync_to_host": (C.c_int, [V]),
+        "paradmm_batch_info": (C.c_int, [V, _IP, _IP, _IP, C.POINTER(C.c_long), _IP]),
         "paradmm_rho_mlp_get_params": (C.c_int, [V, _FP, C.c_int]),
         "paradmm_rho_mlp_forward": (C.c_int, [V, _FP, C.c_int, _FP]),
         "paradmm_rho_mlp_gradient": (C.c_int, [V, _FP, _FP, C.c_int, _FP, _DP]),
@@ -393,3 +409,69 @@ def packing_square(ncircles, seed, rho=1.0, alpha=1.0, a_exp=1.0):
     g.init_rho_alpha(rho, alpha)
     g.init_square(seed)
     return g
+
+
+class Batch:
+    """Many unit-square packings solved in one persistent launch
+    (paradmm_batch_* of paradmm_gpu.h).  Keeps its Graph objects alive."""
+
+    def __init__(self, graphs):
+        self.lib = load()
+        self.graphs = list(graphs)
+        arr = (C.POINTER(GraphStruct) * len(self.graphs))(*[g.ptr for g in self.graphs])
+        h = C.c_void_p()
+        check(self.lib.paradmm_batch_create(C.byref(h), arr, len(self.graphs)), "paradmm_batch_create")
+        self.h = h
+
+    def configure(self, tile, blocks_per_cu=1):
+        check(self.lib.paradmm_batch_configure(self.h, tile, blocks_per_cu), "paradmm_batch_configure")
+
+    def autotune(self, iters=20):
+        chosen = BatchConfig()
+        tried = (BatchConfig * 16)()
+        n = C.c_int()
+        check(self.lib.paradmm_batch_autotune(self.h, iters, C.byref(chosen), tried, 16, C.byref(n)),
+              "paradmm_batch_autotune")
+        return chosen.as_dict(), [tried[i].as_dict() for i in range(n.value)]
+
+    def solve(self, max_iters, tol=1e-4, tol_feas=1e-4, check_every=50):
+        o = SolveOpts(max_iters, check_every, tol, tol, tol_feas, 0)
+        res = (SolveResult * len(self.graphs))()
+        ms = C.c_double()
+        check(self.lib.paradmm_batch_solve(self.h, C.byref(o), res, C.byref(ms)), "paradmm_batch_solve")
+        return [r.as_dict() for r in res], ms.value
+
+    def sync_to_host(self):
+        check(self.lib.paradmm_batch_sync_to_host(self.h), "paradmm_batch_sync_to_host")
+
+    def info(self):
+        t, b, gr, nt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        sl = C.c_long()
+        check(self.lib.paradmm_batch_info(self.h, C.byref(t), C.byref(b), C.byref(gr), C.byref(sl), C.byref(nt)),
+              "paradmm_batch_info")
+        return {"tile": t.value, "blocks_per_cu": b.value, "grid": gr.value, "slots": sl.value, "tiles": nt.value}
+
+    def close(self):
+        if self.h:
+            self.lib.paradmm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_sizes(count, seed, lo=16, hi=512):
+    """Instance sizes N_i ~ U{lo..hi} drawn from the batch seed (splitmix64)."""
+    out = []
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    for _ in range(count):
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        z ^= z >> 31
+        out.append(lo + int(z % (hi - lo + 1)))
+    return out

@@ -182,6 +182,43 @@ int paradmm_gpu_state_restore(graph *G);
  * reported next to the nominal peak. */
 int paradmm_gpu_copy_bandwidth(size_t bytes, int iters, double *gbs);
 
+/* ---- batched solve -----------------------------------------------------------
+ * Many independent unit-square packings (each graph the all-pairs dim-3
+ * packing built by paradmm_packing_square, or the same structure built with
+ * addNode) solved together by the plain fixed-rho method in ONE persistent
+ * cooperative launch.  Each instance is checked every check_every iterations
+ * (primal and dual RMS, max overlap and container violation) and retired
+ * when within tolerance; retired instances drop out of the remaining
+ * iterations.  No reference counterpart: the reference runs one graph per
+ * process (core/paradmm.c:736-741); its single-block GPU loop
+ * (core/paradmm-gpu.cu:177-281) is the closest relative.  At most 1024
+ * graphs; the graphs must outlive the batch. */
+typedef struct paradmm_batch paradmm_batch;
+typedef struct {
+    int tile;            /* pair-tile side: 32 or 64 circles */
+    int blocks_per_cu;   /* persistent workgroups per compute unit */
+    double ms_per_iter;  /* measured by the autotuner */
+} paradmm_batch_config;
+
+int paradmm_batch_create(paradmm_batch **out, graph *const *graphs, int count);
+void paradmm_batch_destroy(paradmm_batch *b);
+/* Fixes the tile side and grid (default on first solve: tile 32, 1 block/CU). */
+int paradmm_batch_configure(paradmm_batch *b, int tile, int blocks_per_cu);
+/* Times `iters` iterations of every candidate (tile 32/64 x blocks per CU up
+ * to the kernel's occupancy) from the graphs' state and configures the
+ * fastest; *chosen gets it, tried[0..max_tried) the candidates measured. */
+int paradmm_batch_autotune(paradmm_batch *b, int iters, paradmm_batch_config *chosen,
+                           paradmm_batch_config *tried, int max_tried, int *ntried);
+/* Uploads every graph's host state (X, U, N, Z, RHOS, ALPHAS) and solves;
+ * res[i] per instance (iterations at retirement, residuals and quality of the
+ * last check, elapsed_ms from the launch start to its retirement, device
+ * clock); *kernel_ms the whole launch.  opts->policy_every must be 0. */
+int paradmm_batch_solve(paradmm_batch *b, const paradmm_solve_opts *opts, paradmm_solve_result *res,
+                        double *kernel_ms);
+/* Writes Z, X, U, RHOS (and M = X + U, N = Z - U) back into every graph. */
+int paradmm_batch_sync_to_host(paradmm_batch *b);
+int paradmm_batch_info(paradmm_batch *b, int *tile, int *blocks_per_cu, int *grid, long *slots, int *tiles);
+
 #ifdef __cplusplus
 }
 #endif

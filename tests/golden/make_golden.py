@@ -9,6 +9,8 @@ Fixtures (numpy .npz, float64):
                        with the oracle's dim-3 operators; Z after iterations 1..200
                        and after 5000.
   square20_seed0_rho100.npz   same at rho=100 (the converging setting).
+  square20_seed0_rho100_converged.npz   Z of the rho=100 run after 12000
+                       iterations (converged: tolerance 1e-4 is met at 8450).
   triangle6_seed1.npz  the reference demo's triangle packing (packing/host.c
                        factor order and operators) N=6, seeded init in [-1,2];
                        Z after iterations 1..200.
@@ -44,6 +46,11 @@ def main():
     h = square(ref, 20, 0, 100.0, 1.0, 1.0, 5000)
     np.savez_compressed(os.path.join(HERE, "square20_seed0_rho100.npz"), z=h[:200], z_final=h[-1],
                         n=20, seed=0, rho=100.0, alpha=1.0, a=1.0, iters_final=5000)
+
+    # converged state of the rho = 100 run (tolerance 1e-4 is met at 8450)
+    h = square(ref, 20, 0, 100.0, 1.0, 1.0, 12000)
+    np.savez_compressed(os.path.join(HERE, "square20_seed0_rho100_converged.npz"), z_final=h[-1],
+                        n=20, seed=0, rho=100.0, alpha=1.0, a=1.0, iters_final=12000)
 
     nc, iters = 6, 200
     hist = np.zeros((iters, 2 * nc * 2))

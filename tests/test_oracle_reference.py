@@ -29,6 +29,14 @@ def test_square_trajectory_matches_reference(oracle, fixture):
     assert np.max(np.abs(g.array("Z") - f["z_final"])) <= TOL
 
 
+def test_square_converged_state_matches_reference(oracle):
+    """12000 iterations at rho = 100 (converged): oracle and reference agree."""
+    f = load("square20_seed0_rho100_converged.npz")
+    g = oracle.Graph.square(int(f["n"]), int(f["seed"]), float(f["rho"]), float(f["alpha"]), float(f["a"]))
+    g.iterate(int(f["iters_final"]))
+    assert np.max(np.abs(g.array("Z") - f["z_final"])) <= 1e-10
+
+
 def test_triangle_trajectory_matches_reference(oracle):
     """The reference demo problem (packing/host.c factor order, its own dim-2 ops)."""
     f = load("triangle6_seed1.npz")
