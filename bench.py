@@ -79,6 +79,52 @@ def cpu_baseline(n, iters, threads):
             "sample": f"N={n} unit-square packing, {iters} OpenMP iterations (FP64, -Ofast), {dt:.2f} s"}
 
 
+def batch_rho(n):
+    """Fixed rho of one batch instance: the fastest-converging fixed rho grows
+    with N (about 35 at N = 40..64, 70 at 100..150, 150 at 300, 150..300 at
+    500 on the plain solver), so the batch uses clip(N/2, 35, 300)."""
+    return float(min(300.0, max(35.0, 0.5 * n)))
+
+
+def batch_bench(P, count, seed, iters, cap):
+    """The batched workload: `count` unit-square packings with N ~ U{16..512}
+    drawn from the batch seed, one persistent launch.  Returns the
+    throughput of `iters` iterations without retirement and the
+    time-to-tolerance solve with per-instance retirement."""
+    t0 = time.time()
+    sizes = P.batch_sizes(count, seed)
+    gs = [P.packing_square(n, 1000 * seed + k, rho=batch_rho(n), alpha=1.0, a_exp=1.0) for k, n in enumerate(sizes)]
+    E = sum(g.nedge for g in gs)
+    b = P.Batch(gs)
+    chosen, tried = b.autotune(20)
+    t_setup = time.time() - t0
+    print(f"# batch: {count} instances, E={E}, setup {t_setup:.1f} s, autotune {chosen} of {tried}",
+          file=sys.stderr, flush=True)
+    _, ms = b.solve(iters, tol=-1.0, tol_feas=-1.0, check_every=iters + 1)
+    thr = E * iters / (ms * 1e-3)
+    print(f"# batch throughput: {ms / iters:.4f} ms/iter, {thr:.4e} edge-updates/s", file=sys.stderr, flush=True)
+    res, ms2 = b.solve(cap, tol=1e-4, tol_feas=1e-4, check_every=50)
+    conv = [r for r in res if r["converged"]]
+    done_edge_iters = sum(r["iterations"] * g.nedge for r, g in zip(res, gs))
+    print(f"# batch to tolerance: {len(conv)}/{count} converged, launch {ms2:.1f} ms, last retirement "
+          f"{max(r['elapsed_ms'] for r in res):.1f} ms", file=sys.stderr, flush=True)
+    info = b.info()
+    b.close()
+    for g in gs:
+        g.close()
+    return {
+        "workload": f"{count} unit-square packings, N ~ U{{16..512}} from batch seed {seed}, rho = clip(N/2, 35, 300), "
+                    f"alpha 1, one persistent launch",
+        "instances": count, "edges": E, "tile": info["tile"], "blocks_per_cu": info["blocks_per_cu"],
+        "autotune_tried": tried,
+        "iters": iters, "ms_per_iter": ms / iters, "edge_updates_per_s": thr,
+        "roofline_frac_72B": thr * EDGE_UPDATE_BYTES / 1e9 / HBM_PEAK_GBS,
+        "to_tol": {"cap": cap, "check_every": 50, "converged": len(conv), "launch_ms": ms2,
+                   "edge_updates_per_s": done_edge_iters / (ms2 * 1e-3),
+                   "iterations_median": float(sorted(r["iterations"] for r in res)[count // 2])},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +136,10 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=256, help="instances of the batched workload (0: skip)")
+    ap.add_argument("--batch-seed", type=int, default=0)
+    ap.add_argument("--batch-iters", type=int, default=200)
+    ap.add_argument("--batch-cap", type=int, default=20000)
     ap.add_argument("--host-loop-iters", type=int, default=0,
                     help="also time this many iterations of the legacy per-pass host loop")
     args = ap.parse_args()
@@ -187,6 +237,8 @@ def main():
     }
     if host_loop:
         line["host_loop_pcie_inclusive"] = host_loop
+    if args.batch > 0:
+        line["batch"] = batch_bench(P, args.batch, args.batch_seed, args.batch_iters, args.batch_cap)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_iters, threads)

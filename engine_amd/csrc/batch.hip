@@ -39,6 +39,7 @@
 
 #include "engine_internal.h"
 #include "engine_state.h"
+#include "gridbar.h"
 #include "pairtile.h"
 #include "paradmm_gpu.h"
 
@@ -85,7 +86,21 @@ struct BArgs {
     double* Q;          // [tiles][3]
     BState* st;
     long long* t_start;
+    unsigned* bar;        // grid barrier words (zeroed before the launch)
+    unsigned spin_limit;  // polls before a barrier wait gives up
+    int barrier;          // 0: cooperative-groups grid sync, 1: two-level barrier
 };
+
+// Grid-wide barrier between phases; false when the two-level barrier gave up
+// (every workgroup then leaves the kernel).
+__device__ __forceinline__ bool phase_barrier(const BArgs& a, cg::grid_group& grid, unsigned& gen)
+{
+    if (a.barrier == 0) {
+        grid.sync();
+        return true;
+    }
+    return grid_barrier(a.bar, gen, a.spin_limit);
+}
 
 struct BLists {
     int n;
@@ -268,6 +283,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     __shared__ BLists L;
     __shared__ TileShared<T> sh;
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.t_start = wall_clock64();
+    unsigned gen = 0;
     rebuild_lists(a, L);
     for (int it = 0;; ++it) {
         const bool final = it >= a.max_iters;
@@ -278,16 +294,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         else if (it == 0) phase_tiles<T, DM_FUSED_CURRENT, false>(a, L, sh);
         else if (check) phase_tiles<T, DM_FUSED, true>(a, L, sh);
         else phase_tiles<T, DM_FUSED, false>(a, L, sh);
-        grid.sync();
+        if (!phase_barrier(a, grid, gen)) return;
         if (check) {
             phase_check(a, L, it, final);
-            grid.sync();
+            if (!phase_barrier(a, grid, gen)) return;
             rebuild_lists(a, L);
             if (L.n == 0) break;
         }
         // phase B: the Z consensus of the instances still active
         phase_circles(a, L);
-        grid.sync();
+        if (!phase_barrier(a, grid, gen)) return;
     }
 }
 
@@ -320,6 +336,8 @@ struct paradmm_batch {
     double *Rp = nullptr, *Rd = nullptr, *Q = nullptr;
     BState* st = nullptr;
     long long* t_start = nullptr;
+    unsigned* bar = nullptr;
+    int barrier = 1;  // two-level barrier by default
     bool solved = false;
 };
 
@@ -419,6 +437,7 @@ static int batch_layout(paradmm_batch* b, int T)
     RC(bmalloc(b, (void**)&b->Rd, sizeof(double) * (size_t)c0));
     RC(bmalloc(b, (void**)&b->st, sizeof(BState) * (size_t)b->ninst));
     RC(bmalloc(b, (void**)&b->t_start, sizeof(long long)));
+    RC(bmalloc(b, (void**)&b->bar, sizeof(unsigned) * GRIDBAR_WORDS));
     CK(hipMemcpy(b->inst_d, b->inst.data(), sizeof(BInst) * (size_t)b->ninst, hipMemcpyHostToDevice));
     CK(hipMemcpy(b->tiles_d, tiles.data(), sizeof(int4) * (size_t)t0, hipMemcpyHostToDevice));
     CK(hipMemcpy(b->aexp_d, b->aexp.data(), sizeof(float) * (size_t)c0, hipMemcpyHostToDevice));
@@ -429,6 +448,20 @@ static int batch_layout(paradmm_batch* b, int T)
     CK(hipMemset(b->Rd, 0, sizeof(double) * (size_t)c0));
     b->T = T;
     b->solved = false;
+    return PARADMM_OK;
+}
+
+// Every instance active again, results cleared.
+static int batch_reset_flags(paradmm_batch* b)
+{
+    std::vector<BState> st(b->ninst);
+    for (BState& s : st) {
+        memset(&s, 0, sizeof s);
+        s.active = 1;
+        s.max_overlap = s.max_box = s.objective = -1.0;
+    }
+    CK(hipMemcpyAsync(b->st, st.data(), sizeof(BState) * st.size(), hipMemcpyHostToDevice, b->stream));
+    CK(hipStreamSynchronize(b->stream));
     return PARADMM_OK;
 }
 
@@ -466,15 +499,7 @@ static int batch_upload(paradmm_batch* b)
     CK(hipMemcpyAsync(b->alpha, al.data(), sizeof(float) * S, hipMemcpyHostToDevice, b->stream));
     CK(hipMemcpyAsync(b->Z, z.data(), sizeof(float) * z.size(), hipMemcpyHostToDevice, b->stream));
     CK(hipMemcpyAsync(b->Zprev, z.data(), sizeof(float) * z.size(), hipMemcpyHostToDevice, b->stream));
-    std::vector<BState> st(b->ninst);
-    for (BState& s : st) {
-        memset(&s, 0, sizeof s);
-        s.active = 1;
-        s.max_overlap = s.max_box = s.objective = -1.0;
-    }
-    CK(hipMemcpyAsync(b->st, st.data(), sizeof(BState) * st.size(), hipMemcpyHostToDevice, b->stream));
-    CK(hipStreamSynchronize(b->stream));
-    return PARADMM_OK;
+    return batch_reset_flags(b);
 }
 
 static int batch_launch(paradmm_batch* b, const paradmm_solve_opts* o, float* ms)
@@ -504,13 +529,23 @@ static int batch_launch(paradmm_batch* b, const paradmm_solve_opts* o, float* ms
     a.Q = b->Q;
     a.st = b->st;
     a.t_start = b->t_start;
+    a.bar = b->bar;
+    a.spin_limit = 1u << 24;
+    a.barrier = b->barrier;
     void* args[] = {&a};
+    CK(hipMemsetAsync(b->bar, 0, sizeof(unsigned) * GRIDBAR_WORDS, b->stream));
     const void* fn = b->T == 32 ? (const void*)k_batch_solve<32> : (const void*)k_batch_solve<64>;
     CK(hipEventRecord(b->ev0, b->stream));
     CK(hipLaunchCooperativeKernel(fn, dim3(b->grid), dim3(256), args, 0, b->stream));
     CK(hipEventRecord(b->ev1, b->stream));
     CK(hipEventSynchronize(b->ev1));
     CK(hipEventElapsedTime(ms, b->ev0, b->ev1));
+    unsigned aborted = 0;
+    CK(hipMemcpy(&aborted, b->bar + (GRIDBAR_GROUPS + 2) * GRIDBAR_LINE, sizeof aborted, hipMemcpyDeviceToHost));
+    if (aborted) {
+        pdm_set_error(PARADMM_ERR_HIP, "batch: a grid barrier wait timed out (workgroups not co-resident?)");
+        return PARADMM_ERR_HIP;
+    }
     return PARADMM_OK;
 }
 
@@ -565,10 +600,10 @@ extern "C" void paradmm_batch_destroy(paradmm_batch* b)
     delete b;
 }
 
-extern "C" int paradmm_batch_configure(paradmm_batch* b, int tile, int blocks_per_cu)
+extern "C" int paradmm_batch_configure(paradmm_batch* b, int tile, int blocks_per_cu, int barrier)
 {
-    if (!b || (tile != 32 && tile != 64)) {
-        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_configure: tile must be 32 or 64");
+    if (!b || (tile != 32 && tile != 64) || (barrier != 0 && barrier != 1)) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_configure: tile must be 32 or 64, barrier 0 or 1");
         return PARADMM_ERR_ARG;
     }
     int grid = 0;
@@ -576,6 +611,7 @@ extern "C" int paradmm_batch_configure(paradmm_batch* b, int tile, int blocks_pe
     if (b->T != tile) RC(batch_layout(b, tile));
     b->blocks_per_cu = blocks_per_cu;
     b->grid = grid;
+    b->barrier = barrier;
     return PARADMM_OK;
 }
 
@@ -586,15 +622,16 @@ extern "C" int paradmm_batch_autotune(paradmm_batch* b, int iters, paradmm_batch
         pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_autotune: bad arguments");
         return PARADMM_ERR_ARG;
     }
-    // every candidate runs the same fixed number of iterations from the
-    // graphs' state, without convergence checks
+    // every candidate runs `iters` iterations without convergence checks
+    // (after one warm launch); the state keeps iterating from the graphs'
+    // state uploaded once per tile side, which does not change the work
     paradmm_solve_opts o;
     memset(&o, 0, sizeof o);
     o.max_iters = iters;
     o.check_every = iters;
     o.tol_primal = o.tol_dual = -1.0;
     o.tol_feas = -1.0;
-    paradmm_batch_config best{0, 0, 0.0};
+    paradmm_batch_config best{0, 0, 0, 0.0};
     int n = 0;
     const int tiles[2] = {32, 64};
     for (int ti = 0; ti < 2; ++ti) {
@@ -603,21 +640,25 @@ extern "C" int paradmm_batch_autotune(paradmm_batch* b, int iters, paradmm_batch
         hipError_t e = T == 32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_solve<32>, 256, 0)
                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_solve<64>, 256, 0);
         if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor", __FILE__, __LINE__);
-        for (int bpc = 1; bpc <= per_cu && bpc <= 4; ++bpc) {
-            RC(paradmm_batch_configure(b, T, bpc));
-            RC(batch_upload(b));
-            float ms = 0.0f;
-            RC(batch_launch(b, &o, &ms));  // warm
-            RC(batch_upload(b));
-            RC(batch_launch(b, &o, &ms));
-            const double per = ms / iters;
-            if (tried && n < max_tried) tried[n] = paradmm_batch_config{T, bpc, per};
-            ++n;
-            if (best.tile == 0 || per < best.ms_per_iter) best = paradmm_batch_config{T, bpc, per};
-        }
+        bool uploaded = false;
+        for (int bpc = 1; bpc <= per_cu && bpc <= 4; ++bpc)
+            for (int bar = 0; bar < 2; ++bar) {
+                RC(paradmm_batch_configure(b, T, bpc, bar));
+                if (!uploaded) RC(batch_upload(b));
+                uploaded = true;
+                float ms = 0.0f;
+                RC(batch_reset_flags(b));
+                RC(batch_launch(b, &o, &ms));  // warm
+                RC(batch_reset_flags(b));      // the last check retired every instance
+                RC(batch_launch(b, &o, &ms));
+                const double per = ms / iters;
+                if (tried && n < max_tried) tried[n] = paradmm_batch_config{T, bpc, bar, per};
+                ++n;
+                if (best.tile == 0 || per < best.ms_per_iter) best = paradmm_batch_config{T, bpc, bar, per};
+            }
     }
     if (ntried) *ntried = n;
-    RC(paradmm_batch_configure(b, best.tile, best.blocks_per_cu));
+    RC(paradmm_batch_configure(b, best.tile, best.blocks_per_cu, best.barrier));
     if (chosen) *chosen = best;
     return PARADMM_OK;
 }
@@ -634,7 +675,7 @@ extern "C" int paradmm_batch_solve(paradmm_batch* b, const paradmm_solve_opts* o
         pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_solve: the batch runs the plain fixed-rho method (policy_every 0)");
         return PARADMM_ERR_ARG;
     }
-    if (!b->T) RC(paradmm_batch_configure(b, 32, 1));
+    if (!b->T) RC(paradmm_batch_configure(b, 64, 1, 1));
     RC(batch_upload(b));
     float ms = 0.0f;
     RC(batch_launch(b, o, &ms));

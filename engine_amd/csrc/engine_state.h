@@ -83,6 +83,8 @@ struct DevGraph {
     // iter_frac = iterations done / max_iters.  Set by the learned-rho module.
     int (*rho_policy)(DevGraph* s, float iter_frac) = nullptr;
     void* policy_ctx = nullptr;
+    bool policy_per_type = false;      // apply the per-type mean of the network's log10 rho
+    void* d_policy_part = nullptr;     // per-type partial sums of that mean
 
     // kernel timing (paradmm_gpu_profile_kernels): eager launches only
     bool prof = false;

@@ -702,9 +702,31 @@ __global__ __launch_bounds__(256) void k_rho_features(PolicyArgs a)
     for (int k = 0; k < MLP_NIN; ++k) a.feat_out[(long)e * MLP_NIN + k] = f[k];
 }
 
-// rho_e <- 10^clamp(MLP(features_e)), U_e <- U_e rho_old / rho_new.  Each
-// wave takes 64 edges per step (one per lane) and pushes them through the
-// network as four 16-row MFMA tiles.
+// Network output of 64 edges per wave-step (one per lane), pushed through
+// the network as four 16-row MFMA tiles; y = clamped log10 rho per lane.
+__device__ __forceinline__ float policy_wave_eval(const MlpShared& w, TileScratch& ts, float (&yv)[64], const float (&f)[MLP_NIN],
+                                                  const int l)
+{
+    for (int t = 0; t < 4; ++t) {
+        // rows of tile t are lanes 16t .. 16t+15
+        if ((l >> 4) == t) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) ts.f[l & 15][k] = k < MLP_NIN ? f2bf(f[k]) : (short)0;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        v4f z1[2], z2[2];
+        tile_forward(w, ts, z1, z2);
+        if (l < 16) yv[16 * t + l] = ts.y[l];
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+    }
+    const float y = fminf(fmaxf(yv[l], MLP_LOG10_RHO_MIN), MLP_LOG10_RHO_MAX);
+    __builtin_amdgcn_wave_barrier();
+    return y;
+}
+
+// Per-edge mode: rho_e <- 10^y_e, U_e <- U_e rho_old / rho_new.
 __global__ __launch_bounds__(256) void k_rho_policy(PolicyArgs a, const float* __restrict__ params)
 {
     __shared__ MlpShared w;
@@ -722,29 +744,97 @@ __global__ __launch_bounds__(256) void k_rho_policy(PolicyArgs a, const float* _
         int type = 0, p = 0;
         if (valid) edge_residuals(a, (int)e, r, s, rho, type, p);
         edge_features(r, s, rho, type, a.frac, f);
-        for (int t = 0; t < 4; ++t) {
-            // rows of tile t are lanes 16t .. 16t+15
-            if ((l >> 4) == t) {
-#pragma unroll
-                for (int k = 0; k < 16; ++k) ts.f[l & 15][k] = k < MLP_NIN ? f2bf(f[k]) : (short)0;
-            }
-            __builtin_amdgcn_s_waitcnt(0);
-            __builtin_amdgcn_wave_barrier();
-            v4f z1[2], z2[2];
-            tile_forward(w, ts, z1, z2);
-            if (l < 16) yv[wv][16 * t + l] = ts.y[l];
-            __builtin_amdgcn_s_waitcnt(0);
-            __builtin_amdgcn_wave_barrier();
-        }
+        const float y = policy_wave_eval(w, ts, yv[wv], f, l);
         if (valid) {
-            const float y = fminf(fmaxf(yv[wv][l], MLP_LOG10_RHO_MIN), MLP_LOG10_RHO_MAX);
             const float rn = exp10f(y);
             const float scale = rho / rn;
             for (int k = 0; k < a.D; ++k) a.U[k * a.stride + p] *= scale;
             a.rho[p] = rn;
         }
-        __builtin_amdgcn_wave_barrier();
     }
+}
+
+// Per-type mode, pass 1: the network's log10 rho of every edge, summed per
+// factor type (pair, container, radius) into this workgroup's FP64 partial
+// slots part[block][3] and counts cnt[block][3].
+__global__ __launch_bounds__(256) void k_rho_policy_type_sums(PolicyArgs a, const float* __restrict__ params,
+                                                              double* __restrict__ part, int* __restrict__ cnt)
+{
+    __shared__ MlpShared w;
+    __shared__ TileScratch tsw[4];
+    __shared__ float yv[4][64];
+    __shared__ double ssum[4][3];
+    __shared__ int scnt[4][3];
+    stage_weights(params, w);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    TileScratch& ts = tsw[wv];
+    double acc[3] = {0.0, 0.0, 0.0};
+    int num[3] = {0, 0, 0};
+    const long nchunks = ((long)a.E + 63) / 64;
+    for (long ch = (long)blockIdx.x * 4 + wv; ch < nchunks; ch += (long)gridDim.x * 4) {
+        const long e = ch * 64 + l;
+        const bool valid = e < a.E;
+        float r = 0.0f, s = 0.0f, rho = 1.0f, f[MLP_NIN];
+        int type = 0, p = 0;
+        if (valid) edge_residuals(a, (int)e, r, s, rho, type, p);
+        edge_features(r, s, rho, type, a.frac, f);
+        const float y = policy_wave_eval(w, ts, yv[wv], f, l);
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                acc[k] += k == type ? (double)y : 0.0;
+                num[k] += k == type ? 1 : 0;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        for (int o = 32; o > 0; o >>= 1) {
+            acc[k] += __shfl_xor(acc[k], o, 64);
+            num[k] += __shfl_xor(num[k], o, 64);
+        }
+    }
+    if (l == 0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            ssum[wv][k] = acc[k];
+            scnt[wv][k] = num[k];
+        }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        part[blockIdx.x * 3 + k] = (ssum[0][k] + ssum[1][k]) + (ssum[2][k] + ssum[3][k]);
+        cnt[blockIdx.x * 3 + k] = scnt[0][k] + scnt[1][k] + scnt[2][k] + scnt[3][k];
+    }
+}
+
+// Per-type mode, pass 2: mean log10 rho per type over the partials in fixed
+// order; rho_type = 10^mean (types without edges keep 0 and are unused).
+__global__ void k_rho_policy_type_final(int nblocks, const double* __restrict__ part, const int* __restrict__ cnt,
+                                        float* __restrict__ rho_type)
+{
+    const int k = threadIdx.x;
+    if (k >= 3) return;
+    double sum = 0.0;
+    long n = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        sum += part[b * 3 + k];
+        n += cnt[b * 3 + k];
+    }
+    rho_type[k] = n ? exp10f((float)(sum / (double)n)) : 0.0f;
+}
+
+// Per-type mode, pass 3: rho_e <- rho_type(e), U_e <- U_e rho_old / rho_new.
+__global__ __launch_bounds__(256) void k_rho_policy_type_apply(PolicyArgs a, const float* __restrict__ rho_type)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.E) return;
+    const int p = a.pos[e];
+    const float rn = rho_type[a.type[e]];
+    const float scale = a.rho[p] / rn;
+    for (int k = 0; k < a.D; ++k) a.U[k * a.stride + p] *= scale;
+    a.rho[p] = rn;
 }
 
 // uniform rho multiplier with the scaled dual rescaled: rho *= f, U /= f
@@ -793,7 +883,20 @@ static int apply_policy(DevGraph* s, float frac)
     const long nchunks = ((long)s->E + 63) / 64;
     const unsigned grid = (unsigned)std::min<long>((nchunks + 3) / 4, 8192);
     prof_begin(s, "rho_policy");
-    k_rho_policy<<<grid, 256, 0, s->stream>>>(a, mlp_device_params(m));
+    if (!s->policy_per_type) {
+        k_rho_policy<<<grid, 256, 0, s->stream>>>(a, mlp_device_params(m));
+    } else {
+        if (!s->d_policy_part) {
+            // partial sums [grid][3] (FP64), counts [grid][3], rho per type [3]
+            RC(dmalloc(s, (void**)&s->d_policy_part, sizeof(double) * 3 * 8192 + sizeof(int) * 3 * 8192 + 16));
+        }
+        double* part = (double*)s->d_policy_part;
+        int* cnt = (int*)(part + 3 * 8192);
+        float* rho_type = (float*)(cnt + 3 * 8192);
+        k_rho_policy_type_sums<<<grid, 256, 0, s->stream>>>(a, mlp_device_params(m), part, cnt);
+        k_rho_policy_type_final<<<1, 64, 0, s->stream>>>((int)grid, part, cnt, rho_type);
+        k_rho_policy_type_apply<<<(unsigned)((s->E + 255) / 256), 256, 0, s->stream>>>(a, rho_type);
+    }
     prof_end(s);
     CK(hipGetLastError());
     s->rho_device_written = true;
@@ -808,6 +911,14 @@ extern "C" int paradmm_gpu_set_rho_policy(graph* G, paradmm_rho_mlp* m)
     RC(pdm_get_state(G, &s, true));
     s->policy_ctx = m;
     s->rho_policy = m ? &apply_policy : nullptr;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_gpu_set_rho_policy_mode(graph* G, int per_type)
+{
+    DevGraph* s;
+    RC(pdm_get_state(G, &s, true));
+    s->policy_per_type = per_type != 0;
     return PARADMM_OK;
 }
 

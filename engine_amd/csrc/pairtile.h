@@ -160,6 +160,17 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
         const int pa = base + il * T + c0;  // 32-bit lane offsets off uniform bases
         const int pb = pa + G::TT;
 
+        float racc[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        // squared primal differences of this row's 4 pairs (<= 24 terms) in
+        // fp32; the accumulation across rows, threads and tiles is FP64
+        float prow = 0.0f;
+        // a lane whose 4 slots hold no pair edge (columns past N, the lower
+        // half of a diagonal tile) reads and writes nothing: padding costs
+        // no HBM traffic
+        bool on = true;
+        if constexpr (VAR == 1) on = J * T + c0 < N;
+        if constexpr (VAR == 2) on = (c0 + 3 > il) && (J * T + max(c0, il + 1) < N);
+        if (on) {
         float4 xa[3], xb[3], ua[3], ub[3], na[3], nbv[3], ma[3], mb[3];
         float4 ra, rb, aa, ab;
         ra = ld4(a.rho + pa);
@@ -173,10 +184,6 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
             if constexpr (PART_M) { ma[k] = ld4((a.Mb + k * S) + pa); mb[k] = ld4((a.Mb + k * S) + pb); }
         }
 
-        float racc[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        // squared primal differences of this row's 4 pairs (<= 24 terms) in
-        // fp32; the accumulation across rows, threads and tiles is FP64
-        float prow = 0.0f;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int jl = c0 + c;
@@ -249,12 +256,13 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
                 if constexpr (WRITE_U) { cmp(ua[k], c) = uA[k]; cmp(ub[k], c) = uB[k]; }
             }
         }
-        if constexpr (APPLY_U) pr += (double)prow;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             if constexpr (WRITE_X) { st4((a.X + k * S) + pa, xa[k]); st4((a.X + k * S) + pb, xb[k]); }
             if constexpr (WRITE_U) { st4((a.U + k * S) + pa, ua[k]); st4((a.U + k * S) + pb, ub[k]); }
         }
+        }  // on
+        if constexpr (APPLY_U) pr += (double)prow;
         if constexpr (PART) {
 #pragma unroll
             for (int q = 0; q < 5; ++q) {

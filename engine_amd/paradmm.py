@@ -62,7 +62,7 @@ class SolveResult(C.Structure):
 
 class BatchConfig(C.Structure):
     """paradmm_batch_config of paradmm_gpu.h"""
-    _fields_ = [("tile", C.c_int), ("blocks_per_cu", C.c_int), ("ms_per_iter", C.c_double)]
+    _fields_ = [("tile", C.c_int), ("blocks_per_cu", C.c_int), ("barrier", C.c_int), ("ms_per_iter", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -124,9 +124,10 @@ def load():
         "paradmm_gpu_square_quality": (C.c_int, [G, _DP, _DP, _DP]),
         "paradmm_rho_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), _FP, C.c_int, C.c_uint64]),
         "paradmm_rho_mlp_destroy": (None, [V]),
+        "paradmm_gpu_set_rho_policy_mode": (C.c_int, [G, C.c_int]),
         "paradmm_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(G), C.c_int]),
         "paradmm_batch_destroy": (None, [V]),
-        "paradmm_batch_configure": (C.c_int, [V, C.c_int, C.c_int]),
+        "paradmm_batch_configure": (C.c_int, [V, C.c_int, C.c_int, C.c_int]),
         "paradmm_batch_autotune": (C.c_int, [V, C.c_int, C.POINTER(BatchConfig), C.POINTER(BatchConfig), C.c_int,
                                              _IP]),
         "paradmm_batch_solve": (C.c_int, [V, C.POINTER(SolveOpts), C.POINTER(SolveResult), _DP]),
@@ -357,6 +358,9 @@ class Graph:
     def set_rho_policy(self, mlp):
         check(self.lib.paradmm_gpu_set_rho_policy(self.ptr, mlp.h if mlp is not None else None), "set_rho_policy")
 
+    def set_rho_policy_mode(self, per_type):
+        check(self.lib.paradmm_gpu_set_rho_policy_mode(self.ptr, 1 if per_type else 0), "set_rho_policy_mode")
+
     def apply_rho_policy(self, mlp, iter_frac):
         check(self.lib.paradmm_gpu_apply_rho_policy(self.ptr, mlp.h, iter_frac), "apply_rho_policy")
 
@@ -423,14 +427,14 @@ class Batch:
         check(self.lib.paradmm_batch_create(C.byref(h), arr, len(self.graphs)), "paradmm_batch_create")
         self.h = h
 
-    def configure(self, tile, blocks_per_cu=1):
-        check(self.lib.paradmm_batch_configure(self.h, tile, blocks_per_cu), "paradmm_batch_configure")
+    def configure(self, tile, blocks_per_cu=1, barrier=1):
+        check(self.lib.paradmm_batch_configure(self.h, tile, blocks_per_cu, barrier), "paradmm_batch_configure")
 
     def autotune(self, iters=20):
         chosen = BatchConfig()
-        tried = (BatchConfig * 16)()
+        tried = (BatchConfig * 32)()
         n = C.c_int()
-        check(self.lib.paradmm_batch_autotune(self.h, iters, C.byref(chosen), tried, 16, C.byref(n)),
+        check(self.lib.paradmm_batch_autotune(self.h, iters, C.byref(chosen), tried, 32, C.byref(n)),
               "paradmm_batch_autotune")
         return chosen.as_dict(), [tried[i].as_dict() for i in range(n.value)]
 

@@ -8,11 +8,12 @@ Training data come from the plain solver itself:
    of a grid; the teacher's rho* is the fastest to converge (ties and
    near-ties within 10% broken by the larger objective sum r); if none
    converges, the rho with the smallest final max(primal, dual, overlap).
-2. Rows.  From several starting rho (1, 10, rho* and a large one) the
-   solver runs `horizon` iterations; every `every` iterations the per-edge
-   features of paradmm_gpu_rho_features are sampled, each with target
-   log10(rho*).  The network thus learns which rho to move to from the
-   dynamics it observes, wherever it starts.
+2. Rows.  From several starting rho (1, 10, rho* and 4 rho*) the solver
+   runs the whole iteration budget; at snapshot iterations spread over it
+   (dense early, SNAPSHOTS) the per-edge features of
+   paradmm_gpu_rho_features are sampled, each with target log10(rho*).  The
+   network thus learns which rho to move to from the dynamics it observes,
+   wherever it starts and at any fraction of the budget.
 3. Training with the library's own bf16-MFMA forward/backward and Adam.
 
 Applied inside paradmm_gpu_solve every `policy_every` iterations, the
@@ -29,15 +30,26 @@ from . import paradmm as P
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "models", "rho_policy.f32")
-RHO_GRID = (20.0, 35.0, 50.0, 70.0, 100.0, 150.0, 200.0, 300.0)
+RHO_GRID = (20.0, 35.0, 50.0, 70.0, 100.0, 150.0, 200.0, 300.0, 500.0, 800.0, 1200.0)
+# snapshot iterations of the training trajectories: dense early, then spread
+# over the whole budget so the iteration-fraction input is covered on [0, 1]
+SNAPSHOTS = tuple(range(50, 1001, 50)) + (1500, 2000, 3000, 4000, 6000, 8000, 10000, 13000, 16000, 20000)
 
 
-def solve_once(n, seed, rho, policy=None, max_iters=20000, tol=1e-4, check_every=50, policy_every=50):
+# rho of the first policy_every iterations of a learned solve: a start above
+# any instance's optimum (a start far below it drives the packing into
+# overlaps it does not leave), which the first application then lowers
+LEARNED_START_RHO = 1000.0
+
+
+def solve_once(n, seed, rho, policy=None, max_iters=20000, tol=1e-4, check_every=50, policy_every=50,
+               per_type=False):
     """One solve to tolerance; policy None is the plain fixed-rho method."""
     g = P.packing_square(n, seed, rho=rho, alpha=1.0)
     g.prepare()
     if policy is not None:
         g.set_rho_policy(policy)
+        g.set_rho_policy_mode(per_type)
     r = g.solve(max_iters, tol=tol, tol_feas=tol, check_every=check_every,
                 policy_every=policy_every if policy is not None else 0)
     g.close()
@@ -58,23 +70,29 @@ def best_fixed_rho(n, seed, grid=RHO_GRID, max_iters=20000, tol=1e-4):
     return best, runs
 
 
-def generate(instances, starts=(1.0, 10.0, None, 400.0), horizon=1000, every=50, max_iters=20000,
-             rows_per_snapshot=1000, rng_seed=0, log=print):
-    """Rows of (features, target) for the given (n, seed) instances."""
+STARTS = (("abs", LEARNED_START_RHO), ("rel", 1.0), ("rel", 0.7), ("rel", 4.0))
+
+
+def generate(instances, starts=STARTS, snapshots=SNAPSHOTS, max_iters=20000,
+             rows_per_snapshot=1000, rng_seed=0, log=print, grid=RHO_GRID):
+    """Rows of (features, target) for the given (n, seed) instances.  Each
+    start is ("abs", rho) or ("rel", multiple of the teacher's rho*)."""
     rng = np.random.default_rng(rng_seed)
     feats, targets, teacher = [], [], {}
     for n, seed in instances:
         t0 = time.time()
-        rho_star, runs = best_fixed_rho(n, seed, max_iters=max_iters)
+        rho_star, runs = best_fixed_rho(n, seed, grid=grid, max_iters=max_iters)
         teacher[f"{n}/{seed}"] = {"rho_star": rho_star,
                                   "iterations": {str(k): (v["iterations"] if v["converged"] else None)
                                                  for k, v in runs.items()}}
-        for start in starts:
-            rho0 = rho_star if start is None else start
+        for kind, v in starts:
+            rho0 = v * rho_star if kind == "rel" else v
             g = P.packing_square(int(n), int(seed), rho=rho0, alpha=1.0)
             g.prepare()
-            for it in range(every, horizon + 1, every):
-                g.iterate(every)
+            done = 0
+            for it in snapshots:
+                g.iterate(it - done)
+                done = it
                 f = g.rho_features(it / max_iters)
                 take = rng.choice(f.shape[0], size=min(rows_per_snapshot, f.shape[0]), replace=False)
                 feats.append(f[take])
@@ -107,13 +125,15 @@ def load_policy(path=DEFAULT_WEIGHTS):
 
 
 def compare(n, seeds, policy, fixed=(1.0, 50.0, 100.0), max_iters=20000, log=print):
-    """Time to tolerance of fixed rho values and of the learned policy (from rho = 1)."""
+    """Time to tolerance of fixed rho values and of the learned policy (per
+    edge and per type, from LEARNED_START_RHO)."""
     out = []
     for seed in seeds:
         row = {"n": n, "seed": seed}
         for rho in fixed:
             row[f"fixed{rho:g}"] = solve_once(n, seed, rho, None, max_iters=max_iters)
-        row["learned"] = solve_once(n, seed, 1.0, policy, max_iters=max_iters)
+        row["learned"] = solve_once(n, seed, LEARNED_START_RHO, policy, max_iters=max_iters)
+        row["learned_type"] = solve_once(n, seed, LEARNED_START_RHO, policy, max_iters=max_iters, per_type=True)
         out.append(row)
         log(f"# N={n} seed={seed}: " + ", ".join(
             f"{k} {'%d it %.0f ms' % (v['iterations'], v['elapsed_ms']) if v['converged'] else 'not converged'}"

@@ -171,6 +171,12 @@ int paradmm_gpu_rho_features(graph *G, float iter_frac, float *feat);
 int paradmm_gpu_set_rho_policy(graph *G, paradmm_rho_mlp *m);
 /* apply the policy once now */
 int paradmm_gpu_apply_rho_policy(graph *G, paradmm_rho_mlp *m, float iter_frac);
+/* per_type = 0 (default): each edge takes the network's own output.
+ * per_type = 1: each edge takes 10^(mean of the network's log10 rho over the
+ * edges of its factor type: pair, container, radius) -- per-edge inference,
+ * per-type application, for problems where uneven rho across one type slows
+ * the consensus. */
+int paradmm_gpu_set_rho_policy_mode(graph *G, int per_type);
 /* rho_e *= factor on every edge, U_e /= factor (uniform change of rho) */
 int paradmm_gpu_scale_rho(graph *G, double factor);
 /* save / restore the device iterate (X, U, rho, Z, Zprev): counterfactual rollouts */
@@ -197,16 +203,18 @@ typedef struct paradmm_batch paradmm_batch;
 typedef struct {
     int tile;            /* pair-tile side: 32 or 64 circles */
     int blocks_per_cu;   /* persistent workgroups per compute unit */
+    int barrier;         /* grid barrier: 0 cooperative-groups sync, 1 two-level (bounded wait) */
     double ms_per_iter;  /* measured by the autotuner */
 } paradmm_batch_config;
 
 int paradmm_batch_create(paradmm_batch **out, graph *const *graphs, int count);
 void paradmm_batch_destroy(paradmm_batch *b);
-/* Fixes the tile side and grid (default on first solve: tile 32, 1 block/CU). */
-int paradmm_batch_configure(paradmm_batch *b, int tile, int blocks_per_cu);
+/* Fixes the tile side, grid and barrier (default on first solve: tile 64,
+ * 1 block/CU, two-level barrier). */
+int paradmm_batch_configure(paradmm_batch *b, int tile, int blocks_per_cu, int barrier);
 /* Times `iters` iterations of every candidate (tile 32/64 x blocks per CU up
- * to the kernel's occupancy) from the graphs' state and configures the
- * fastest; *chosen gets it, tried[0..max_tried) the candidates measured. */
+ * to the kernel's occupancy x barrier kind) and configures the fastest;
+ * *chosen gets it, tried[0..max_tried) the candidates measured. */
 int paradmm_batch_autotune(paradmm_batch *b, int iters, paradmm_batch_config *chosen,
                            paradmm_batch_config *tried, int max_tried, int *ntried);
 /* Uploads every graph's host state (X, U, N, Z, RHOS, ALPHAS) and solves;

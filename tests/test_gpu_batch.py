@@ -157,8 +157,12 @@ def test_batch_autotune_and_rerun():
     b = P.Batch(gs)
     chosen, tried = b.autotune(10)
     print("autotune:", chosen, tried)
-    assert chosen in tried and len(tried) >= 2
-    assert {t["tile"] for t in tried} == {32, 64}
+    assert chosen in tried and len(tried) >= 4
+    assert {t["tile"] for t in tried} == {32, 64} and {t["barrier"] for t in tried} == {0, 1}
+    # every candidate timed real iterations: none is faster than the
+    # chosen configuration's own fixed-iteration run by a large factor
+    _, ms_fixed = b.solve(10, tol=-1.0, tol_feas=-1.0, check_every=1000)
+    assert chosen["ms_per_iter"] > 0.3 * ms_fixed / 10
     info = b.info()
     assert info["tile"] == chosen["tile"] and info["blocks_per_cu"] == chosen["blocks_per_cu"]
     # a solve after tuning starts from the graphs' state again
@@ -192,3 +196,22 @@ def test_batch_rejects_other_graphs():
     b = P.Batch([P.packing_square(20, 0), g])
     with pytest.raises(P.ParadmmError, match="not an all-pairs unit-square packing"):
         b.configure(32, 1)
+
+
+def test_batch_barrier_kinds_agree():
+    """The cooperative-groups grid sync and the two-level barrier give the
+    same iterations bit for bit (the barrier only orders the phases), with
+    one and with two workgroups per compute unit."""
+    sizes = [16, 40, 100, 130]
+    zs = []
+    for bpc, barrier in ((1, 0), (1, 1), (2, 1)):
+        gs = _instances(sizes, 20.0)
+        b = P.Batch(gs)
+        b.configure(64, bpc, barrier)
+        res, ms = b.solve(60, tol=-1.0, tol_feas=-1.0, check_every=25)
+        b.sync_to_host()
+        zs.append([g.array("Z").copy() for g in gs])
+        print(f"bpc {bpc} barrier {barrier}: {ms:.2f} ms for 60 iterations")
+    for other in zs[1:]:
+        for a, c in zip(zs[0], other):
+            assert np.array_equal(a, c)

@@ -110,6 +110,31 @@ def test_policy_application_matches_network():
     assert np.allclose(U1, U0 * (rho / rho_new)[:, None], rtol=1e-5, atol=1e-7)
 
 
+def test_policy_per_type_mode_applies_type_means():
+    n = 48
+    g = P.packing_square(n, 6, rho=20.0, alpha=1.0)
+    g.iterate(40)
+    p = (mo.init_params(3) * 0.3).astype(np.float32)
+    m = P.RhoMLP(params=p)
+    feat = g.rho_features(0.1)
+    g.sync_to_host()
+    rho = g.array("RHOS").copy()
+    U0 = g.array("U").reshape(-1, 3).copy()
+    y = np.clip(m.forward(feat).astype(np.float64), -1.0, 4.0)
+    typ = np.zeros(g.nedge, int)
+    typ[:n] = 2
+    typ[n:2 * n] = 1
+    g.set_rho_policy_mode(True)
+    g.apply_rho_policy(m, 0.1)
+    g.sync_to_host()
+    rho_new = g.array("RHOS")
+    for t in (0, 1, 2):
+        expect = 10.0 ** y[typ == t].mean()
+        assert np.allclose(rho_new[typ == t], expect, rtol=1e-5)
+    U1 = g.array("U").reshape(-1, 3)
+    assert np.allclose(U1, U0 * (rho / rho_new)[:, None], rtol=1e-5, atol=1e-7)
+
+
 def test_state_save_restore_and_scale():
     g = P.packing_square(50, 1, rho=10.0)
     g.iterate(10)
