@@ -10,7 +10,7 @@
 //   circles[c0_b, c0_b + N_b)           Z, Zprev, radius exponents
 //   tiles  [t0_b, t0_b + nt_b)          (instance, I, J), FP64 primal and
 //                                       quality partials
-//   partials P + p0_b: [5][N_b][nb_b]   per-block sums for the Z consensus
+//   partials P + p0_b: [5][nb_b][N_b]   per-block sums for the Z consensus
 //
 // k_batch_solve is a cooperative (co-resident) grid that loops over
 // iterations with grid-wide barriers between phases:
@@ -217,7 +217,7 @@ __device__ void phase_circles(const BArgs& a, const BLists& L)
         float s[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         for (int K = 0; K < in.nb; ++K)
 #pragma unroll
-            for (int q = 0; q < 5; ++q) s[q] += P[(q * N + c) * in.nb + K];
+            for (int q = 0; q < 5; ++q) s[q] += P[((long)q * in.nb + K) * N + c];
         float* Z = a.Z + 3L * (in.c0 + c);
         float* Zp = a.Zprev + 3L * (in.c0 + c);
         double dz2 = 0.0;
@@ -390,7 +390,7 @@ static int batch_layout(paradmm_batch* b, int T)
     for (int i = 0; i < b->ninst; ++i) {
         graph* G = b->graphs[i];
         std::vector<int> fk;
-        RC(pdm_factor_kinds(G, fk));
+        RC(pdm_factor_kinds(G, fk, false));
         std::vector<float> ae;
         int nb = 0, nt = 0;
         const int rc = dense_structure(G, fk, T, b->pos[i], ae, nb, nt);

@@ -49,7 +49,7 @@ constexpr int DTT = DT * DT;  // edge slots per sub-block
 struct DenseState {
     int N = 0, nb = 0, nt = 0;
     int2* d_tiles = nullptr;  // [nt] (I, J)
-    float* d_P = nullptr;     // [5][N][nb] partial sums
+    float* d_P = nullptr;     // [5][nb][N] partial sums
     double* d_Rp = nullptr;   // [nt] primal residual partials
     double* d_Rd = nullptr;   // [N] dual residual partials
     float* d_aexp = nullptr;  // [N] radius exponent per circle
@@ -66,44 +66,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDM_PAIRTIL
     tile_pass<DT, MODE, false>(a, t, ij.x, ij.y, sh, a.Rp + t, nullptr);
 }
 
-__device__ __forceinline__ float wsum64(float v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Z_c = sum rho M / sum rho over the circle's nb block partials; one
-// wavefront per circle, lanes over blocks in fixed order.
+// Z_c = sum rho M / sum rho over the circle's nb block partials.  A
+// workgroup takes 32 circles; its 8 slices of 32 threads sum the blocks
+// K = slice, slice + 8, ... (coalesced rows of P[q][K][.]), then the slices
+// are added in fixed order.
 __global__ __launch_bounds__(256) void k_dense_zreduce(int N, int nb, const float* __restrict__ P, float* __restrict__ Z,
                                                        float* __restrict__ Zprev, double* __restrict__ Rd)
 {
-    const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (c >= N) return;
+    __shared__ float part[8][5][32];
+    const int ks = threadIdx.x >> 5, cl = threadIdx.x & 31;
+    const int c = blockIdx.x * 32 + cl;
     float s[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    for (int K = lane; K < nb; K += 64)
+    if (c < N)
+        for (int K = ks; K < nb; K += 8)
 #pragma unroll
-        for (int q = 0; q < 5; ++q) s[q] += P[((long)q * N + c) * nb + K];
+            for (int q = 0; q < 5; ++q) s[q] += P[((long)q * nb + K) * N + c];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) s[q] = wsum64(s[q]);
+    for (int q = 0; q < 5; ++q) part[ks][q][cl] = s[q];
+    __syncthreads();
+    if (threadIdx.x >= 32 || c >= N) return;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        float t = part[0][q][cl];
+        for (int k = 1; k < 8; ++k) t += part[k][q][cl];
+        s[q] = t;
+    }
     double dz2 = 0.0;
-    float z[3], zo[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        z[k] = s[k] / s[3];
-        zo[k] = Z[3 * c + k];
-        const double d = (double)(z[k] - zo[k]);
+        const float z = s[k] / s[3];
+        const float zo = Z[3 * c + k];
+        const double d = (double)(z - zo);
         dz2 += d * d;
+        Z[3 * c + k] = z;
+        Zprev[3 * c + k] = zo;
     }
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            Z[3 * c + k] = z[k];
-            Zprev[3 * c + k] = zo[k];
-        }
-        Rd[c] = (double)s[4] * dz2;
-    }
+    Rd[c] = (double)s[4] * dz2;
 }
 
 // primal = sum Rp[0..nt), dual = sum Rd[0..N): one workgroup, fixed order.
@@ -304,7 +302,7 @@ static int launch_zreduce(DevGraph* s)
 {
     DenseState* d = s->dense;
     prof_begin(s, "zreduce");
-    k_dense_zreduce<<<(unsigned)((d->N + 3) / 4), 256, 0, s->stream>>>(d->N, d->nb, d->d_P, s->d_Z, s->d_Zprev, d->d_Rd);
+    k_dense_zreduce<<<(unsigned)((d->N + 31) / 32), 256, 0, s->stream>>>(d->N, d->nb, d->d_P, s->d_Z, s->d_Zprev, d->d_Rd);
     prof_end(s);
     CK(hipGetLastError());
     return PARADMM_OK;

@@ -458,6 +458,11 @@ void free_dev(DevGraph* s)
     if (!s) return;
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     if (s->ghandle) (void)hipGraphDestroy(s->ghandle);
+    if (s->host) {
+        if (s->host->h_buf) (void)hipHostFree(s->host->h_buf);
+        delete s->host;
+        s->host = nullptr;
+    }
     for (void* p : s->allocs) (void)hipFree(p);
     s->allocs.clear();
     for (hipEvent_t e : s->prof_pool) (void)hipEventDestroy(e);
@@ -562,25 +567,52 @@ static int finalize_generic(DevGraph* s, graph* G, const std::vector<int>& fkind
         }
         s->kinds.push_back(kt);
     }
-    (void)D;
+    // factors whose callback runs on the host
+    HostProx hp;
+    std::vector<int> hpos;
+    for (int f = 0; f < F; ++f) {
+        if (fkind[f] >= 0) continue;
+        const int fd = G->operatorix_to_numedges[f];
+        hp.fn.push_back((void*)G->list_of_operators_functions_pointers[f]);
+        hp.params.push_back(G->list_of_operators_param_pointers[f]);
+        hp.first.push_back((int)hpos.size());
+        hp.deg.push_back(fd);
+        if (fd > hp.maxdeg) hp.maxdeg = fd;
+        for (int q = 0; q < fd; ++q) hpos.push_back(pos[G->operatorix_to_edgelistix[f][q]]);
+    }
+    if (!hpos.empty()) {
+        HostProx* h = new HostProx(std::move(hp));
+        s->host = h;
+        h->nedges = (int)hpos.size();
+        const size_t nb = sizeof(float) * (size_t)h->nedges * (2 * D + 1);
+        RC(dmalloc(s, (void**)&h->d_pos, sizeof(int) * hpos.size()));
+        RC(dmalloc(s, (void**)&h->d_buf, nb));
+        CK(hipHostMalloc((void**)&h->h_buf, nb, hipHostMallocDefault));
+        CK(hipMemcpy(h->d_pos, hpos.data(), sizeof(int) * hpos.size(), hipMemcpyHostToDevice));
+    }
     return PARADMM_OK;
 }
 
 namespace pdm {
 // device operator kind of every factor, with degree / dimension checks
-int pdm_factor_kinds(graph* G, std::vector<int>& fkind)
+int pdm_factor_kinds(graph* G, std::vector<int>& fkind, bool allow_host)
 {
     const int F = G->num_opts, D = G->num_dims;
     fkind.assign(F, 0);
     for (int f = 0; f < F; ++f) {
         paradmm_prox_fn fn = (paradmm_prox_fn)G->list_of_operators_functions_pointers[f];
         const int K = paradmm_prox_kind_of(fn);
+        if (K == PARADMM_KIND_NONE && allow_host && fn) {
+            fkind[f] = -1;
+            continue;
+        }
         if (K == PARADMM_KIND_NONE) {
             Dl_info info;
             const char* nm = (dladdr((void*)fn, &info) && info.dli_sname) ? info.dli_sname : "?";
             pdm_set_error(PARADMM_ERR_KIND,
-                          "factor %d: callback %p (%s) is not a device prox kind; use the paradmm_prox_* operators, "
-                          "register it with paradmm_register_prox_kind, or link the caller with -rdynamic",
+                          "factor %d: callback %p (%s) is not a device prox kind and host callbacks are off "
+                          "(PARADMM_HOST_PROX=0); use the paradmm_prox_* operators, register it with "
+                          "paradmm_register_prox_kind, or link the caller with -rdynamic",
                           f, (void*)fn, nm);
             return PARADMM_ERR_KIND;
         }
@@ -610,7 +642,8 @@ static int finalize(graph* G, DevGraph** out)
         return PARADMM_ERR_STATE;
     }
     std::vector<int> fkind;
-    RC(pdm_factor_kinds(G, fkind));
+    const char* hp = getenv("PARADMM_HOST_PROX");
+    RC(pdm_factor_kinds(G, fkind, !(hp && hp[0] == '0')));
 
     DevGraph* s = new DevGraph();
     s->G = G;
@@ -796,6 +829,82 @@ static int launch_x_kind(DevGraph* s, const KindTable& kt, int from_n)
     return PARADMM_OK;
 }
 
+// Host-callback factors: gather (N, X, rho) of their edges, run the
+// callbacks in FP64 on the host in factor order, scatter (X, rho) back.
+__global__ __launch_bounds__(256) void k_host_gather(int n, const int* __restrict__ pos, const int* __restrict__ edge_var,
+                                                     const float* __restrict__ Z, const float* __restrict__ U,
+                                                     const float* __restrict__ Nbuf, const float* __restrict__ X,
+                                                     const float* __restrict__ rho, int from_n, int D, long stride,
+                                                     float* __restrict__ out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int p = pos[i];
+    const int v = edge_var[p];
+    float* o = out + (long)i * (2 * D + 1);
+    for (int k = 0; k < D; ++k) {
+        o[k] = from_n ? Nbuf[k * stride + p] : Z[v * D + k] - U[k * stride + p];
+        o[D + k] = X[k * stride + p];
+    }
+    o[2 * D] = rho[p];
+}
+
+__global__ __launch_bounds__(256) void k_host_scatter(int n, const int* __restrict__ pos, int D, long stride,
+                                                      const float* __restrict__ in, float* __restrict__ X,
+                                                      float* __restrict__ rho)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int p = pos[i];
+    const float* o = in + (long)i * (2 * D + 1);
+    for (int k = 0; k < D; ++k) X[k * stride + p] = o[D + k];
+    if (rho) rho[p] = o[2 * D];
+}
+
+static int run_host_prox(DevGraph* s, int from_n)
+{
+    HostProx* h = s->host;
+    const int D = s->D, n = h->nedges, W = 2 * D + 1;
+    prof_begin(s, "host_prox_gather");
+    k_host_gather<<<nblk(n), 256, 0, s->stream>>>(n, h->d_pos, s->d_edge_var_dev, s->d_Z, s->d_U, s->d_N, s->d_X,
+                                                  s->d_rho, from_n, D, s->stride, h->d_buf);
+    prof_end(s);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(h->h_buf, h->d_buf, sizeof(float) * (size_t)n * W, hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    std::vector<double> x((size_t)h->maxdeg * D), nn((size_t)h->maxdeg * D), r(h->maxdeg);
+    bool rho_changed = false;
+    for (size_t f = 0; f < h->fn.size(); ++f) {
+        const int deg = h->deg[f];
+        float* b = h->h_buf + (size_t)h->first[f] * W;
+        for (int q = 0; q < deg; ++q) {
+            for (int k = 0; k < D; ++k) {
+                nn[(size_t)q * D + k] = b[(size_t)q * W + k];
+                x[(size_t)q * D + k] = b[(size_t)q * W + D + k];
+            }
+            r[q] = b[(size_t)q * W + 2 * D];
+        }
+        ((paradmm_prox_fn)h->fn[f])(x.data(), nn.data(), r.data(), deg, D, h->params[f]);
+        for (int q = 0; q < deg; ++q) {
+            for (int k = 0; k < D; ++k) b[(size_t)q * W + D + k] = (float)x[(size_t)q * D + k];
+            const float rn = (float)r[q];
+            rho_changed = rho_changed || rn != b[(size_t)q * W + 2 * D];
+            b[(size_t)q * W + 2 * D] = rn;
+        }
+    }
+    CK(hipMemcpyAsync(h->d_buf, h->h_buf, sizeof(float) * (size_t)n * W, hipMemcpyHostToDevice, s->stream));
+    prof_begin(s, "host_prox_scatter");
+    k_host_scatter<<<nblk(n), 256, 0, s->stream>>>(n, h->d_pos, D, s->stride, h->d_buf, s->d_X,
+                                                   rho_changed ? s->d_rho : nullptr);
+    prof_end(s);
+    CK(hipGetLastError());
+    if (rho_changed) {
+        s->rho_device_written = true;
+        s->rho_changed = true;
+    }
+    return PARADMM_OK;
+}
+
 static int launch_xpass_generic(DevGraph* s, int from_n)
 {
     for (const KindTable& kt : s->kinds) {
@@ -809,6 +918,7 @@ static int launch_xpass_generic(DevGraph* s, int from_n)
         default: break;
         }
     }
+    if (s->host) RC(run_host_prox(s, from_n));
     return PARADMM_OK;
 }
 
@@ -930,7 +1040,8 @@ static int run_fused(DevGraph* s, int iters)
         --iters;
     }
     const int GI = s->graph_iters;
-    if (iters >= GI && GI > 1) {
+    // host-callback factors synchronise with the host every iteration: no capture
+    if (iters >= GI && GI > 1 && !s->host) {
         if (!s->gexec) {
             CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
             int rc = PARADMM_OK;
@@ -1128,6 +1239,13 @@ extern "C" int paradmm_gpu_sync_to_host(graph* G)
 {
     DevGraph* s;
     RC(get_state(G, &s, false));
+    return sync_down(s);
+}
+
+extern "C" int pdm_sync_if_device_newer(graph* G)
+{
+    DevGraph* s = lookup(G);
+    if (!s || !s->dev_newer) return PARADMM_OK;
     return sync_down(s);
 }
 

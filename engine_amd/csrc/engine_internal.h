@@ -31,6 +31,9 @@ int pdm_legacy_warm_start(graph *G, int maxiter);
 
 /* The host arrays were (re)allocated or rewritten by an initialiser. */
 void pdm_host_written(graph *G);
+/* Brings the host arrays up to date when the device holds newer state
+ * (no-op without device state). */
+int pdm_sync_if_device_newer(graph *G);
 /* Frees the device state of G (closeG). */
 void pdm_release_graph(graph *G);
 

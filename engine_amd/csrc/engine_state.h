@@ -26,6 +26,20 @@ struct KindTable {
 
 struct DenseState;  // dense.hip
 
+// Factors whose callback has no device kind: their prox runs on the host,
+// in FP64, between a device gather of (N, X, rho) and a scatter of (X, rho).
+struct HostProx {
+    std::vector<void*> fn;      // the callbacks (reference signature)
+    std::vector<void*> params;  // their parameter pointers
+    std::vector<int> first;     // first staged edge of each factor
+    std::vector<int> deg;       // its degree
+    int nedges = 0;
+    int maxdeg = 0;
+    int* d_pos = nullptr;       // [nedges] device positions, factor order
+    float* d_buf = nullptr;     // [nedges][2D+1]: N, X, rho
+    float* h_buf = nullptr;     // pinned host mirror
+};
+
 // One timed kernel launch: events recorded on the engine stream around it.
 struct ProfRec {
     const char* name;
@@ -78,6 +92,7 @@ struct DevGraph {
     hipGraphExec_t gexec = nullptr;
 
     DenseState* dense = nullptr;
+    HostProx* host = nullptr;  // generic layout only
 
     // rho policy applied by paradmm_gpu_solve every policy_every iterations;
     // iter_frac = iterations done / max_iters.  Set by the learned-rho module.
@@ -100,7 +115,9 @@ void prof_end(DevGraph* s);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 // device state of G, finalised; with upload, host writes since are uploaded
 int pdm_get_state(graph* G, DevGraph** out, bool upload);
-int pdm_factor_kinds(graph* G, std::vector<int>& fkind);
+// kind of every factor; with allow_host, a callback without a device kind
+// gets kind -1 (its prox runs on the host) instead of an error
+int pdm_factor_kinds(graph* G, std::vector<int>& fkind, bool allow_host);
 // the dense pair-tile slot of every edge for tile side T; -1 when the graph
 // is not the all-pairs unit-square packing
 int dense_structure(graph* G, const std::vector<int>& fkind, int T, std::vector<int>& pos, std::vector<float>& aexp,

@@ -16,6 +16,7 @@
 #define _DEFAULT_SOURCE
 #include "paradmm.h"
 #include "engine_internal.h"
+#include "paradmm_gpu.h"
 
 #include <stdint.h>
 #include <stdio.h>
@@ -331,4 +332,29 @@ void initialize_X_N_Z_M_U_optimal(graph *G, double Zinit[], double Uinit[], int 
     }
     pdm_host_written(G);
     if (maxiter > 0) pdm_legacy_warm_start(G, maxiter);
+}
+
+/* ------------------------------------------------------------ bulk construction */
+
+int paradmm_add_nodes(graph *G, void (*op)(double *, double *, double *, int, int, void *), void *params,
+                      size_t param_stride, int count, int nvars, const int *vars)
+{
+    /* `count` factors of one operator and degree: factor i has variables
+     * vars[i*nvars .. i*nvars+nvars) and parameters params + i*param_stride
+     * bytes (param_stride 0: all share params).  Same edge numbering as
+     * count addNode calls (core/paradmm.c:381-466). */
+    if (!G || !op || count < 0 || nvars < 1 || (count > 0 && !vars)) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_add_nodes: bad arguments");
+        return PARADMM_ERR_ARG;
+    }
+    for (int i = 0; i < count; i++)
+        for (int k = 0; k < nvars; k++)
+            if (vars[(size_t)i * nvars + k] < 0) {
+                pdm_set_error(PARADMM_ERR_ARG, "paradmm_add_nodes: factor %d has a negative variable index", i);
+                return PARADMM_ERR_ARG;
+            }
+    for (int i = 0; i < count; i++)
+        add_factor(G, op, params ? (char *)params + (size_t)i * param_stride : NULL, 0, nvars,
+                   vars + (size_t)i * nvars);
+    return PARADMM_OK;
 }

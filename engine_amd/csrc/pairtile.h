@@ -47,7 +47,7 @@ struct DArgs {
     const float* Nb;  // [3][stride]
     const float* Mb;  // [3][stride]
     const float* Z;   // [N][3]
-    float* P;         // [5][N][nb]
+    float* P;         // [5][nb][N] block partials
     double* Rp;       // [nt] primal partials (single-instance kernel)
     const float* aexp;
     const int2* tiles;
@@ -383,7 +383,7 @@ __device__ __forceinline__ double wsum64d(double v)
 }
 
 // One whole tile: pairs, the diagonal's unary factors, the circles' block
-// partials (P[q][c][K]: sum over the tile's edges of circle c in block K),
+// partials (P[q][K][c]: sum over the tile's edges of circle c in block K),
 // the tile's FP64 primal partial (*rp) and, with QUAL, its quality partials
 // q3 = {max pair overlap, max container violation, sum of radii}.
 // Called by all 256 threads of a workgroup; ends with a barrier-free
@@ -423,19 +423,20 @@ __device__ __forceinline__ void tile_pass(const DArgs& a, const int t, const int
     }
     __syncthreads();
     if constexpr (PART) {
+        // P[q][K][c]: the T partials of one block are contiguous
         const int N = a.N, nb = a.nb;
         if (I != J) {
             if (tid < T) {
                 const int i = I * T + tid;
                 if (i < N)
 #pragma unroll
-                    for (int q = 0; q < 5; ++q) a.P[((long)q * N + i) * nb + J] = sh.srow[q][tid];
+                    for (int q = 0; q < 5; ++q) a.P[((long)q * nb + J) * N + i] = sh.srow[q][tid];
             } else if (tid < 2 * T) {
                 const int jl = tid - T, j = J * T + jl;
                 if (j < N)
 #pragma unroll
                     for (int q = 0; q < 5; ++q)
-                        a.P[((long)q * N + j) * nb + I] =
+                        a.P[((long)q * nb + I) * N + j] =
                             sh.scol[0][q][jl] + sh.scol[1][q][jl] + sh.scol[2][q][jl] + sh.scol[3][q][jl];
             }
         } else if (tid < T) {
@@ -443,7 +444,7 @@ __device__ __forceinline__ void tile_pass(const DArgs& a, const int t, const int
             if (c < N)
 #pragma unroll
                 for (int q = 0; q < 5; ++q)
-                    a.P[((long)q * N + c) * nb + I] =
+                    a.P[((long)q * nb + I) * N + c] =
                         sh.srow[q][tid] + (sh.scol[0][q][tid] + sh.scol[1][q][tid] + sh.scol[2][q][tid] + sh.scol[3][q][tid]);
         }
     }

@@ -125,6 +125,9 @@ def load():
         "paradmm_rho_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), _FP, C.c_int, C.c_uint64]),
         "paradmm_rho_mlp_destroy": (None, [V]),
         "paradmm_gpu_set_rho_policy_mode": (C.c_int, [G, C.c_int]),
+        "paradmm_add_nodes": (C.c_int, [G, V, V, C.c_size_t, C.c_int, C.c_int, _IP]),
+        "paradmm_save_state": (C.c_int, [G, C.c_char_p]),
+        "paradmm_load_state": (C.c_int, [G, C.c_char_p]),
         "paradmm_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(G), C.c_int]),
         "paradmm_batch_destroy": (None, [V]),
         "paradmm_batch_configure": (C.c_int, [V, C.c_int, C.c_int, C.c_int]),
@@ -271,6 +274,25 @@ class Graph:
 
     def packing_square(self, ncircles, a_exp=1.0):
         check(self.lib.paradmm_packing_square(self.ptr, ncircles, a_exp), "paradmm_packing_square")
+
+    def add_factors(self, fn_addr, variables, params=None):
+        """Many factors of one operator and degree (paradmm_add_nodes):
+        variables is a (count, nvars) int array; params an optional
+        (count, k) float64 array, one row per factor."""
+        v = np.ascontiguousarray(variables, dtype=np.int32)
+        p, stride = None, 0
+        if params is not None:
+            pa = np.ascontiguousarray(params, dtype=np.float64)
+            self._keep.append(pa)
+            p, stride = pa.ctypes.data_as(C.c_void_p), pa.strides[0] if pa.ndim > 1 else 0
+        check(self.lib.paradmm_add_nodes(self.ptr, fn_addr, p, stride, v.shape[0], v.shape[1],
+                                         v.ctypes.data_as(_IP)), "paradmm_add_nodes")
+
+    def save_state(self, path):
+        check(self.lib.paradmm_save_state(self.ptr, os.fsencode(path)), "paradmm_save_state")
+
+    def load_state(self, path):
+        check(self.lib.paradmm_load_state(self.ptr, os.fsencode(path)), "paradmm_load_state")
 
     def init_rho_alpha(self, rho, alpha):
         self.lib.initialize_RHOS_APHAS(self.ptr, rho, alpha)

@@ -74,9 +74,11 @@ STARTS = (("abs", LEARNED_START_RHO), ("rel", 1.0), ("rel", 0.7), ("rel", 4.0))
 
 
 def generate(instances, starts=STARTS, snapshots=SNAPSHOTS, max_iters=20000,
-             rows_per_snapshot=1000, rng_seed=0, log=print, grid=RHO_GRID):
+             rows_per_snapshot=1000, rng_seed=0, log=print, grid=RHO_GRID, first_weight=1):
     """Rows of (features, target) for the given (n, seed) instances.  Each
-    start is ("abs", rho) or ("rel", multiple of the teacher's rho*)."""
+    start is ("abs", rho) or ("rel", multiple of the teacher's rho*).  The
+    first start's first three snapshots (the states a learned solve sees at
+    its first applications) get first_weight times the rows."""
     rng = np.random.default_rng(rng_seed)
     feats, targets, teacher = [], [], {}
     for n, seed in instances:
@@ -85,16 +87,17 @@ def generate(instances, starts=STARTS, snapshots=SNAPSHOTS, max_iters=20000,
         teacher[f"{n}/{seed}"] = {"rho_star": rho_star,
                                   "iterations": {str(k): (v["iterations"] if v["converged"] else None)
                                                  for k, v in runs.items()}}
-        for kind, v in starts:
+        for si, (kind, v) in enumerate(starts):
             rho0 = v * rho_star if kind == "rel" else v
             g = P.packing_square(int(n), int(seed), rho=rho0, alpha=1.0)
             g.prepare()
             done = 0
-            for it in snapshots:
+            for k, it in enumerate(snapshots):
                 g.iterate(it - done)
                 done = it
                 f = g.rho_features(it / max_iters)
-                take = rng.choice(f.shape[0], size=min(rows_per_snapshot, f.shape[0]), replace=False)
+                rows = rows_per_snapshot * (first_weight if si == 0 and k < 3 else 1)
+                take = rng.choice(f.shape[0], size=min(rows, f.shape[0]), replace=False)
                 feats.append(f[take])
                 targets.append(np.full(take.size, np.log10(rho_star), dtype=np.float32))
             g.close()
@@ -124,7 +127,7 @@ def load_policy(path=DEFAULT_WEIGHTS):
     return P.RhoMLP(params=p)
 
 
-def compare(n, seeds, policy, fixed=(1.0, 50.0, 100.0), max_iters=20000, log=print):
+def compare(n, seeds, policy, fixed=(1.0, 50.0, 100.0), max_iters=20000, log=print, per_edge=True):
     """Time to tolerance of fixed rho values and of the learned policy (per
     edge and per type, from LEARNED_START_RHO)."""
     out = []
@@ -132,7 +135,8 @@ def compare(n, seeds, policy, fixed=(1.0, 50.0, 100.0), max_iters=20000, log=pri
         row = {"n": n, "seed": seed}
         for rho in fixed:
             row[f"fixed{rho:g}"] = solve_once(n, seed, rho, None, max_iters=max_iters)
-        row["learned"] = solve_once(n, seed, LEARNED_START_RHO, policy, max_iters=max_iters)
+        if per_edge:
+            row["learned"] = solve_once(n, seed, LEARNED_START_RHO, policy, max_iters=max_iters)
         row["learned_type"] = solve_once(n, seed, LEARNED_START_RHO, policy, max_iters=max_iters, per_type=True)
         out.append(row)
         log(f"# N={n} seed={seed}: " + ", ".join(

@@ -48,6 +48,25 @@ enum {
 /* Last error message of this thread ("" if none). */
 const char *paradmm_last_error(void);
 
+/* ---- bulk construction -------------------------------------------------------
+ * `count` factors of one operator and degree in one call: factor i has the
+ * variables vars[i*nvars .. i*nvars+nvars) and the parameters
+ * params + i*param_stride bytes (param_stride 0: one shared block).  Edge
+ * numbering is that of `count` addNode calls (core/paradmm.c:381-466);
+ * the reference has no bulk form. */
+int paradmm_add_nodes(graph *G, paradmm_prox_fn op, void *params, size_t param_stride, int count, int nvars,
+                      const int *vars);
+
+/* ---- snapshots -----------------------------------------------------------------
+ * The iteration state held in the graph's host arrays (X, M, N, U, Z, RHOS,
+ * ALPHAS; core/paradmm.h:53-61) to and from a file, with the graph's
+ * dimensions and a hash of its edge numbering; loading into a graph of
+ * another structure fails with PARADMM_ERR_SHAPE.  Saving first downloads
+ * newer device state; loading refreshes the device state before its next
+ * use.  No reference counterpart (the reference has no file format). */
+int paradmm_save_state(graph *G, const char *path);
+int paradmm_load_state(graph *G, const char *path);
+
 /* ---- prox kinds ------------------------------------------------------------
  * The reference calls host function pointers from the X pass
  * (core/paradmm.c:590) and device function pointers on the GPU

@@ -76,8 +76,11 @@ def test_addnode_sorts_a_copy_and_handles_growth(pdm):
     assert total == 12000
 
 
-def test_unknown_callback_is_reported(pdm):
-    """A host callback the engine cannot run on the GPU: an error code, no device work."""
+def test_unknown_callback_is_reported_when_host_callbacks_are_off(pdm, monkeypatch):
+    """With PARADMM_HOST_PROX=0 a callback the engine cannot run on the GPU is
+    an error code before any device work; by default such a callback runs on
+    the host (tested on the GPU)."""
+    monkeypatch.setenv("PARADMM_HOST_PROX", "0")
     cb = pdm.PROX_FN(lambda *a: None)
     g = pdm.Graph(3)
     vv = (C.c_int * 1)(0)
@@ -87,6 +90,7 @@ def test_unknown_callback_is_reported(pdm):
     rc = g.lib.paradmm_gpu_prepare(g.ptr)
     assert rc == 2
     assert "not a device prox kind" in g.lib.paradmm_last_error().decode()
+    assert g.lib.paradmm_prox_kind_of(C.cast(cb, C.c_void_p)) == 0
     # registering it as a kind makes it acceptable (checked again on the GPU)
     assert g.lib.paradmm_register_prox_kind(C.cast(cb, C.c_void_p), 5) == 0
     assert g.lib.paradmm_prox_kind_of(C.cast(cb, C.c_void_p)) == 5
@@ -128,3 +132,59 @@ def test_seeded_init_equals_oracle(pdm, oracle):
     assert np.array_equal(g.array("Z"), o.array("Z"))
     assert np.array_equal(g.array("X"), o.array("X"))
     assert np.array_equal(g.array("ALPHAS"), o.array("ALPHA"))
+
+
+def test_add_nodes_matches_addnode_loop(pdm):
+    """The bulk call numbers edges and builds adjacency exactly as the same
+    addNode calls do."""
+    rng = np.random.default_rng(4)
+    pairs = np.array([(i, j) for i in range(12) for j in range(i + 1, 12)], dtype=np.int32)
+    rng.shuffle(pairs)
+    pairs[::3] = pairs[::3, ::-1]  # unsorted variable lists too
+    fn = pdm.prox_address("paradmm_prox_disk_collision_3d")
+    a, b = pdm.Graph(3), pdm.Graph(3)
+    for i, j in pairs:
+        a.add_factor(fn, [int(i), int(j)])
+    b.add_factors(fn, pairs)
+    assert a.g.num_opts == b.g.num_opts and a.g.num_edges == b.g.num_edges and a.g.num_vars == b.g.num_vars
+    for e in range(a.g.num_edges):
+        assert a.g.edgeix_to_varlistix[e] == b.g.edgeix_to_varlistix[e]
+    for v in range(a.g.num_vars):
+        n = a.g.varix_to_numedges[v]
+        assert n == b.g.varix_to_numedges[v]
+        assert [a.g.varix_to_edgelistix[v][k] for k in range(n)] == [b.g.varix_to_edgelistix[v][k] for k in range(n)]
+    # per-factor parameter rows
+    rad = pdm.prox_address("paradmm_prox_max_radius_3d")
+    c = pdm.Graph(3)
+    prm = np.array([[1.0], [2.0], [1.0]])
+    c.add_factors(rad, np.array([[0], [1], [2]]), prm)
+    pp = [C.cast(c.g.list_of_operators_param_pointers[f], C.POINTER(C.c_double))[0] for f in range(3)]
+    assert pp == [1.0, 2.0, 1.0]
+    with pytest.raises(pdm.ParadmmError):
+        c.add_factors(rad, np.array([[-1]]))
+
+
+def test_snapshot_round_trip(pdm, tmp_path):
+    g = pdm.packing_square(10, 3, rho=7.0, alpha=0.5)
+    rng = np.random.default_rng(1)
+    for n in ("X", "M", "N", "U", "Z", "RHOS", "ALPHAS"):
+        g.array(n)[:] = rng.uniform(-1, 1, size=g.array(n).shape)
+    saved = {n: g.array(n).copy() for n in ("X", "M", "N", "U", "Z", "RHOS", "ALPHAS")}
+    path = str(tmp_path / "state.pdm")
+    g.save_state(path)
+    for n in saved:
+        g.array(n)[:] = 0.0
+    g.load_state(path)
+    for n in saved:
+        assert np.array_equal(g.array(n), saved[n]), n
+    # another structure, a truncated file and a foreign file are refused
+    other = pdm.packing_square(11, 3)
+    with pytest.raises(pdm.ParadmmError, match="another graph structure"):
+        other.load_state(path)
+    raw = open(path, "rb").read()
+    open(path, "wb").write(raw[:-100])
+    with pytest.raises(pdm.ParadmmError, match="truncated"):
+        g.load_state(path)
+    open(path, "wb").write(b"not a snapshot at all" * 4)
+    with pytest.raises(pdm.ParadmmError, match="not a snapshot"):
+        g.load_state(path)
