@@ -58,6 +58,8 @@ struct BInst {
     long s0;        // first slot
     long p0;        // first float of the partial region
     double E;       // edges: N(N-1) pair edges + 2N unary
+    int uni;        // every edge of the instance has rho_u and alpha_u
+    float rho_u, alpha_u;
 };
 
 struct BState {
@@ -127,6 +129,8 @@ __device__ __forceinline__ DArgs inst_view(const BArgs& a, const BInst& in)
     v.Rp = nullptr;
     v.aexp = a.aexp + in.c0;
     v.tiles = nullptr;
+    v.rho_u = in.rho_u;
+    v.alpha_u = in.alpha_u;
     return v;
 }
 
@@ -198,7 +202,14 @@ __device__ void phase_tiles(const BArgs& a, const BLists& L, TileShared<T>& sh)
         const int t = in.t0 + lt;
         const int4 td = a.tiles[t];
         const DArgs v = inst_view(a, in);
-        tile_pass<T, MODE, QUAL>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
+        // in the steady state, instances whose edges share one rho and alpha
+        // skip those streams
+        if constexpr (MODE == DM_FUSED && !QUAL) {
+            if (in.uni) tile_pass<T, MODE, QUAL, true>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
+            else tile_pass<T, MODE, QUAL, false>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
+        } else {
+            tile_pass<T, MODE, QUAL, false>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
+        }
         __syncthreads();  // the tile's shared scratch is reused by the next tile
     }
 }
@@ -491,7 +502,14 @@ static int batch_upload(paradmm_batch* b)
         }
         for (int v = 0; v < in.N; ++v)
             for (int k = 0; k < 3; ++k) z[3L * (in.c0 + v) + k] = (float)G->Z[3L * v + k];
+        BInst& bi = b->inst[i];
+        bi.rho_u = (float)G->RHOS[0];
+        bi.alpha_u = (float)G->ALPHAS[0];
+        bi.uni = 1;
+        for (int e = 1; e < G->num_edges && bi.uni; ++e)
+            bi.uni = (float)G->RHOS[e] == bi.rho_u && (float)G->ALPHAS[e] == bi.alpha_u;
     }
+    CK(hipMemcpyAsync(b->inst_d, b->inst.data(), sizeof(BInst) * (size_t)b->ninst, hipMemcpyHostToDevice, b->stream));
     CK(hipMemcpyAsync(b->X, x.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice, b->stream));
     CK(hipMemcpyAsync(b->U, u.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice, b->stream));
     CK(hipMemcpyAsync(b->Nb, n.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice, b->stream));

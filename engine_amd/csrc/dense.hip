@@ -53,17 +53,27 @@ struct DenseState {
     double* d_Rp = nullptr;   // [nt] primal residual partials
     double* d_Rd = nullptr;   // [N] dual residual partials
     float* d_aexp = nullptr;  // [N] radius exponent per circle
+    float* d_uni = nullptr;   // [2] common rho, alpha (uniform runs)
     bool u_pending = false;   // U still lacks the update of the last iteration
 };
 
-// one workgroup per tile
-template <int MODE>
+// one workgroup per tile; UNI: one rho and one alpha for every edge
+template <int MODE, bool UNI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDM_PAIRTILE_WAVES))) void k_pairtile(DArgs a)
 {
     __shared__ TileShared<DT> sh;
     const int t = blockIdx.x;
     const int2 ij = a.tiles[t];
-    tile_pass<DT, MODE, false>(a, t, ij.x, ij.y, sh, a.Rp + t, nullptr);
+    if constexpr (UNI) {
+        // the common values live in device memory, so captured iterations
+        // follow later changes of them
+        DArgs b = a;
+        b.rho_u = a.uni[0];
+        b.alpha_u = a.uni[1];
+        tile_pass<DT, MODE, false, UNI>(b, t, ij.x, ij.y, sh, a.Rp + t, nullptr);
+    } else {
+        tile_pass<DT, MODE, false, UNI>(a, t, ij.x, ij.y, sh, a.Rp + t, nullptr);
+    }
 }
 
 // Z_c = sum rho M / sum rho over the circle's nb block partials.  A
@@ -236,6 +246,7 @@ int dense_alloc_state(DevGraph* s)
     RC(dmalloc(s, (void**)&d->d_P, sizeof(float) * 5 * (size_t)d->N * d->nb));
     RC(dmalloc(s, (void**)&d->d_Rp, sizeof(double) * (size_t)d->nt));
     RC(dmalloc(s, (void**)&d->d_Rd, sizeof(double) * (size_t)d->N));
+    RC(dmalloc(s, (void**)&d->d_uni, sizeof(float) * 2));
     CK(hipMemsetAsync(d->d_Rp, 0, sizeof(double) * (size_t)d->nt, s->stream));
     CK(hipMemsetAsync(d->d_Rd, 0, sizeof(double) * (size_t)d->N, s->stream));
     return PARADMM_OK;
@@ -265,6 +276,9 @@ static DArgs dargs(DevGraph* s)
     a.Rp = d->d_Rp;
     a.aexp = d->d_aexp;
     a.tiles = d->d_tiles;
+    a.rho_u = s->rho_u;
+    a.alpha_u = s->alpha_u;
+    a.uni = d->d_uni;
     return a;
 }
 
@@ -284,14 +298,26 @@ static int launch_pairtile(DevGraph* s, int mode)
 {
     DArgs a = dargs(s);
     const unsigned nt = (unsigned)s->dense->nt;
+    // the steady-state modes have a variant without the per-edge rho/alpha
+    // streams when every edge shares one rho and one alpha
+    const bool uni = s->rho_uni && s->alpha_uni && mode <= DM_FUSED_NBUF;
     prof_begin(s, mode_name(mode));
     switch (mode) {
-    case DM_FUSED: k_pairtile<DM_FUSED><<<nt, 256, 0, s->stream>>>(a); break;
-    case DM_FUSED_CURRENT: k_pairtile<DM_FUSED_CURRENT><<<nt, 256, 0, s->stream>>>(a); break;
-    case DM_FUSED_NBUF: k_pairtile<DM_FUSED_NBUF><<<nt, 256, 0, s->stream>>>(a); break;
-    case DM_LEGACY_X: k_pairtile<DM_LEGACY_X><<<nt, 256, 0, s->stream>>>(a); break;
-    case DM_LEGACY_Z: k_pairtile<DM_LEGACY_Z><<<nt, 256, 0, s->stream>>>(a); break;
-    default: k_pairtile<DM_FINISH><<<nt, 256, 0, s->stream>>>(a); break;
+    case DM_FUSED:
+        if (uni) k_pairtile<DM_FUSED, true><<<nt, 256, 0, s->stream>>>(a);
+        else k_pairtile<DM_FUSED, false><<<nt, 256, 0, s->stream>>>(a);
+        break;
+    case DM_FUSED_CURRENT:
+        if (uni) k_pairtile<DM_FUSED_CURRENT, true><<<nt, 256, 0, s->stream>>>(a);
+        else k_pairtile<DM_FUSED_CURRENT, false><<<nt, 256, 0, s->stream>>>(a);
+        break;
+    case DM_FUSED_NBUF:
+        if (uni) k_pairtile<DM_FUSED_NBUF, true><<<nt, 256, 0, s->stream>>>(a);
+        else k_pairtile<DM_FUSED_NBUF, false><<<nt, 256, 0, s->stream>>>(a);
+        break;
+    case DM_LEGACY_X: k_pairtile<DM_LEGACY_X, false><<<nt, 256, 0, s->stream>>>(a); break;
+    case DM_LEGACY_Z: k_pairtile<DM_LEGACY_Z, false><<<nt, 256, 0, s->stream>>>(a); break;
+    default: k_pairtile<DM_FINISH, false><<<nt, 256, 0, s->stream>>>(a); break;
     }
     prof_end(s);
     CK(hipGetLastError());
@@ -319,6 +345,8 @@ int dense_iteration(DevGraph* s, int from_n)
 }
 
 bool dense_steady(DevGraph* s) { return s->dense && s->dense->u_pending; }
+
+float* dense_uni_ptr(DevGraph* s) { return s->dense ? s->dense->d_uni : nullptr; }
 
 void dense_reset(DevGraph* s)
 {

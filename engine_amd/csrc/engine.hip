@@ -768,9 +768,31 @@ static int upload(DevGraph* s, int which)
         CK(hipGetLastError());
     }
     if (which & (A_RHO | A_ALPHA)) s->rho_changed = true;
+    // one common value over every edge (compared in fp32, the device type)
+    auto common = [E](const double* h, float& v) {
+        v = (float)h[0];
+        for (int e = 1; e < E; ++e)
+            if ((float)h[e] != v) return false;
+        return true;
+    };
+    if (which & A_RHO) s->rho_uni = common(G->RHOS, s->rho_u);
+    if (which & A_ALPHA) s->alpha_uni = common(G->ALPHAS, s->alpha_u);
     CK(hipStreamSynchronize(s->stream));  // the staging area is reused
+    if (which & (A_RHO | A_ALPHA)) RC(sync_uniform(s));
     return PARADMM_OK;
 }
+
+namespace pdm {
+int sync_uniform(DevGraph* s)
+{
+    float* d = dense_uni_ptr(s);
+    if (!d) return PARADMM_OK;
+    const float v[2] = {s->rho_u, s->alpha_u};
+    CK(hipStreamSynchronize(s->stream));
+    CK(hipMemcpy(d, v, sizeof v, hipMemcpyHostToDevice));
+    return PARADMM_OK;
+}
+}  // namespace pdm
 
 static int download(DevGraph* s, int which)
 {
@@ -901,6 +923,7 @@ static int run_host_prox(DevGraph* s, int from_n)
     if (rho_changed) {
         s->rho_device_written = true;
         s->rho_changed = true;
+        s->rho_uni = false;
     }
     return PARADMM_OK;
 }
@@ -1042,7 +1065,16 @@ static int run_fused(DevGraph* s, int iters)
     const int GI = s->graph_iters;
     // host-callback factors synchronise with the host every iteration: no capture
     if (iters >= GI && GI > 1 && !s->host) {
+        const int variant = (s->rho_uni && s->alpha_uni) ? 1 : 0;
+        if (s->gexec && s->graph_variant != variant) {
+            // recorded with the other kernel variant: record again
+            CK(hipGraphExecDestroy(s->gexec));
+            CK(hipGraphDestroy(s->ghandle));
+            s->gexec = nullptr;
+            s->ghandle = nullptr;
+        }
         if (!s->gexec) {
+            s->graph_variant = variant;
             CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
             int rc = PARADMM_OK;
             for (int i = 0; i < GI && !rc; ++i) rc = fused_iteration(s, 0);

@@ -75,6 +75,8 @@ struct DevGraph {
     unsigned char* d_edge_type_api = nullptr;  // api edge -> PDM_EDGE_* (pair, container, radius)
     float* d_ckpt = nullptr;       // saved X, U, rho, Z, Zprev (paradmm_gpu_state_save)
     bool ckpt_n_from_buffer = false;
+    bool ckpt_rho_uni = false;
+    float ckpt_rho_u = 0.0f;
 
     // coherence
     bool host_newer = true;    // host arrays hold data the device has not seen
@@ -83,6 +85,11 @@ struct DevGraph {
     bool m_valid = false;      // the M buffer holds the M of the last iteration
     bool rho_changed = true;   // rho/alpha uploaded since the last per-variable sums
     bool rho_device_written = false;
+    // every edge holds the same rho / alpha (set at upload, cleared by a
+    // per-edge change on the device): the dense kernels then skip those streams
+    bool rho_uni = false, alpha_uni = false;
+    float rho_u = 0.0f, alpha_u = 0.0f;
+    int graph_variant = -1;  // kernel variant the captured graph was recorded with
     bool iters_done = false;
     bool dense_unary_stale = false;
 
@@ -98,8 +105,11 @@ struct DevGraph {
     // iter_frac = iterations done / max_iters.  Set by the learned-rho module.
     int (*rho_policy)(DevGraph* s, float iter_frac) = nullptr;
     void* policy_ctx = nullptr;
-    bool policy_per_type = false;      // apply the per-type mean of the network's log10 rho
+    int policy_per_type = 0;           // 0 per edge, 1 per-type mean, 2 one mean over all edges
+    bool policy_average = false;       // average the estimates over the applications so far
     void* d_policy_part = nullptr;     // per-type partial sums of that mean
+    int* d_slot_var = nullptr;         // device slot -> variable (-1: padding), policy kernels
+    unsigned char* d_slot_type = nullptr;
 
     // kernel timing (paradmm_gpu_profile_kernels): eager launches only
     bool prof = false;
@@ -143,6 +153,10 @@ int dense_residual_reduce(DevGraph* s);
 void dense_free(DevGraph* s);
 // true when the next dense iteration is the steady-state (captured) kind
 bool dense_steady(DevGraph* s);
+// device copy {rho_u, alpha_u} read by the uniform kernel variant (dense only)
+float* dense_uni_ptr(DevGraph* s);
+// writes s->rho_u, s->alpha_u to that copy (stream-ordered, synchronous)
+int sync_uniform(DevGraph* s);
 // the device state was replaced from the host: U is current
 void dense_reset(DevGraph* s);
 // applies the pending U update (and its primal residual) so U is current

@@ -669,7 +669,41 @@ struct PolicyArgs {
     const float* Zprev;
     float frac;
     float* feat_out;  // features only (training data) when non-null
+    // device-slot order (policy application): variable and type of every
+    // slot, variable -1 for padding slots
+    long nslots;
+    const int* slot_var;
+    const unsigned char* slot_type;
 };
+
+// residuals of the edge in device slot p (coalesced in slot order)
+__device__ __forceinline__ void slot_residuals(const PolicyArgs& a, long p, int v, float& r, float& s, float& rho)
+{
+    float r2 = 0.0f, dz2 = 0.0f;
+    for (int k = 0; k < a.D; ++k) {
+        const float z = a.Z[v * a.D + k];
+        const float d = a.X[k * a.stride + p] - z;
+        const float q = z - a.Zprev[v * a.D + k];
+        r2 += d * d;
+        dz2 += q * q;
+    }
+    rho = a.rho[p];
+    r = sqrtf(r2);
+    s = rho * sqrtf(dz2);
+}
+
+// slot -> variable / type maps from the api edge maps
+__global__ __launch_bounds__(256) void k_build_slot_maps(int E, const int* __restrict__ pos,
+                                                         const int* __restrict__ edge_var,
+                                                         const unsigned char* __restrict__ type,
+                                                         int* __restrict__ slot_var, unsigned char* __restrict__ slot_type)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= E) return;
+    const int p = pos[e];
+    slot_var[p] = edge_var[e];
+    slot_type[p] = type[e];
+}
 
 __device__ __forceinline__ void edge_residuals(const PolicyArgs& a, int e, float& r, float& s, float& rho, int& type,
                                                int& p)
@@ -736,13 +770,17 @@ __global__ __launch_bounds__(256) void k_rho_policy(PolicyArgs a, const float* _
     __syncthreads();
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
     TileScratch& ts = tsw[wv];
-    const long nchunks = ((long)a.E + 63) / 64;
+    const long nchunks = (a.nslots + 63) / 64;
     for (long ch = (long)blockIdx.x * 4 + wv; ch < nchunks; ch += (long)gridDim.x * 4) {
-        const long e = ch * 64 + l;
-        const bool valid = e < a.E;
+        const long p = ch * 64 + l;
+        const int v = p < a.nslots ? a.slot_var[p] : -1;
+        const bool valid = v >= 0;
         float r = 0.0f, s = 0.0f, rho = 1.0f, f[MLP_NIN];
-        int type = 0, p = 0;
-        if (valid) edge_residuals(a, (int)e, r, s, rho, type, p);
+        int type = 0;
+        if (valid) {
+            slot_residuals(a, p, v, r, s, rho);
+            type = a.slot_type[p];
+        }
         edge_features(r, s, rho, type, a.frac, f);
         const float y = policy_wave_eval(w, ts, yv[wv], f, l);
         if (valid) {
@@ -771,13 +809,17 @@ __global__ __launch_bounds__(256) void k_rho_policy_type_sums(PolicyArgs a, cons
     TileScratch& ts = tsw[wv];
     double acc[3] = {0.0, 0.0, 0.0};
     int num[3] = {0, 0, 0};
-    const long nchunks = ((long)a.E + 63) / 64;
+    const long nchunks = (a.nslots + 63) / 64;
     for (long ch = (long)blockIdx.x * 4 + wv; ch < nchunks; ch += (long)gridDim.x * 4) {
-        const long e = ch * 64 + l;
-        const bool valid = e < a.E;
+        const long p = ch * 64 + l;
+        const int v = p < a.nslots ? a.slot_var[p] : -1;
+        const bool valid = v >= 0;
         float r = 0.0f, s = 0.0f, rho = 1.0f, f[MLP_NIN];
-        int type = 0, p = 0;
-        if (valid) edge_residuals(a, (int)e, r, s, rho, type, p);
+        int type = 0;
+        if (valid) {
+            slot_residuals(a, p, v, r, s, rho);
+            type = a.slot_type[p];
+        }
         edge_features(r, s, rho, type, a.frac, f);
         const float y = policy_wave_eval(w, ts, yv[wv], f, l);
         if (valid) {
@@ -811,27 +853,70 @@ __global__ __launch_bounds__(256) void k_rho_policy_type_sums(PolicyArgs a, cons
 
 // Per-type mode, pass 2: mean log10 rho per type over the partials in fixed
 // order; rho_type = 10^mean (types without edges keep 0 and are unused).
-__global__ void k_rho_policy_type_final(int nblocks, const double* __restrict__ part, const int* __restrict__ cnt,
-                                        float* __restrict__ rho_type)
+// global: one mean over all edges for every type.  hist (optional):
+// running sums {sum_k mean_k, count} per type over the applications so far;
+// the applied value is then the average of this and the earlier estimates.
+__global__ __launch_bounds__(256) void k_rho_policy_type_final(int nblocks, const double* __restrict__ part,
+                                                               const int* __restrict__ cnt,
+                                                               float* __restrict__ rho_type, int global,
+                                                               double* __restrict__ hist)
 {
-    const int k = threadIdx.x;
-    if (k >= 3) return;
-    double sum = 0.0;
-    long n = 0;
-    for (int b = 0; b < nblocks; ++b) {
-        sum += part[b * 3 + k];
-        n += cnt[b * 3 + k];
+    // thread t sums the partials of blocks t, t + 256, ... (fixed order),
+    // then a fixed-shape tree over the 256 threads
+    __shared__ double ssum[3][256];
+    __shared__ long scnt[3][256];
+    const int t = threadIdx.x;
+    double sum[3] = {0.0, 0.0, 0.0};
+    long n[3] = {0, 0, 0};
+    for (int b = t; b < nblocks; b += 256)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            sum[q] += part[b * 3 + q];
+            n[q] += cnt[b * 3 + q];
+        }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        ssum[q][t] = sum[q];
+        scnt[q][t] = n[q];
     }
-    rho_type[k] = n ? exp10f((float)(sum / (double)n)) : 0.0f;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                ssum[q][t] += ssum[q][t + w];
+                scnt[q][t] += scnt[q][t + w];
+            }
+        __syncthreads();
+    }
+    if (t >= 3) return;
+    const int k = t;
+    double S = 0.0;
+    long Nn = 0;
+    for (int q = 0; q < 3; ++q)
+        if (global || q == k) {
+            S += ssum[q][0];
+            Nn += scnt[q][0];
+        }
+    if (!Nn) {
+        rho_type[k] = 0.0f;
+        return;
+    }
+    double m = S / (double)Nn;
+    if (hist) {
+        hist[2 * k] += m;
+        hist[2 * k + 1] += 1.0;
+        m = hist[2 * k] / hist[2 * k + 1];
+    }
+    rho_type[k] = exp10f((float)m);
 }
 
 // Per-type mode, pass 3: rho_e <- rho_type(e), U_e <- U_e rho_old / rho_new.
 __global__ __launch_bounds__(256) void k_rho_policy_type_apply(PolicyArgs a, const float* __restrict__ rho_type)
 {
-    const long e = (long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= a.E) return;
-    const int p = a.pos[e];
-    const float rn = rho_type[a.type[e]];
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= a.nslots || a.slot_var[p] < 0) return;
+    const float rn = rho_type[a.slot_type[p]];
     const float scale = a.rho[p] / rn;
     for (int k = 0; k < a.D; ++k) a.U[k * a.stride + p] *= scale;
     a.rho[p] = rn;
@@ -864,7 +949,32 @@ static PolicyArgs policy_args(DevGraph* s, float frac)
     a.Zprev = s->d_Zprev;
     a.frac = frac;
     a.feat_out = nullptr;
+    a.nslots = s->stride;
+    a.slot_var = s->d_slot_var;
+    a.slot_type = s->d_slot_type;
     return a;
+}
+
+// slot maps for the policy kernels, built on first use
+static int ensure_slot_maps(DevGraph* s)
+{
+    if (s->d_slot_var) return PARADMM_OK;
+    RC(dmalloc(s, (void**)&s->d_slot_var, sizeof(int) * (size_t)s->stride));
+    RC(dmalloc(s, (void**)&s->d_slot_type, (size_t)s->stride));
+    CK(hipMemsetAsync(s->d_slot_var, 0xff, sizeof(int) * (size_t)s->stride, s->stream));
+    CK(hipMemsetAsync(s->d_slot_type, 0, (size_t)s->stride, s->stream));
+    k_build_slot_maps<<<(unsigned)((s->E + 255) / 256), 256, 0, s->stream>>>(s->E, s->d_pos, s->d_edge_var_api,
+                                                                           s->d_edge_type_api, s->d_slot_var,
+                                                                           s->d_slot_type);
+    CK(hipGetLastError());
+    return PARADMM_OK;
+}
+
+// the per-type rho of the last per-type / global application
+static float* rho_type_ptr(DevGraph* s)
+{
+    double* part = (double*)s->d_policy_part + 6;
+    return (float*)((int*)(part + 3 * 8192) + 3 * 8192);
 }
 
 // makes U current (dense layout) so it can be rescaled
@@ -879,27 +989,40 @@ static int apply_policy(DevGraph* s, float frac)
     paradmm_rho_mlp* m = (paradmm_rho_mlp*)s->policy_ctx;
     if (!m) return PARADMM_OK;
     RC(current_u(s));
+    RC(ensure_slot_maps(s));
     PolicyArgs a = policy_args(s, frac);
-    const long nchunks = ((long)s->E + 63) / 64;
+    const long nchunks = (a.nslots + 63) / 64;
     const unsigned grid = (unsigned)std::min<long>((nchunks + 3) / 4, 8192);
     prof_begin(s, "rho_policy");
     if (!s->policy_per_type) {
         k_rho_policy<<<grid, 256, 0, s->stream>>>(a, mlp_device_params(m));
     } else {
         if (!s->d_policy_part) {
-            // partial sums [grid][3] (FP64), counts [grid][3], rho per type [3]
-            RC(dmalloc(s, (void**)&s->d_policy_part, sizeof(double) * 3 * 8192 + sizeof(int) * 3 * 8192 + 16));
+            // running sums [3][2] (FP64), partial sums [grid][3] (FP64),
+            // counts [grid][3], rho per type [3]
+            RC(dmalloc(s, (void**)&s->d_policy_part, sizeof(double) * (6 + 3 * 8192) + sizeof(int) * 3 * 8192 + 16));
+            CK(hipMemsetAsync(s->d_policy_part, 0, sizeof(double) * 6, s->stream));
         }
-        double* part = (double*)s->d_policy_part;
+        double* hist = (double*)s->d_policy_part;
+        double* part = hist + 6;
         int* cnt = (int*)(part + 3 * 8192);
         float* rho_type = (float*)(cnt + 3 * 8192);
         k_rho_policy_type_sums<<<grid, 256, 0, s->stream>>>(a, mlp_device_params(m), part, cnt);
-        k_rho_policy_type_final<<<1, 64, 0, s->stream>>>((int)grid, part, cnt, rho_type);
-        k_rho_policy_type_apply<<<(unsigned)((s->E + 255) / 256), 256, 0, s->stream>>>(a, rho_type);
+        k_rho_policy_type_final<<<1, 256, 0, s->stream>>>((int)grid, part, cnt, rho_type, s->policy_per_type == 2,
+                                                         s->policy_average ? hist : nullptr);
+        k_rho_policy_type_apply<<<(unsigned)((a.nslots + 255) / 256), 256, 0, s->stream>>>(a, rho_type);
     }
     prof_end(s);
     CK(hipGetLastError());
     s->rho_device_written = true;
+    s->rho_uni = false;
+    if (s->policy_per_type == 2) {
+        // one rho for every edge: the steady-state kernels may use it as a scalar
+        CK(hipMemcpyAsync(&s->rho_u, rho_type_ptr(s), sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        CK(hipStreamSynchronize(s->stream));
+        s->rho_uni = true;
+        RC(sync_uniform(s));
+    }
     return PARADMM_OK;
 }
 
@@ -914,11 +1037,17 @@ extern "C" int paradmm_gpu_set_rho_policy(graph* G, paradmm_rho_mlp* m)
     return PARADMM_OK;
 }
 
-extern "C" int paradmm_gpu_set_rho_policy_mode(graph* G, int per_type)
+extern "C" int paradmm_gpu_set_rho_policy_mode(graph* G, int mode, int average)
 {
+    if (mode < 0 || mode > 2) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_gpu_set_rho_policy_mode: mode 0, 1 or 2");
+        return PARADMM_ERR_ARG;
+    }
     DevGraph* s;
     RC(pdm_get_state(G, &s, true));
-    s->policy_per_type = per_type != 0;
+    s->policy_per_type = mode;
+    s->policy_average = average != 0;
+    if (s->d_policy_part) CK(hipMemsetAsync(s->d_policy_part, 0, sizeof(double) * 6, s->stream));
     return PARADMM_OK;
 }
 
@@ -974,8 +1103,10 @@ extern "C" int paradmm_gpu_scale_rho(graph* G, double factor)
     RC(current_u(s));
     k_scale_rho<<<(unsigned)((s->stride + 255) / 256), 256, 0, s->stream>>>(s->stride, s->D, s->stride, (float)factor,
                                                                              s->d_rho, s->d_U);
+    s->rho_u *= (float)factor;
     CK(hipGetLastError());
     CK(hipStreamSynchronize(s->stream));
+    RC(sync_uniform(s));
     s->rho_device_written = true;
     s->dev_newer = true;
     s->m_valid = false;
@@ -998,6 +1129,8 @@ extern "C" int paradmm_gpu_state_save(graph* G)
     CK(hipMemcpyAsync(c + 2 * se + s->stride + sv, s->d_Zprev, sizeof(float) * sv, hipMemcpyDeviceToDevice, s->stream));
     CK(hipStreamSynchronize(s->stream));
     s->ckpt_n_from_buffer = s->n_from_buffer;
+    s->ckpt_rho_uni = s->rho_uni;
+    s->ckpt_rho_u = s->rho_u;
     return PARADMM_OK;
 }
 
@@ -1019,6 +1152,9 @@ extern "C" int paradmm_gpu_state_restore(graph* G)
     CK(hipStreamSynchronize(s->stream));
     if (s->layout == 2) dense_reset(s);  // the saved U was current
     s->n_from_buffer = s->ckpt_n_from_buffer;
+    s->rho_uni = s->ckpt_rho_uni;
+    s->rho_u = s->ckpt_rho_u;
+    RC(sync_uniform(s));
     s->host_newer = false;
     s->dev_newer = true;
     s->m_valid = false;

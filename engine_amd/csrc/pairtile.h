@@ -51,6 +51,8 @@ struct DArgs {
     double* Rp;       // [nt] primal partials (single-instance kernel)
     const float* aexp;
     const int2* tiles;
+    float rho_u, alpha_u;  // the common rho and alpha when UNI (per-edge arrays unread)
+    const float* uni;      // device copy {rho_u, alpha_u} (kernels replayed from a graph read it here)
 };
 
 // Edge streams are read and written once per iteration; PDM_DENSE_NT=1 marks
@@ -110,7 +112,9 @@ struct TileShared {
 // The pairs of one tile.  VAR 0: interior tile (no masks), 1: tile touching
 // the padded end (circle masks), 2: diagonal tile (masks, unary slots kept).
 // QUAL: also the largest pair overlap r_i + r_j - |c_i - c_j| of Z (FP64).
-template <int T, int MODE, int VAR, bool QUAL>
+// UNI: every edge has the same rho and alpha (a.rho_u, a.alpha_u): the
+// per-edge arrays are not read (48 instead of 56 bytes per edge).
+template <int T, int MODE, int VAR, bool QUAL, bool UNI = false>
 __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int I, const int J, TileShared<T>& sh,
                                           double& pr, double& qov)
 {
@@ -173,9 +177,14 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
         if (on) {
         float4 xa[3], xb[3], ua[3], ub[3], na[3], nbv[3], ma[3], mb[3];
         float4 ra, rb, aa, ab;
-        ra = ld4(a.rho + pa);
-        rb = ld4(a.rho + pb);
-        if constexpr (NEED_A) { aa = ld4(a.alpha + pa); ab = ld4(a.alpha + pb); }
+        if constexpr (UNI) {
+            ra = rb = make_float4(a.rho_u, a.rho_u, a.rho_u, a.rho_u);
+            if constexpr (NEED_A) aa = ab = make_float4(a.alpha_u, a.alpha_u, a.alpha_u, a.alpha_u);
+        } else {
+            ra = ld4(a.rho + pa);
+            rb = ld4(a.rho + pb);
+            if constexpr (NEED_A) { aa = ld4(a.alpha + pa); ab = ld4(a.alpha + pb); }
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             if constexpr (NEED_X) { xa[k] = ld4((a.X + k * S) + pa); xb[k] = ld4((a.X + k * S) + pb); }
@@ -289,7 +298,7 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
 // B[c][c] of circle c = I*T + tid, one lane per circle, run after the pair
 // sweep; their partial sums are added to the circle's row sum.  QUAL: the
 // circle's container violation and radius (FP64).
-template <int T, int MODE, bool QUAL>
+template <int T, int MODE, bool QUAL, bool UNI = false>
 __device__ __noinline__ void diag_unary(const DArgs& a, const int t, const int I, TileShared<T>& sh, double& pr,
                                         double& qbox, double& qsum)
 {
@@ -319,7 +328,7 @@ __device__ __noinline__ void diag_unary(const DArgs& a, const int t, const int I
 #pragma unroll
     for (int side = 0; side < 2; ++side) {  // 0: box (A), 1: radius (B)
         const long q = p[side];
-        const float r = a.rho[q];
+        const float r = UNI ? a.rho_u : a.rho[q];
         float u[3], x[3], n[3], m[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -327,7 +336,7 @@ __device__ __noinline__ void diag_unary(const DArgs& a, const int t, const int I
             if constexpr (APPLY_U) x[k] = a.X[k * S + q];
         }
         if constexpr (APPLY_U) {
-            const float al = a.alpha[q];
+            const float al = UNI ? a.alpha_u : a.alpha[q];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const float d = x[k] - z[k];
@@ -389,7 +398,7 @@ __device__ __forceinline__ double wsum64d(double v)
 // Called by all 256 threads of a workgroup; ends with a barrier-free
 // epilogue after one __syncthreads, so consecutive calls need a barrier in
 // between (the caller's loop provides it).
-template <int T, int MODE, bool QUAL>
+template <int T, int MODE, bool QUAL, bool UNI = false>
 __device__ __forceinline__ void tile_pass(const DArgs& a, const int t, const int I, const int J, TileShared<T>& sh,
                                           double* rp, double* q3)
 {
@@ -398,13 +407,13 @@ __device__ __forceinline__ void tile_pass(const DArgs& a, const int t, const int
     const int tid = threadIdx.x;
     double pr = 0.0, qov = 0.0, qbox = 0.0, qsum = 0.0;
     if (I == J) {
-        pair_tile<T, MODE, 2, QUAL>(a, t, I, J, sh, pr, qov);
+        pair_tile<T, MODE, 2, QUAL, UNI>(a, t, I, J, sh, pr, qov);
         __syncthreads();  // row sums are in srow
-        if (tid < T) diag_unary<T, MODE, QUAL>(a, t, I, sh, pr, qbox, qsum);
+        if (tid < T) diag_unary<T, MODE, QUAL, UNI>(a, t, I, sh, pr, qbox, qsum);
     } else if ((J + 1) * T > a.N) {
-        pair_tile<T, MODE, 1, QUAL>(a, t, I, J, sh, pr, qov);
+        pair_tile<T, MODE, 1, QUAL, UNI>(a, t, I, J, sh, pr, qov);
     } else {
-        pair_tile<T, MODE, 0, QUAL>(a, t, I, J, sh, pr, qov);
+        pair_tile<T, MODE, 0, QUAL, UNI>(a, t, I, J, sh, pr, qov);
     }
 
     if constexpr (APPLY_U) {

@@ -124,7 +124,7 @@ def load():
         "paradmm_gpu_square_quality": (C.c_int, [G, _DP, _DP, _DP]),
         "paradmm_rho_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), _FP, C.c_int, C.c_uint64]),
         "paradmm_rho_mlp_destroy": (None, [V]),
-        "paradmm_gpu_set_rho_policy_mode": (C.c_int, [G, C.c_int]),
+        "paradmm_gpu_set_rho_policy_mode": (C.c_int, [G, C.c_int, C.c_int]),
         "paradmm_add_nodes": (C.c_int, [G, V, V, C.c_size_t, C.c_int, C.c_int, _IP]),
         "paradmm_save_state": (C.c_int, [G, C.c_char_p]),
         "paradmm_load_state": (C.c_int, [G, C.c_char_p]),
@@ -380,8 +380,11 @@ class Graph:
     def set_rho_policy(self, mlp):
         check(self.lib.paradmm_gpu_set_rho_policy(self.ptr, mlp.h if mlp is not None else None), "set_rho_policy")
 
-    def set_rho_policy_mode(self, per_type):
-        check(self.lib.paradmm_gpu_set_rho_policy_mode(self.ptr, 1 if per_type else 0), "set_rho_policy_mode")
+    def set_rho_policy_mode(self, mode, average=False):
+        """0 per edge, 1 per-type mean, 2 one mean over all edges (True/False
+        map to 1/0); average: mean over the applications so far."""
+        check(self.lib.paradmm_gpu_set_rho_policy_mode(self.ptr, int(mode), 1 if average else 0),
+              "set_rho_policy_mode")
 
     def apply_rho_policy(self, mlp, iter_frac):
         check(self.lib.paradmm_gpu_apply_rho_policy(self.ptr, mlp.h, iter_frac), "apply_rho_policy")

@@ -30,6 +30,10 @@ from . import paradmm as P
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "models", "rho_policy.f32")
+# how the committed policy is applied (paradmm_gpu_set_rho_policy_mode):
+# one rho per graph, the running mean of the network's estimates
+DEFAULT_MODE, DEFAULT_AVERAGE = 2, True
+DEFAULT_MAX_ITERS = 30000
 RHO_GRID = (20.0, 35.0, 50.0, 70.0, 100.0, 150.0, 200.0, 300.0, 500.0, 800.0, 1200.0)
 # snapshot iterations of the training trajectories: dense early, then spread
 # over the whole budget so the iteration-fraction input is covered on [0, 1]
@@ -43,13 +47,15 @@ LEARNED_START_RHO = 1000.0
 
 
 def solve_once(n, seed, rho, policy=None, max_iters=20000, tol=1e-4, check_every=50, policy_every=50,
-               per_type=False):
-    """One solve to tolerance; policy None is the plain fixed-rho method."""
+               mode=0, average=False):
+    """One solve to tolerance; policy None is the plain fixed-rho method.
+    mode / average: how the network's outputs are applied
+    (paradmm_gpu_set_rho_policy_mode)."""
     g = P.packing_square(n, seed, rho=rho, alpha=1.0)
     g.prepare()
     if policy is not None:
         g.set_rho_policy(policy)
-        g.set_rho_policy_mode(per_type)
+        g.set_rho_policy_mode(mode, average)
     r = g.solve(max_iters, tol=tol, tol_feas=tol, check_every=check_every,
                 policy_every=policy_every if policy is not None else 0)
     g.close()
@@ -127,17 +133,22 @@ def load_policy(path=DEFAULT_WEIGHTS):
     return P.RhoMLP(params=p)
 
 
-def compare(n, seeds, policy, fixed=(1.0, 50.0, 100.0), max_iters=20000, log=print, per_edge=True):
-    """Time to tolerance of fixed rho values and of the learned policy (per
-    edge and per type, from LEARNED_START_RHO)."""
+LEARNED_VARIANTS = {"learned": (0, False), "learned_type": (1, False), "learned_global": (2, False),
+                    "learned_global_avg": (2, True)}
+
+
+def compare(n, seeds, policy, fixed=(1.0, 50.0, 100.0), max_iters=20000, log=print,
+            variants=("learned_global", "learned_global_avg")):
+    """Time to tolerance of fixed rho values and of the learned policy (the
+    given application variants, from LEARNED_START_RHO)."""
     out = []
     for seed in seeds:
         row = {"n": n, "seed": seed}
         for rho in fixed:
             row[f"fixed{rho:g}"] = solve_once(n, seed, rho, None, max_iters=max_iters)
-        if per_edge:
-            row["learned"] = solve_once(n, seed, LEARNED_START_RHO, policy, max_iters=max_iters)
-        row["learned_type"] = solve_once(n, seed, LEARNED_START_RHO, policy, max_iters=max_iters, per_type=True)
+        for v in variants:
+            mode, avg = LEARNED_VARIANTS[v]
+            row[v] = solve_once(n, seed, LEARNED_START_RHO, policy, max_iters=max_iters, mode=mode, average=avg)
         out.append(row)
         log(f"# N={n} seed={seed}: " + ", ".join(
             f"{k} {'%d it %.0f ms' % (v['iterations'], v['elapsed_ms']) if v['converged'] else 'not converged'}"

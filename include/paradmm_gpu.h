@@ -190,12 +190,15 @@ int paradmm_gpu_rho_features(graph *G, float iter_frac, float *feat);
 int paradmm_gpu_set_rho_policy(graph *G, paradmm_rho_mlp *m);
 /* apply the policy once now */
 int paradmm_gpu_apply_rho_policy(graph *G, paradmm_rho_mlp *m, float iter_frac);
-/* per_type = 0 (default): each edge takes the network's own output.
- * per_type = 1: each edge takes 10^(mean of the network's log10 rho over the
- * edges of its factor type: pair, container, radius) -- per-edge inference,
- * per-type application, for problems where uneven rho across one type slows
- * the consensus. */
-int paradmm_gpu_set_rho_policy_mode(graph *G, int per_type);
+/* How the network's per-edge outputs (log10 rho) are applied:
+ *   mode 0 (default): each edge takes its own output;
+ *   mode 1: each edge takes 10^(mean of the outputs over the edges of its
+ *           factor type: pair, container, radius);
+ *   mode 2: every edge takes 10^(mean over all edges) -- one rho per graph.
+ * average = 1 (modes 1, 2): the applied mean is the average of this and
+ * every earlier application's estimate since the mode was set.  Modes 1
+ * and 2 reduce in a fixed order (deterministic). */
+int paradmm_gpu_set_rho_policy_mode(graph *G, int mode, int average);
 /* rho_e *= factor on every edge, U_e /= factor (uniform change of rho) */
 int paradmm_gpu_scale_rho(graph *G, double factor);
 /* save / restore the device iterate (X, U, rho, Z, Zprev): counterfactual rollouts */

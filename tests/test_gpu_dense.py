@@ -253,3 +253,50 @@ def test_final_radii_vs_reference(layout, monkeypatch):
     print(f"{layout}: final radii after {iters} iterations: max |dr| vs reference {err_ref:.2e}, vs FP64 oracle {err_or:.2e}")
     assert g.layout() == (2 if layout == "dense" else 1)
     assert err_ref < 1e-4 and err_or < 1e-4
+
+
+def test_graph_recorded_again_when_rho_stops_being_uniform():
+    """Uniform rho/alpha runs the kernel variant without the per-edge rho and
+    alpha streams; after a per-edge change the captured iterations are
+    recorded again with the general variant.  Replays equal eager runs."""
+    from engine_amd import paradmm as P
+    from oracle import mlp_oracle as mo
+    m = P.RhoMLP(params=(mo.init_params(5) * 0.3).astype(np.float32))
+    zs = []
+    for eager in (False, True):
+        g = P.packing_square(90, 4, rho=8.0)
+        for _ in range(2):
+            if eager:
+                for _ in range(40):
+                    g.iterate(1)
+            else:
+                g.iterate(40)
+            g.apply_rho_policy(m, 0.2)
+        g.sync_to_host()
+        assert g.layout() == 2
+        assert len(np.unique(g.array("RHOS"))) > 1
+        zs.append(g.array("Z").copy())
+    assert np.array_equal(zs[0], zs[1])
+
+
+def test_uniform_rho_change_seen_by_captured_iterations():
+    """A uniform rho changed after the iterations were captured (scale_rho
+    keeps it uniform) is what the replays use: replays equal eager runs, and
+    differ from a run without the change."""
+    from engine_amd import paradmm as P
+    zs = []
+    for eager, scale in ((False, 2.0), (True, 2.0), (False, 1.0)):
+        g = P.packing_square(90, 4, rho=8.0)
+        for k in range(2):
+            if eager:
+                for _ in range(40):
+                    g.iterate(1)
+            else:
+                g.iterate(40)
+            if k == 0:
+                g.scale_rho(scale)
+        g.sync_to_host()
+        assert g.layout() == 2
+        zs.append(g.array("Z").copy())
+    assert np.array_equal(zs[0], zs[1])
+    assert not np.array_equal(zs[0], zs[2])

@@ -135,6 +135,25 @@ def test_policy_per_type_mode_applies_type_means():
     assert np.allclose(U1, U0 * (rho / rho_new)[:, None], rtol=1e-5, atol=1e-7)
 
 
+def test_policy_global_mode_with_average():
+    """mode 2: one rho for every edge, 10^(mean of all outputs); with
+    averaging, the second application applies the mean of both estimates."""
+    n = 40
+    g = P.packing_square(n, 2, rho=30.0, alpha=1.0)
+    g.iterate(25)
+    m = P.RhoMLP(params=(mo.init_params(4) * 0.3).astype(np.float32))
+    g.set_rho_policy_mode(2, average=True)
+    est = []
+    for k in range(2):
+        feat = g.rho_features(0.1 * (k + 1))
+        est.append(np.clip(m.forward(feat).astype(np.float64), -1.0, 4.0).mean())
+        g.apply_rho_policy(m, 0.1 * (k + 1))
+        g.sync_to_host()
+        rho = g.array("RHOS")
+        assert np.allclose(rho, 10.0 ** np.mean(est), rtol=1e-5)
+        g.iterate(10)
+
+
 def test_state_save_restore_and_scale():
     g = P.packing_square(50, 1, rho=10.0)
     g.iterate(10)
