@@ -125,6 +125,38 @@ def batch_bench(P, count, seed, iters, cap):
     }
 
 
+def learned_bench(P, cases, cap):
+    """Time to tolerance (primal, dual, overlap, container <= 1e-4) on one
+    GPU: plain fixed rho = 1 (the configured rho), plain fixed rho = 0.4 N
+    (the best of the teacher's grid at large N), and the learned policy
+    (committed weights, one rho per graph, running mean of the network's
+    estimates, applied every 50 iterations from rho = 1000)."""
+    from engine_amd import rho_train as rt
+    m = rt.load_policy()
+    out = []
+    for n, seed in cases:
+        row = {"n": n, "seed": seed, "cap": cap}
+        runs = {"fixed_rho1": (1.0, None), f"fixed_rho{0.4 * n:g}": (0.4 * n, None),
+                "learned": (rt.LEARNED_START_RHO, m)}
+        for name, (rho, pol) in runs.items():
+            r = rt.solve_once(n, seed, rho, pol, max_iters=cap, mode=rt.DEFAULT_MODE, average=rt.DEFAULT_AVERAGE)
+            row[name] = {"converged": bool(r["converged"]), "iterations": r["iterations"],
+                         "ms": r["elapsed_ms"], "objective": r["objective"], "max_overlap": r["max_overlap"]}
+        for base in [k for k in runs if k.startswith("fixed")]:
+            L, B = row["learned"], row[base]
+            if L["converged"] and B["converged"]:
+                row[f"speedup_vs_{base}"] = B["ms"] / L["ms"]
+            elif L["converged"]:
+                row[f"speedup_vs_{base}"] = f">{B['ms'] / L['ms']:.2f} ({base} not converged by the cap)"
+            else:
+                row[f"speedup_vs_{base}"] = None
+        print(f"# learned vs fixed N={n} seed={seed}: " + ", ".join(
+            f"{k} {'%d it %.0f ms' % (v['iterations'], v['ms']) if v['converged'] else 'not converged'}"
+            for k, v in row.items() if isinstance(v, dict)), file=sys.stderr, flush=True)
+        out.append(row)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +172,7 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--batch-iters", type=int, default=200)
     ap.add_argument("--batch-cap", type=int, default=20000)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-fixed time to tolerance")
     ap.add_argument("--host-loop-iters", type=int, default=0,
                     help="also time this many iterations of the legacy per-pass host loop")
     args = ap.parse_args()
@@ -176,7 +209,9 @@ def main():
         dom_bytes = EDGE_UPDATE_BYTES * dom_edges
     else:
         dom_bytes = BYTES_PER_EDGE.get(dom, 0) * dom_edges
-    kernel_bytes = BYTES_PER_EDGE.get(dom, 0) * dom_edges
+    # rho = 1 and alpha = 1 on every edge: the tile kernel runs its variant
+    # without the per-edge rho/alpha streams (48 B per edge instead of 56)
+    kernel_bytes = (48 if dom == "pairtile_fused" else BYTES_PER_EDGE.get(dom, 0)) * dom_edges
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(dom, args.n, args.seed)
     print(f"# kernels (mean ms per launch): " + ", ".join(f"{k} {v[0]:.4f}x{v[1]}" for k, v in prof.items()),
@@ -239,6 +274,8 @@ def main():
         line["host_loop_pcie_inclusive"] = host_loop
     if args.batch > 0:
         line["batch"] = batch_bench(P, args.batch, args.batch_seed, args.batch_iters, args.batch_cap)
+    if not args.no_learned:
+        line["time_to_tol_learned_vs_fixed"] = learned_bench(P, ((2000, 3), (1000, 3)), 30000)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_iters, threads)

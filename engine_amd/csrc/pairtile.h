@@ -298,9 +298,20 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
 // B[c][c] of circle c = I*T + tid, one lane per circle, run after the pair
 // sweep; their partial sums are added to the circle's row sum.  QUAL: the
 // circle's container violation and radius (FP64).
-template <int T, int MODE, bool QUAL, bool UNI = false>
-__device__ __noinline__ void diag_unary(const DArgs& a, const int t, const int I, TileShared<T>& sh, double& pr,
-                                        double& qbox, double& qsum)
+//
+// The body is a separate (not inlined) function so the pair sweep keeps its
+// registers; it takes scalars and pointers by value and returns its sums by
+// value: a reference to the kernel's argument block would make the caller
+// spill that block to scratch memory in every workgroup.
+struct UnaryAcc {
+    double pr, qbox, qsum;
+};
+
+template <int T, int MODE, bool QUAL, bool UNI>
+__device__ __noinline__ UnaryAcc diag_unary_body(float* X, float* U, const float* rho, const float* alpha,
+                                                 const float* Nb, const float* Mb, const float* Zv,
+                                                 const float* aexp, const long S, const int N, const float rho_u,
+                                                 const float alpha_u, const int t, const int I, float* srow)
 {
     constexpr int TT = T * T;
     constexpr bool APPLY_U = (MODE == DM_FUSED || MODE == DM_FINISH);
@@ -308,71 +319,83 @@ __device__ __noinline__ void diag_unary(const DArgs& a, const int t, const int I
     constexpr bool NBUF = (MODE == DM_FUSED_NBUF || MODE == DM_LEGACY_X);
     constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z);
     constexpr bool PART_M = (MODE == DM_LEGACY_Z);
+    UnaryAcc acc = {0.0, 0.0, 0.0};
     const int cl = threadIdx.x;
     const int c = I * T + cl;
-    if (c >= a.N) return;
-    const long S = a.stride;
+    if (c >= N) return acc;
     const long p[2] = {(long)t * (2 * TT) + (long)cl * T + cl, (long)t * (2 * TT) + TT + (long)cl * T + cl};
     float z[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) z[k] = a.Z[3 * c + k];
+    for (int k = 0; k < 3; ++k) z[k] = Zv[3 * c + k];
     if constexpr (QUAL) {
         const double cx = z[0], cy = z[1], r = z[2];
         const double v[5] = {r - cx, cx + r - 1.0, r - cy, cy + r - 1.0, -r};
 #pragma unroll
-        for (int k = 0; k < 5; ++k) qbox = v[k] > qbox ? v[k] : qbox;
-        qsum += r;
+        for (int k = 0; k < 5; ++k) acc.qbox = v[k] > acc.qbox ? v[k] : acc.qbox;
+        acc.qsum = r;
     }
     float part[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     float prs = 0.0f;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {  // 0: box (A), 1: radius (B)
         const long q = p[side];
-        const float r = UNI ? a.rho_u : a.rho[q];
+        const float r = UNI ? rho_u : rho[q];
         float u[3], x[3], n[3], m[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            u[k] = a.U[k * S + q];
-            if constexpr (APPLY_U) x[k] = a.X[k * S + q];
+            u[k] = U[k * S + q];
+            if constexpr (APPLY_U) x[k] = X[k * S + q];
         }
         if constexpr (APPLY_U) {
-            const float al = UNI ? a.alpha_u : a.alpha[q];
+            const float al = UNI ? alpha_u : alpha[q];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const float d = x[k] - z[k];
                 u[k] += al * d;
                 prs += d * d;
-                a.U[k * S + q] = u[k];
+                U[k * S + q] = u[k];
             }
         }
         if constexpr (PROX) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) n[k] = NBUF ? a.Nb[k * S + q] : z[k] - u[k];
+            for (int k = 0; k < 3; ++k) n[k] = NBUF ? Nb[k * S + q] : z[k] - u[k];
             if (side == 0) {
                 box3(x[0], x[1], x[2], n[0], n[1], n[2]);
             } else {
                 x[0] = n[0];
                 x[1] = n[1];
-                x[2] = max_alpha_radius(n[2], r, a.aexp[c]);
+                x[2] = max_alpha_radius(n[2], r, aexp[c]);
             }
 #pragma unroll
-            for (int k = 0; k < 3; ++k) a.X[k * S + q] = x[k];
+            for (int k = 0; k < 3; ++k) X[k * S + q] = x[k];
         }
         if constexpr (PART) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                m[k] = PART_M ? a.Mb[k * S + q] : x[k] + u[k];
+                m[k] = PART_M ? Mb[k * S + q] : x[k] + u[k];
                 part[k] += r * m[k];
             }
             part[3] += r;
             part[4] += r * r;
         }
     }
-    if constexpr (APPLY_U) pr += (double)prs;
+    if constexpr (APPLY_U) acc.pr = (double)prs;
     if constexpr (PART) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) sh.srow[k][cl] += part[k];
+        for (int k = 0; k < 5; ++k) srow[k * T + cl] += part[k];
     }
+    return acc;
+}
+
+template <int T, int MODE, bool QUAL, bool UNI = false>
+__device__ __forceinline__ void diag_unary(const DArgs& a, const int t, const int I, TileShared<T>& sh, double& pr,
+                                           double& qbox, double& qsum)
+{
+    const UnaryAcc r = diag_unary_body<T, MODE, QUAL, UNI>(a.X, a.U, a.rho, a.alpha, a.Nb, a.Mb, a.Z, a.aexp, a.stride,
+                                                           a.N, a.rho_u, a.alpha_u, t, I, &sh.srow[0][0]);
+    pr += r.pr;
+    qbox = r.qbox > qbox ? r.qbox : qbox;
+    qsum += r.qsum;
 }
 
 __device__ __forceinline__ double wmax64d(double v)
