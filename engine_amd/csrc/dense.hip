@@ -62,8 +62,9 @@ template <int MODE, bool UNI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDM_PAIRTILE_WAVES))) void k_pairtile(DArgs a)
 {
     __shared__ TileShared<DT> sh;
-    const int t = blockIdx.x;
-    const int2 ij = a.tiles[t];
+    // launch order (a.tiles) differs from layout order: t is the layout index
+    const int2 ij = a.tiles[blockIdx.x];
+    const int t = ij.x * a.nb - ij.x * (ij.x - 1) / 2 + (ij.y - ij.x);
     if constexpr (UNI) {
         // the common values live in device memory, so captured iterations
         // follow later changes of them
@@ -218,9 +219,18 @@ int dense_try_finalize(DevGraph* s, graph* G, const std::vector<int>& fkind)
     const int N = G->num_vars, E = G->num_edges;
     const int* ev = G->edgeix_to_varlistix;
     const long stride = (long)nt * 2 * DTT;
-    std::vector<int2> tiles(nt);
-    for (int I = 0, t = 0; I < nb; ++I)
-        for (int J = I; J < nb; ++J, ++t) tiles[t] = make_int2(I, J);
+    // launch order: the diagonal tiles (the longest: pairs, then the unary
+    // factors) first, the tiles of the partial last block (the shortest)
+    // last, so the final round of workgroups is the cheapest
+    std::vector<int2> tiles;
+    tiles.reserve(nt);
+    const bool ragged = N % DT != 0;
+    for (int I = 0; I < nb; ++I) tiles.push_back(make_int2(I, I));
+    for (int I = 0; I < nb; ++I)
+        for (int J = I + 1; J < nb; ++J)
+            if (!(ragged && J == nb - 1)) tiles.push_back(make_int2(I, J));
+    if (ragged)
+        for (int I = 0; I < nb - 1; ++I) tiles.push_back(make_int2(I, nb - 1));
 
     DenseState* d = new DenseState();
     s->dense = d;
