@@ -129,8 +129,8 @@ def learned_bench(P, cases, cap):
     """Time to tolerance (primal, dual, overlap, container <= 1e-4) on one
     GPU: plain fixed rho = 1 (the configured rho), plain fixed rho = 0.4 N
     (the best of the teacher's grid at large N), and the learned policy
-    (committed weights, one rho per graph, running mean of the network's
-    estimates, applied every 50 iterations from rho = 1000)."""
+    (committed weights, one rho per graph from the mean of the network's
+    per-edge estimates, applied every 50 iterations from rho = 1000)."""
     from engine_amd import rho_train as rt
     m = rt.load_policy()
     out = []
@@ -275,7 +275,7 @@ def main():
     if args.batch > 0:
         line["batch"] = batch_bench(P, args.batch, args.batch_seed, args.batch_iters, args.batch_cap)
     if not args.no_learned:
-        line["time_to_tol_learned_vs_fixed"] = learned_bench(P, ((2000, 3), (1000, 3)), 30000)
+        line["time_to_tol_learned_vs_fixed"] = learned_bench(P, ((2000, 3), (2000, 4), (1000, 3), (1000, 4)), 30000)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_iters, threads)

@@ -31,8 +31,8 @@ from . import paradmm as P
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "models", "rho_policy.f32")
 # how the committed policy is applied (paradmm_gpu_set_rho_policy_mode):
-# one rho per graph, the running mean of the network's estimates
-DEFAULT_MODE, DEFAULT_AVERAGE = 2, True
+# one rho per graph, the mean of the network's estimates over all edges
+DEFAULT_MODE, DEFAULT_AVERAGE = 2, False
 DEFAULT_MAX_ITERS = 30000
 RHO_GRID = (20.0, 35.0, 50.0, 70.0, 100.0, 150.0, 200.0, 300.0, 500.0, 800.0, 1200.0)
 # snapshot iterations of the training trajectories: dense early, then spread
@@ -80,19 +80,26 @@ STARTS = (("abs", LEARNED_START_RHO), ("rel", 1.0), ("rel", 0.7), ("rel", 4.0))
 
 
 def generate(instances, starts=STARTS, snapshots=SNAPSHOTS, max_iters=20000,
-             rows_per_snapshot=1000, rng_seed=0, log=print, grid=RHO_GRID, first_weight=1):
+             rows_per_snapshot=1000, rng_seed=0, log=print, grid=RHO_GRID, first_weight=1, known=None):
     """Rows of (features, target) for the given (n, seed) instances.  Each
     start is ("abs", rho) or ("rel", multiple of the teacher's rho*).  The
     first start's first three snapshots (the states a learned solve sees at
-    its first applications) get first_weight times the rows."""
+    its first applications) get first_weight times the rows.  known: an
+    earlier teacher record ({"n/seed": {"rho_star": ...}}) whose rho* is
+    reused instead of running the grid again."""
     rng = np.random.default_rng(rng_seed)
     feats, targets, teacher = [], [], {}
     for n, seed in instances:
         t0 = time.time()
-        rho_star, runs = best_fixed_rho(n, seed, grid=grid, max_iters=max_iters)
-        teacher[f"{n}/{seed}"] = {"rho_star": rho_star,
-                                  "iterations": {str(k): (v["iterations"] if v["converged"] else None)
-                                                 for k, v in runs.items()}}
+        key = f"{n}/{seed}"
+        if known is not None and key in known:
+            rho_star = known[key]["rho_star"]
+            teacher[key] = known[key]
+        else:
+            rho_star, runs = best_fixed_rho(n, seed, grid=grid, max_iters=max_iters)
+            teacher[key] = {"rho_star": rho_star,
+                            "iterations": {str(k): (v["iterations"] if v["converged"] else None)
+                                           for k, v in runs.items()}}
         for si, (kind, v) in enumerate(starts):
             rho0 = v * rho_star if kind == "rel" else v
             g = P.packing_square(int(n), int(seed), rho=rho0, alpha=1.0)
@@ -110,6 +117,34 @@ def generate(instances, starts=STARTS, snapshots=SNAPSHOTS, max_iters=20000,
         log(f"# teacher N={n} seed={seed}: rho*={rho_star:g} "
             f"({teacher[f'{n}/{seed}']['iterations']}), {time.time() - t0:.1f} s")
     return np.concatenate(feats).astype(np.float32), np.concatenate(targets).astype(np.float32), teacher
+
+
+def generate_on_policy(instances, policy, teacher, max_iters=20000, every=50, sample_every=500,
+                       rows_per_snapshot=1000, mode=DEFAULT_MODE, average=DEFAULT_AVERAGE, rng_seed=1, log=print):
+    """Rows from the states the learned policy itself visits (one round of
+    dataset aggregation): the policy is applied every `every` iterations
+    from LEARNED_START_RHO as in a learned solve; every `sample_every`
+    iterations the features are sampled, labelled with the instance's
+    teacher rho* (teacher: {"n/seed": rho*})."""
+    rng = np.random.default_rng(rng_seed)
+    feats, targets = [], []
+    for n, seed in instances:
+        rho_star = teacher[f"{n}/{seed}"]
+        g = P.packing_square(int(n), int(seed), rho=LEARNED_START_RHO, alpha=1.0)
+        g.prepare()
+        g.set_rho_policy_mode(mode, average)
+        for it in range(every, max_iters + 1, every):
+            g.iterate(every)
+            if it % sample_every == 0 or it <= 4 * every:
+                f = g.rho_features(it / max_iters)
+                take = rng.choice(f.shape[0], size=min(rows_per_snapshot, f.shape[0]), replace=False)
+                feats.append(f[take])
+                targets.append(np.full(take.size, np.log10(rho_star), dtype=np.float32))
+            g.apply_rho_policy(policy, it / max_iters)
+        g.sync_to_host()
+        log(f"# on-policy N={n} seed={seed}: rho*={rho_star:g}, final rho {g.array('RHOS')[0]:.1f}")
+        g.close()
+    return np.concatenate(feats).astype(np.float32), np.concatenate(targets).astype(np.float32)
 
 
 def train(feat, target, epochs=30, batch=4096, lr=2e-3, seed=0):
