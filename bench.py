@@ -112,6 +112,25 @@ def batch_bench(P, count, seed, iters, cap):
     b.close()
     for g in gs:
         g.close()
+    # the same batch with the learned rho policy (from rho = 1000, one rho per
+    # instance every 50 iterations) against configured fixed rho = 1
+    from engine_amd import rho_train as rt
+    learned = {}
+    for name, rho0, pol in (("fixed_rho1", 1.0, None), ("learned", rt.LEARNED_START_RHO, rt.load_policy())):
+        gs = [P.packing_square(n, 1000 * seed + k, rho=rho0, alpha=1.0, a_exp=1.0) for k, n in enumerate(sizes)]
+        b = P.Batch(gs)
+        b.configure(info["tile"], info["blocks_per_cu"])
+        if pol is not None:
+            b.set_rho_policy(pol)
+        r2, ms3 = b.solve(cap, tol=1e-4, tol_feas=1e-4, check_every=50, policy_every=50 if pol is not None else 0)
+        c2 = [r for r in r2 if r["converged"]]
+        learned[name] = {"converged": len(c2), "ms": ms3,
+                         "iterations_median_converged": (float(sorted(r["iterations"] for r in c2)[len(c2) // 2])
+                                                         if c2 else None)}
+        print(f"# batch {name}: {len(c2)}/{count} converged, {ms3:.1f} ms", file=sys.stderr, flush=True)
+        b.close()
+        for g in gs:
+            g.close()
     return {
         "workload": f"{count} unit-square packings, N ~ U{{16..512}} from batch seed {seed}, rho = clip(N/2, 35, 300), "
                     f"alpha 1, one persistent launch",
@@ -119,6 +138,7 @@ def batch_bench(P, count, seed, iters, cap):
         "autotune_tried": tried,
         "iters": iters, "ms_per_iter": ms / iters, "edge_updates_per_s": thr,
         "roofline_frac_72B": thr * EDGE_UPDATE_BYTES / 1e9 / HBM_PEAK_GBS,
+        "rho_policy_vs_fixed1": learned,
         "to_tol": {"cap": cap, "check_every": 50, "converged": len(conv), "launch_ms": ms2,
                    "edge_updates_per_s": done_edge_iters / (ms2 * 1e-3),
                    "iterations_median": float(sorted(r["iterations"] for r in res)[count // 2])},

@@ -40,6 +40,7 @@
 #include "engine_internal.h"
 #include "engine_state.h"
 #include "gridbar.h"
+#include "mlp_device.h"
 #include "pairtile.h"
 #include "paradmm_gpu.h"
 
@@ -74,7 +75,6 @@ struct BState {
 struct BArgs {
     int ninst;
     int max_iters, check_every;
-    int first_nbuf;  // the first pass reads N from Nb
     double tol_p, tol_d, tol_f;
     long stride;
     float *X, *U;
@@ -91,7 +91,16 @@ struct BArgs {
     unsigned* bar;        // grid barrier words (zeroed before the launch)
     unsigned spin_limit;  // polls before a barrier wait gives up
     int barrier;          // 0: cooperative-groups grid sync, 1: two-level barrier
+    // iterations [it0, it_end) of this launch; the first pass reads N from
+    // the N buffer (FIRST_NBUF), forms N = Z - U with U current
+    // (FIRST_CURRENT, after a rho policy) or applies the pending U update
+    // (FIRST_FUSED); finish_only: the pending U update and the check of
+    // iteration it0, nothing else
+    int it0, it_end, first_mode, finish_only;
+    const float* inst_rho;  // [ninst][2]: common rho, alpha of each instance
 };
+
+enum { FIRST_NBUF = 0, FIRST_CURRENT = 1, FIRST_FUSED = 2 };
 
 // Grid-wide barrier between phases; false when the two-level barrier gave up
 // (every workgroup then leaves the kernel).
@@ -112,7 +121,7 @@ struct BLists {
     int scan[3][256];
 };
 
-__device__ __forceinline__ DArgs inst_view(const BArgs& a, const BInst& in)
+__device__ __forceinline__ DArgs inst_view(const BArgs& a, const BInst& in, const int i)
 {
     DArgs v;
     v.N = in.N;
@@ -129,8 +138,9 @@ __device__ __forceinline__ DArgs inst_view(const BArgs& a, const BInst& in)
     v.Rp = nullptr;
     v.aexp = a.aexp + in.c0;
     v.tiles = nullptr;
-    v.rho_u = in.rho_u;
-    v.alpha_u = in.alpha_u;
+    v.rho_u = a.inst_rho[2 * i];
+    v.alpha_u = a.inst_rho[2 * i + 1];
+    v.uni = nullptr;
     return v;
 }
 
@@ -201,7 +211,7 @@ __device__ void phase_tiles(const BArgs& a, const BLists& L, TileShared<T>& sh)
         const int lt = w - L.tpre[k];
         const int t = in.t0 + lt;
         const int4 td = a.tiles[t];
-        const DArgs v = inst_view(a, in);
+        const DArgs v = inst_view(a, in, L.inst[k]);
         // in the steady state, instances whose edges share one rho and alpha
         // skip those streams
         if constexpr (MODE == DM_FUSED && !QUAL) {
@@ -293,16 +303,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     cg::grid_group grid = cg::this_grid();
     __shared__ BLists L;
     __shared__ TileShared<T> sh;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.t_start = wall_clock64();
+    if (a.it0 == 0 && !a.finish_only && blockIdx.x == 0 && threadIdx.x == 0) *a.t_start = wall_clock64();
     unsigned gen = 0;
     rebuild_lists(a, L);
-    for (int it = 0;; ++it) {
+    if (a.finish_only) {
+        // the pending U update of iteration it0 (its primal residual) and the
+        // quality of Z_it0, then the check of iteration it0
+        phase_tiles<T, DM_FINISH, true>(a, L, sh);
+        if (!phase_barrier(a, grid, gen)) return;
+        phase_check(a, L, a.it0, a.it0 >= a.max_iters);
+        return;
+    }
+    for (int it = a.it0;; ++it) {
+        if (it >= a.it_end) break;  // Z of iteration it is ready, its U update pending
         const bool final = it >= a.max_iters;
-        const bool check = it > 0 && (final || it % a.check_every == 0);
-        // phase A: the tiles (first pass: N from the buffer or Z - U)
+        const bool first = it == a.it0;
+        const bool check = it > 0 && (final || it % a.check_every == 0) && !(first && a.first_mode != FIRST_FUSED);
+        // phase A: the tiles (first pass: N from the buffer, or Z - U)
         if (final) phase_tiles<T, DM_FINISH, true>(a, L, sh);
-        else if (it == 0 && a.first_nbuf) phase_tiles<T, DM_FUSED_NBUF, false>(a, L, sh);
-        else if (it == 0) phase_tiles<T, DM_FUSED_CURRENT, false>(a, L, sh);
+        else if (first && a.first_mode == FIRST_NBUF) phase_tiles<T, DM_FUSED_NBUF, false>(a, L, sh);
+        else if (first && a.first_mode == FIRST_CURRENT) phase_tiles<T, DM_FUSED_CURRENT, false>(a, L, sh);
         else if (check) phase_tiles<T, DM_FUSED, true>(a, L, sh);
         else phase_tiles<T, DM_FUSED, false>(a, L, sh);
         if (!phase_barrier(a, grid, gen)) return;
@@ -315,6 +335,148 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         // phase B: the Z consensus of the instances still active
         phase_circles(a, L);
         if (!phase_barrier(a, grid, gen)) return;
+    }
+}
+
+// ---- learned rho on the batch: one rho per instance, 10^(mean of the
+// network's log10 rho over the instance's edges), between launches
+
+struct PolicySlot {
+    bool valid;
+    int c, type;  // instance-local circle, PDM_EDGE_* type
+};
+
+// slot q of tile (I, J) of an instance of N circles with tile side T
+template <int T>
+__device__ __forceinline__ PolicySlot policy_slot(const int q, const int I, const int J, const int N)
+{
+    constexpr int TT = T * T;
+    const int side = q / TT, rem = q % TT, il = rem / T, jl = rem % T;
+    const int i = I * T + il, j = J * T + jl;
+    PolicySlot s{false, 0, PDM_EDGE_PAIR};
+    if (I != J) {
+        s.valid = i < N && j < N;
+        s.c = side ? j : i;
+    } else if (il < jl) {
+        s.valid = j < N;
+        s.c = side ? j : i;
+    } else if (il == jl) {
+        s.valid = i < N;
+        s.c = i;
+        s.type = side ? PDM_EDGE_RADIUS : PDM_EDGE_CONTAINER;
+    }
+    return s;
+}
+
+struct BPolicyArgs {
+    const int* ptiles;   // active tiles (global tile ids), instance order
+    const BInst* inst;
+    const int4* tiles;
+    long stride;
+    const float* X;
+    float* U;
+    float* rho;
+    const float* Z;
+    const float* Zprev;
+    float* inst_rho;     // [ninst][2]
+    float frac;
+    double* part;        // [n ptiles] sum of log10 rho
+    int* cnt;            // [n ptiles]
+    const int2* iranges; // [n active] {instance, first ptile index}, then the end
+    int nact;
+    float* scale;        // [ninst] rho_old / rho_new
+};
+
+// pass 1: the network's log10 rho of every edge of one active tile, summed
+template <int T>
+__global__ __launch_bounds__(256) void k_batch_policy_eval(BPolicyArgs a, const float* __restrict__ params)
+{
+    constexpr int NS = 2 * T * T;
+    __shared__ MlpShared w;
+    __shared__ TileScratch tsw[4];
+    __shared__ float yv[4][64];
+    __shared__ double ssum[4];
+    __shared__ int scnt[4];
+    stage_weights(params, w);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int tg = a.ptiles[blockIdx.x];
+    const int4 td = a.tiles[tg];
+    const BInst in = a.inst[td.x];
+    const long base = in.s0 + (long)td.w * NS;
+    const float rho = a.inst_rho[2 * td.x];
+    double acc = 0.0;
+    int num = 0;
+    for (int q0 = wv * 64; q0 < NS; q0 += 256) {
+        const int q = q0 + l;
+        const PolicySlot ps = policy_slot<T>(q, td.y, td.z, in.N);
+        float r = 0.0f, sres = 0.0f, f[MLP_NIN];
+        if (ps.valid) {
+            const long p = base + q;
+            const int v = in.c0 + ps.c;
+            float r2 = 0.0f, dz2 = 0.0f;
+            for (int k = 0; k < 3; ++k) {
+                const float z = a.Z[3L * v + k];
+                const float d = a.X[k * a.stride + p] - z;
+                const float dz = z - a.Zprev[3L * v + k];
+                r2 += d * d;
+                dz2 += dz * dz;
+            }
+            r = sqrtf(r2);
+            sres = rho * sqrtf(dz2);
+        }
+        edge_features(r, sres, rho, ps.type, a.frac, f);
+        const float y = policy_wave_eval(w, tsw[wv], yv[wv], f, l);
+        acc += ps.valid ? (double)y : 0.0;
+        num += ps.valid ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        acc += __shfl_xor(acc, o, 64);
+        num += __shfl_xor(num, o, 64);
+    }
+    if (l == 0) {
+        ssum[wv] = acc;
+        scnt[wv] = num;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.part[blockIdx.x] = (ssum[0] + ssum[1]) + (ssum[2] + ssum[3]);
+        a.cnt[blockIdx.x] = scnt[0] + scnt[1] + scnt[2] + scnt[3];
+    }
+}
+
+// pass 2: one thread per active instance, its tiles' sums in fixed order
+__global__ __launch_bounds__(256) void k_batch_policy_reduce(BPolicyArgs a)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= a.nact) return;
+    const int i = a.iranges[k].x;
+    double sum = 0.0;
+    long n = 0;
+    for (int t = a.iranges[k].y; t < a.iranges[k + 1].y; ++t) {
+        sum += a.part[t];
+        n += a.cnt[t];
+    }
+    const float rn = exp10f((float)(sum / (double)(n ? n : 1)));
+    a.scale[i] = a.inst_rho[2 * i] / rn;
+    a.inst_rho[2 * i] = rn;
+}
+
+// pass 3: U_e <- U_e rho_old / rho_new, rho_e <- rho_new on every edge
+template <int T>
+__global__ __launch_bounds__(256) void k_batch_policy_apply(BPolicyArgs a)
+{
+    constexpr int NS = 2 * T * T;
+    const int tg = a.ptiles[blockIdx.x];
+    const int4 td = a.tiles[tg];
+    const BInst in = a.inst[td.x];
+    const long base = in.s0 + (long)td.w * NS;
+    const float sc = a.scale[td.x], rn = a.inst_rho[2 * td.x];
+    for (int q = threadIdx.x; q < NS; q += 256) {
+        if (!policy_slot<T>(q, td.y, td.z, in.N).valid) continue;
+        const long p = base + q;
+        for (int k = 0; k < 3; ++k) a.U[k * a.stride + p] *= sc;
+        a.rho[p] = rn;
     }
 }
 
@@ -350,6 +512,14 @@ struct paradmm_batch {
     unsigned* bar = nullptr;
     int barrier = 1;  // two-level barrier by default
     bool solved = false;
+    // learned rho: one rho per instance between launches
+    paradmm_rho_mlp* policy = nullptr;
+    float* inst_rho = nullptr;  // [ninst][2] device
+    int* ptiles = nullptr;      // [ntiles] device
+    int2* iranges = nullptr;    // [ninst + 1] device
+    double* ppart = nullptr;    // [ntiles]
+    int* pcnt = nullptr;        // [ntiles]
+    float* pscale = nullptr;    // [ninst]
 };
 
 static void batch_free_dev(paradmm_batch* b)
@@ -449,6 +619,12 @@ static int batch_layout(paradmm_batch* b, int T)
     RC(bmalloc(b, (void**)&b->st, sizeof(BState) * (size_t)b->ninst));
     RC(bmalloc(b, (void**)&b->t_start, sizeof(long long)));
     RC(bmalloc(b, (void**)&b->bar, sizeof(unsigned) * GRIDBAR_WORDS));
+    RC(bmalloc(b, (void**)&b->inst_rho, sizeof(float) * 2 * (size_t)b->ninst));
+    RC(bmalloc(b, (void**)&b->ptiles, sizeof(int) * (size_t)t0));
+    RC(bmalloc(b, (void**)&b->iranges, sizeof(int2) * ((size_t)b->ninst + 1)));
+    RC(bmalloc(b, (void**)&b->ppart, sizeof(double) * (size_t)t0));
+    RC(bmalloc(b, (void**)&b->pcnt, sizeof(int) * (size_t)t0));
+    RC(bmalloc(b, (void**)&b->pscale, sizeof(float) * (size_t)b->ninst));
     CK(hipMemcpy(b->inst_d, b->inst.data(), sizeof(BInst) * (size_t)b->ninst, hipMemcpyHostToDevice));
     CK(hipMemcpy(b->tiles_d, tiles.data(), sizeof(int4) * (size_t)t0, hipMemcpyHostToDevice));
     CK(hipMemcpy(b->aexp_d, b->aexp.data(), sizeof(float) * (size_t)c0, hipMemcpyHostToDevice));
@@ -510,6 +686,12 @@ static int batch_upload(paradmm_batch* b)
             bi.uni = (float)G->RHOS[e] == bi.rho_u && (float)G->ALPHAS[e] == bi.alpha_u;
     }
     CK(hipMemcpyAsync(b->inst_d, b->inst.data(), sizeof(BInst) * (size_t)b->ninst, hipMemcpyHostToDevice, b->stream));
+    std::vector<float> ir(2 * (size_t)b->ninst);
+    for (int i = 0; i < b->ninst; ++i) {
+        ir[2 * i] = b->inst[i].rho_u;
+        ir[2 * i + 1] = b->inst[i].alpha_u;
+    }
+    CK(hipMemcpyAsync(b->inst_rho, ir.data(), sizeof(float) * ir.size(), hipMemcpyHostToDevice, b->stream));
     CK(hipMemcpyAsync(b->X, x.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice, b->stream));
     CK(hipMemcpyAsync(b->U, u.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice, b->stream));
     CK(hipMemcpyAsync(b->Nb, n.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice, b->stream));
@@ -520,13 +702,19 @@ static int batch_upload(paradmm_batch* b)
     return batch_reset_flags(b);
 }
 
-static int batch_launch(paradmm_batch* b, const paradmm_solve_opts* o, float* ms)
+// One cooperative launch over iterations [it0, it_end) (see BArgs).
+static int batch_launch(paradmm_batch* b, const paradmm_solve_opts* o, float* ms, int it0 = 0,
+                        int it_end = 0x7fffffff, int first_mode = FIRST_NBUF, int finish_only = 0)
 {
     BArgs a;
     a.ninst = b->ninst;
     a.max_iters = o->max_iters;
     a.check_every = o->check_every;
-    a.first_nbuf = 1;
+    a.it0 = it0;
+    a.it_end = it_end;
+    a.first_mode = first_mode;
+    a.finish_only = finish_only;
+    a.inst_rho = b->inst_rho;
     a.tol_p = o->tol_primal;
     a.tol_d = o->tol_dual;
     a.tol_f = o->tol_feas;
@@ -563,6 +751,71 @@ static int batch_launch(paradmm_batch* b, const paradmm_solve_opts* o, float* ms
     if (aborted) {
         pdm_set_error(PARADMM_ERR_HIP, "batch: a grid barrier wait timed out (workgroups not co-resident?)");
         return PARADMM_ERR_HIP;
+    }
+    return PARADMM_OK;
+}
+
+// The learned-rho batch: launches of policy_every iterations; after each,
+// the pending U update and the check of every active instance (a
+// finish-only launch), then the policy on the instances still active.
+static int batch_policy_solve(paradmm_batch* b, const paradmm_solve_opts* o, std::vector<BState>& st, float* total_ms)
+{
+    const int K = o->policy_every;
+    float ms = 0.0f;
+    *total_ms = 0.0f;
+    int it = 0, first = FIRST_NBUF;
+    for (;;) {
+        const int end = std::min(it + K, o->max_iters);
+        RC(batch_launch(b, o, &ms, it, end, first));
+        *total_ms += ms;
+        it = end;
+        RC(batch_launch(b, o, &ms, it, it, FIRST_FUSED, 1));
+        *total_ms += ms;
+        CK(hipMemcpy(st.data(), b->st, sizeof(BState) * st.size(), hipMemcpyDeviceToHost));
+        if (it >= o->max_iters) break;
+        // active tiles in instance order, and each active instance's range
+        std::vector<int> pt;
+        std::vector<int2> ir;
+        for (int i = 0; i < b->ninst; ++i) {
+            if (!st[i].active) continue;
+            ir.push_back(make_int2(i, (int)pt.size()));
+            for (int t = 0; t < b->inst[i].nt; ++t) pt.push_back(b->inst[i].t0 + t);
+        }
+        if (ir.empty()) break;
+        ir.push_back(make_int2(-1, (int)pt.size()));
+        const auto t0 = std::chrono::steady_clock::now();
+        CK(hipMemcpyAsync(b->ptiles, pt.data(), sizeof(int) * pt.size(), hipMemcpyHostToDevice, b->stream));
+        CK(hipMemcpyAsync(b->iranges, ir.data(), sizeof(int2) * ir.size(), hipMemcpyHostToDevice, b->stream));
+        BPolicyArgs pa;
+        pa.ptiles = b->ptiles;
+        pa.inst = b->inst_d;
+        pa.tiles = b->tiles_d;
+        pa.stride = b->slots;
+        pa.X = b->X;
+        pa.U = b->U;
+        pa.rho = b->rho;
+        pa.Z = b->Z;
+        pa.Zprev = b->Zprev;
+        pa.inst_rho = b->inst_rho;
+        pa.frac = (float)it / (float)o->max_iters;
+        pa.part = b->ppart;
+        pa.cnt = b->pcnt;
+        pa.iranges = b->iranges;
+        pa.nact = (int)ir.size() - 1;
+        pa.scale = b->pscale;
+        const unsigned npt = (unsigned)pt.size();
+        const float* params = mlp_device_params(b->policy);
+        if (b->T == 32) k_batch_policy_eval<32><<<npt, 256, 0, b->stream>>>(pa, params);
+        else k_batch_policy_eval<64><<<npt, 256, 0, b->stream>>>(pa, params);
+        CK(hipGetLastError());
+        k_batch_policy_reduce<<<(unsigned)((pa.nact + 255) / 256), 256, 0, b->stream>>>(pa);
+        CK(hipGetLastError());
+        if (b->T == 32) k_batch_policy_apply<32><<<npt, 256, 0, b->stream>>>(pa);
+        else k_batch_policy_apply<64><<<npt, 256, 0, b->stream>>>(pa);
+        CK(hipGetLastError());
+        CK(hipStreamSynchronize(b->stream));
+        *total_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        first = FIRST_CURRENT;
     }
     return PARADMM_OK;
 }
@@ -689,15 +942,20 @@ extern "C" int paradmm_batch_solve(paradmm_batch* b, const paradmm_solve_opts* o
         return PARADMM_ERR_ARG;
     }
     RC(check_opts(o));
-    if (o->policy_every > 0) {
-        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_solve: the batch runs the plain fixed-rho method (policy_every 0)");
+    if (o->policy_every > 0 && (!b->policy || o->check_every != o->policy_every)) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_solve: policy_every > 0 needs a policy "
+                                       "(paradmm_batch_set_rho_policy) and check_every == policy_every");
         return PARADMM_ERR_ARG;
     }
     if (!b->T) RC(paradmm_batch_configure(b, 64, 1, 1));
     RC(batch_upload(b));
     float ms = 0.0f;
-    RC(batch_launch(b, o, &ms));
     std::vector<BState> st(b->ninst);
+    if (o->policy_every <= 0) {
+        RC(batch_launch(b, o, &ms));
+    } else {
+        RC(batch_policy_solve(b, o, st, &ms));
+    }
     long long t0 = 0;
     CK(hipMemcpy(st.data(), b->st, sizeof(BState) * st.size(), hipMemcpyDeviceToHost));
     CK(hipMemcpy(&t0, b->t_start, sizeof t0, hipMemcpyDeviceToHost));
@@ -714,6 +972,16 @@ extern "C" int paradmm_batch_solve(paradmm_batch* b, const paradmm_solve_opts* o
     }
     if (kernel_ms) *kernel_ms = ms;
     b->solved = true;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_batch_set_rho_policy(paradmm_batch* b, paradmm_rho_mlp* m)
+{
+    if (!b) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_set_rho_policy: NULL batch");
+        return PARADMM_ERR_ARG;
+    }
+    b->policy = m;
     return PARADMM_OK;
 }
 

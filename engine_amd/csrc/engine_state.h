@@ -7,7 +7,12 @@
 
 #include "paradmm.h"
 
+struct paradmm_rho_mlp;
+
 namespace pdm {
+
+// device parameter vector of a rho network (mlp.hip)
+const float* mlp_device_params(paradmm_rho_mlp* m);
 
 struct KindInfo {
     int deg;      // edges per factor; 0 = any degree, each edge independent

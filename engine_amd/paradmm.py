@@ -135,6 +135,7 @@ def load():
                                              _IP]),
         "paradmm_batch_solve": (C.c_int, [V, C.POINTER(SolveOpts), C.POINTER(SolveResult), _DP]),
         "paradmm_batch_sync_to_host": (C.c_int, [V]),
+        "paradmm_batch_set_rho_policy": (C.c_int, [V, V]),
         "paradmm_batch_info": (C.c_int, [V, _IP, _IP, _IP, C.POINTER(C.c_long), _IP]),
         "paradmm_rho_mlp_get_params": (C.c_int, [V, _FP, C.c_int]),
         "paradmm_rho_mlp_forward": (C.c_int, [V, _FP, C.c_int, _FP]),
@@ -463,8 +464,13 @@ class Batch:
               "paradmm_batch_autotune")
         return chosen.as_dict(), [tried[i].as_dict() for i in range(n.value)]
 
-    def solve(self, max_iters, tol=1e-4, tol_feas=1e-4, check_every=50):
-        o = SolveOpts(max_iters, check_every, tol, tol, tol_feas, 0)
+    def set_rho_policy(self, mlp):
+        check(self.lib.paradmm_batch_set_rho_policy(self.h, mlp.h if mlp is not None else None),
+              "paradmm_batch_set_rho_policy")
+        self._policy = mlp
+
+    def solve(self, max_iters, tol=1e-4, tol_feas=1e-4, check_every=50, policy_every=0):
+        o = SolveOpts(max_iters, check_every, tol, tol, tol_feas, policy_every)
         res = (SolveResult * len(self.graphs))()
         ms = C.c_double()
         check(self.lib.paradmm_batch_solve(self.h, C.byref(o), res, C.byref(ms)), "paradmm_batch_solve")

@@ -242,9 +242,17 @@ int paradmm_batch_autotune(paradmm_batch *b, int iters, paradmm_batch_config *ch
 /* Uploads every graph's host state (X, U, N, Z, RHOS, ALPHAS) and solves;
  * res[i] per instance (iterations at retirement, residuals and quality of the
  * last check, elapsed_ms from the launch start to its retirement, device
- * clock); *kernel_ms the whole launch.  opts->policy_every must be 0. */
+ * clock); *kernel_ms the whole launch (with a rho policy: the launches and
+ * the policy steps). */
 int paradmm_batch_solve(paradmm_batch *b, const paradmm_solve_opts *opts, paradmm_solve_result *res,
                         double *kernel_ms);
+/* Learned rho on the batch (m = NULL detaches): with opts->policy_every > 0
+ * (and check_every == policy_every) the batch runs policy_every iterations
+ * per cooperative launch; after each, every active instance is checked and
+ * then takes one rho, 10^(mean of the network's log10 rho over its edges),
+ * with U rescaled by rho_old / rho_new -- the per-graph mode of
+ * paradmm_gpu_set_rho_policy_mode applied per instance. */
+int paradmm_batch_set_rho_policy(paradmm_batch *b, paradmm_rho_mlp *m);
 /* Writes Z, X, U, RHOS (and M = X + U, N = Z - U) back into every graph. */
 int paradmm_batch_sync_to_host(paradmm_batch *b);
 int paradmm_batch_info(paradmm_batch *b, int *tile, int *blocks_per_cu, int *grid, long *slots, int *tiles);

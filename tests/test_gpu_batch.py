@@ -215,3 +215,54 @@ def test_batch_barrier_kinds_agree():
     for other in zs[1:]:
         for a, c in zip(zs[0], other):
             assert np.array_equal(a, c)
+
+
+def test_batch_rho_policy_matches_single_instance():
+    """Learned rho on the batch (one rho per instance from the network's mean
+    estimate every 50 iterations) follows the single-instance solver with the
+    same policy in its per-graph mode."""
+    from engine_amd import rho_train as rt
+    m = rt.load_policy()
+    specs = [(64, 3), (120, 5)]
+    gs = [P.packing_square(n, s, rho=rt.LEARNED_START_RHO) for n, s in specs]
+    b = P.Batch(gs)
+    b.configure(64, 1, 1)
+    b.set_rho_policy(m)
+    res, ms = b.solve(150, tol=-1.0, tol_feas=-1.0, check_every=50, policy_every=50)
+    b.sync_to_host()
+    for (n, s), g, r in zip(specs, gs, res):
+        assert r["iterations"] == 150
+        one = P.packing_square(n, s, rho=rt.LEARNED_START_RHO)
+        one.prepare()
+        one.set_rho_policy(m)
+        one.set_rho_policy_mode(2, False)
+        one.solve(150, tol=-1.0, tol_feas=-1.0, check_every=50, policy_every=50)
+        one.sync_to_host()
+        err = max_rel(g.array("Z"), one.array("Z"))
+        rb = g.array("RHOS")
+        print(f"N={n}: batch rho {rb[0]:.2f} (one value: {np.all(rb == rb[0])}), Z vs single {err:.2e}")
+        assert np.all(rb == rb[0]) and rb[0] != rt.LEARNED_START_RHO
+        assert err < 1e-4
+
+
+def test_batch_rho_policy_retires_converged_instances():
+    from engine_amd import rho_train as rt
+    m = rt.load_policy()
+    specs = [(64, 3), (64, 4), (30, 1)]
+    gs = [P.packing_square(n, s, rho=rt.LEARNED_START_RHO) for n, s in specs]
+    b = P.Batch(gs)
+    b.set_rho_policy(m)
+    res, ms = b.solve(30000, tol=1e-4, tol_feas=1e-4, check_every=50, policy_every=50)
+    b.sync_to_host()
+    print("batch with learned rho:", [(r["iterations"], r["converged"]) for r in res], f"{ms:.0f} ms")
+    assert sum(r["converged"] for r in res) >= 2
+    for (n, s), g, r in zip(specs, gs, res):
+        if not r["converged"]:
+            continue
+        assert r["primal_rms"] <= 1e-4 and r["dual_rms"] <= 1e-4
+        assert r["max_overlap"] <= 1e-4 and r["max_box"] <= 1e-4
+        o = po.Graph.square(n, s, 1.0)
+        o.array("Z")[:] = g.array("Z")
+        ov, bx, sr = o.square_quality(n)
+        assert abs(ov - r["max_overlap"]) < 1e-6 and abs(sr - r["objective"]) < 1e-5
+        o.close()

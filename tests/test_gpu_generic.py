@@ -1,7 +1,7 @@
 """GPU parity of the generic (CSR) layout against the FP64 oracle, through
 the C ABI.  Tolerances are fp32 vs fp64: single passes 2e-5 of the array's
 scale (a handful of fp32 roundings on inputs in [-1, 2]); Z trajectories
-1e-5 relative (the north-star bound) where the dynamics allow it."""
+1e-5 relative where the dynamics allow it."""
 import numpy as np
 import pytest
 
