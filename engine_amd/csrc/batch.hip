@@ -33,6 +33,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -89,6 +90,7 @@ struct BArgs {
     BState* st;
     long long* t_start;
     unsigned* bar;        // grid barrier words (zeroed before the launch)
+    const int* order;     // [ninst] the order instances are handed out in
     unsigned spin_limit;  // polls before a barrier wait gives up
     int barrier;          // 0: cooperative-groups grid sync, 1: two-level barrier
     // iterations [it0, it_end) of this launch; the first pass reads N from
@@ -144,16 +146,16 @@ __device__ __forceinline__ DArgs inst_view(const BArgs& a, const BInst& in, cons
     return v;
 }
 
-// Active instances in index order with prefix sums of their tile and circle
-// counts; every workgroup builds the same lists from st[].active.
+// Active instances in the launch's order with prefix sums of their tile and
+// circle counts; every workgroup builds the same lists from st[].active.
 __device__ void rebuild_lists(const BArgs& a, BLists& L)
 {
     const int tid = threadIdx.x;
     const int per = (a.ninst + 255) / 256;
     const int lo = min(tid * per, a.ninst), hi = min(lo + per, a.ninst);
     int na = 0, nt = 0, nc = 0;
-    for (int i = lo; i < hi; ++i)
-        if (__hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    for (int j = lo; j < hi; ++j)
+        if (const int i = a.order[j]; __hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             ++na;
             nt += a.inst[i].nt;
             nc += a.inst[i].N;
@@ -172,8 +174,8 @@ __device__ void rebuild_lists(const BArgs& a, BLists& L)
     }
     __syncthreads();
     int k = L.scan[0][tid], tp = L.scan[1][tid], cp = L.scan[2][tid];
-    for (int i = lo; i < hi; ++i)
-        if (__hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    for (int j = lo; j < hi; ++j)
+        if (const int i = a.order[j]; __hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             L.inst[k] = i;
             L.tpre[k] = tp;
             L.cpre[k] = cp;
@@ -201,61 +203,120 @@ __device__ __forceinline__ int find_prefix(const int* pre, int n, int w)
     return lo;
 }
 
-template <int T, int MODE, bool QUAL>
-__device__ void phase_tiles(const BArgs& a, const BLists& L, TileShared<T>& sh)
+// Z consensus of circle c of one instance: the block partials summed in
+// fixed order; the dual-residual term rho^2 |Z - Zprev|^2 of the circle.
+__device__ __forceinline__ void circle_consensus(const BArgs& a, const BInst& in, const int c)
+{
+    const float* P = a.P + in.p0;
+    const long N = in.N;
+    float s[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int K = 0; K < in.nb; ++K)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s[q] += P[((long)q * in.nb + K) * N + c];
+    float* Z = a.Z + 3L * (in.c0 + c);
+    float* Zp = a.Zprev + 3L * (in.c0 + c);
+    double dz2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const float z = s[q] / s[3];
+        const float zo = Z[q];
+        const double d = (double)(z - zo);
+        dz2 += d * d;
+        Zp[q] = zo;
+        Z[q] = z;
+    }
+    a.Rd[in.c0 + c] = (double)s[4] * dz2;
+}
+
+// phase A for tile side 64: one workgroup per tile
+template <int MODE, bool QUAL>
+__device__ __forceinline__ void phase_tiles_block(const BArgs& a, const BLists& L, TileShared<64>& sh)
 {
     const int ntiles = L.tpre[L.n];
     for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const int k = find_prefix(L.tpre, L.n, w);
-        const BInst in = a.inst[L.inst[k]];
+        const int i = L.inst[k];
+        const BInst in = a.inst[i];
         const int lt = w - L.tpre[k];
         const int t = in.t0 + lt;
         const int4 td = a.tiles[t];
-        const DArgs v = inst_view(a, in, L.inst[k]);
+        const DArgs v = inst_view(a, in, i);
         // in the steady state, instances whose edges share one rho and alpha
         // skip those streams
         if constexpr (MODE == DM_FUSED && !QUAL) {
-            if (in.uni) tile_pass<T, MODE, QUAL, true>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
-            else tile_pass<T, MODE, QUAL, false>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
+            if (in.uni) tile_pass<64, MODE, QUAL, true>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
+            else tile_pass<64, MODE, QUAL, false>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
         } else {
-            tile_pass<T, MODE, QUAL, false>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
+            tile_pass<64, MODE, QUAL, false>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
         }
         __syncthreads();  // the tile's shared scratch is reused by the next tile
     }
 }
 
-// Z consensus of every active circle, one thread per circle, blocks summed
-// in fixed order.
+// phase A for tile side 32: one wavefront per tile, waves independent
+template <int MODE, bool QUAL>
+__device__ __forceinline__ void phase_tiles_wave(const BArgs& a, const BLists& L, WaveShared& ws)
+{
+    const int ntiles = L.tpre[L.n];
+    const int nw = gridDim.x * 4;
+    for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < ntiles; w += nw) {
+        const int k = find_prefix(L.tpre, L.n, w);
+        const int i = L.inst[k];
+        const BInst in = a.inst[i];
+        const int lt = w - L.tpre[k];
+        const int t = in.t0 + lt;
+        const int4 td = a.tiles[t];
+        const DArgs v = inst_view(a, in, i);
+        if constexpr (MODE == DM_FUSED && !QUAL) {
+            if (in.uni) wave_tile_pass<MODE, QUAL, true>(v, lt, td.y, td.z, ws, a.Rp + t, a.Q + 3L * t);
+            else wave_tile_pass<MODE, QUAL, false>(v, lt, td.y, td.z, ws, a.Rp + t, a.Q + 3L * t);
+        } else {
+            wave_tile_pass<MODE, QUAL, false>(v, lt, td.y, td.z, ws, a.Rp + t, a.Q + 3L * t);
+        }
+    }
+}
+
+// The steady-state pass is inlined into the kernel; the passes of the first iteration, of checks and of the final
+// iteration are calls, so their registers do not weigh on the steady state
+// (a call saves and restores the registers it uses, which only these rare
+// passes pay for).
+template <int MODE, bool QUAL>
+__device__ __noinline__ void phase_tiles_wave_call(const BArgs& a, const BLists& L, WaveShared& ws)
+{
+    phase_tiles_wave<MODE, QUAL>(a, L, ws);
+}
+template <int MODE, bool QUAL>
+__device__ __noinline__ void phase_tiles_block_call(const BArgs& a, const BLists& L, TileShared<64>& sh)
+{
+    phase_tiles_block<MODE, QUAL>(a, L, sh);
+}
+
+// `a` is the kernel's argument block (inlined use only), `sa` its copy in
+// LDS for the calls: a callee given a reference to the argument block would
+// make every workgroup copy the block to scratch.
+template <int T, int MODE, bool QUAL>
+__device__ __forceinline__ void phase_tiles(const BArgs& a, const BArgs& sa, const BLists& L, TileShared<64>& sh,
+                                            WaveShared* ws)
+{
+    constexpr bool STEADY = MODE == DM_FUSED && !QUAL;
+    if constexpr (T == 32) {
+        if constexpr (STEADY) phase_tiles_wave<MODE, QUAL>(a, L, ws[threadIdx.x >> 6]);
+        else phase_tiles_wave_call<MODE, QUAL>(sa, L, ws[threadIdx.x >> 6]);
+    } else {
+        if constexpr (STEADY) phase_tiles_block<MODE, QUAL>(a, L, sh);
+        else phase_tiles_block_call<MODE, QUAL>(sa, L, sh);
+    }
+}
+
+// Z consensus of every active circle, one thread per circle.
 __device__ void phase_circles(const BArgs& a, const BLists& L)
 {
     const int ncirc = L.cpre[L.n];
     for (int w = blockIdx.x * 256 + threadIdx.x; w < ncirc; w += gridDim.x * 256) {
         const int k = find_prefix(L.cpre, L.n, w);
-        const BInst in = a.inst[L.inst[k]];
-        const int c = w - L.cpre[k];
-        const float* P = a.P + in.p0;
-        const long N = in.N;
-        float s[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        for (int K = 0; K < in.nb; ++K)
-#pragma unroll
-            for (int q = 0; q < 5; ++q) s[q] += P[((long)q * in.nb + K) * N + c];
-        float* Z = a.Z + 3L * (in.c0 + c);
-        float* Zp = a.Zprev + 3L * (in.c0 + c);
-        double dz2 = 0.0;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const float z = s[q] / s[3];
-            const float zo = Z[q];
-            const double d = (double)(z - zo);
-            dz2 += d * d;
-            Zp[q] = zo;
-            Z[q] = z;
-        }
-        a.Rd[in.c0 + c] = (double)s[4] * dz2;
+        circle_consensus(a, a.inst[L.inst[k]], w - L.cpre[k]);
     }
 }
-
-// One wavefront per active instance: residuals, quality, retirement.
 __device__ void phase_check(const BArgs& a, const BLists& L, int it, bool final)
 {
     const int lane = threadIdx.x & 63;
@@ -302,14 +363,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 {
     cg::grid_group grid = cg::this_grid();
     __shared__ BLists L;
-    __shared__ TileShared<T> sh;
+    // tile scratch: one block tile (T = 64) or four wave tiles (T = 32)
+    __shared__ TileShared<64> sh;
+    __shared__ WaveShared ws[4];
+    __shared__ BArgs sa;
+    if (threadIdx.x == 0) sa = a;
+    __syncthreads();
     if (a.it0 == 0 && !a.finish_only && blockIdx.x == 0 && threadIdx.x == 0) *a.t_start = wall_clock64();
     unsigned gen = 0;
     rebuild_lists(a, L);
     if (a.finish_only) {
         // the pending U update of iteration it0 (its primal residual) and the
         // quality of Z_it0, then the check of iteration it0
-        phase_tiles<T, DM_FINISH, true>(a, L, sh);
+        phase_tiles<T, DM_FINISH, true>(a, sa, L, sh, ws);
         if (!phase_barrier(a, grid, gen)) return;
         phase_check(a, L, a.it0, a.it0 >= a.max_iters);
         return;
@@ -320,11 +386,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const bool first = it == a.it0;
         const bool check = it > 0 && (final || it % a.check_every == 0) && !(first && a.first_mode != FIRST_FUSED);
         // phase A: the tiles (first pass: N from the buffer, or Z - U)
-        if (final) phase_tiles<T, DM_FINISH, true>(a, L, sh);
-        else if (first && a.first_mode == FIRST_NBUF) phase_tiles<T, DM_FUSED_NBUF, false>(a, L, sh);
-        else if (first && a.first_mode == FIRST_CURRENT) phase_tiles<T, DM_FUSED_CURRENT, false>(a, L, sh);
-        else if (check) phase_tiles<T, DM_FUSED, true>(a, L, sh);
-        else phase_tiles<T, DM_FUSED, false>(a, L, sh);
+        if (final) phase_tiles<T, DM_FINISH, true>(a, sa, L, sh, ws);
+        else if (first && a.first_mode == FIRST_NBUF) phase_tiles<T, DM_FUSED_NBUF, false>(a, sa, L, sh, ws);
+        else if (first && a.first_mode == FIRST_CURRENT)
+            phase_tiles<T, DM_FUSED_CURRENT, false>(a, sa, L, sh, ws);
+        else if (check) phase_tiles<T, DM_FUSED, true>(a, sa, L, sh, ws);
+        else phase_tiles<T, DM_FUSED, false>(a, sa, L, sh, ws);
         if (!phase_barrier(a, grid, gen)) return;
         if (check) {
             phase_check(a, L, it, final);
@@ -509,7 +576,9 @@ struct paradmm_batch {
     double *Rp = nullptr, *Rd = nullptr, *Q = nullptr;
     BState* st = nullptr;
     long long* t_start = nullptr;
-    unsigned* bar = nullptr;
+    unsigned* bar = nullptr;  // grid barrier words
+    size_t bar_words = 0;
+    int* order_d = nullptr;   // instances by descending tile count
     int barrier = 1;  // two-level barrier by default
     bool solved = false;
     // learned rho: one rho per instance between launches
@@ -618,7 +687,9 @@ static int batch_layout(paradmm_batch* b, int T)
     RC(bmalloc(b, (void**)&b->Rd, sizeof(double) * (size_t)c0));
     RC(bmalloc(b, (void**)&b->st, sizeof(BState) * (size_t)b->ninst));
     RC(bmalloc(b, (void**)&b->t_start, sizeof(long long)));
-    RC(bmalloc(b, (void**)&b->bar, sizeof(unsigned) * GRIDBAR_WORDS));
+    b->bar_words = (size_t)GRIDBAR_WORDS;
+    RC(bmalloc(b, (void**)&b->bar, sizeof(unsigned) * b->bar_words));
+    RC(bmalloc(b, (void**)&b->order_d, sizeof(int) * (size_t)b->ninst));
     RC(bmalloc(b, (void**)&b->inst_rho, sizeof(float) * 2 * (size_t)b->ninst));
     RC(bmalloc(b, (void**)&b->ptiles, sizeof(int) * (size_t)t0));
     RC(bmalloc(b, (void**)&b->iranges, sizeof(int2) * ((size_t)b->ninst + 1)));
@@ -627,6 +698,11 @@ static int batch_layout(paradmm_batch* b, int T)
     RC(bmalloc(b, (void**)&b->pscale, sizeof(float) * (size_t)b->ninst));
     CK(hipMemcpy(b->inst_d, b->inst.data(), sizeof(BInst) * (size_t)b->ninst, hipMemcpyHostToDevice));
     CK(hipMemcpy(b->tiles_d, tiles.data(), sizeof(int4) * (size_t)t0, hipMemcpyHostToDevice));
+    // larger instances first, so the tiles handed out last are small ones
+    std::vector<int> order(b->ninst);
+    for (int i = 0; i < b->ninst; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->inst[x].nt > b->inst[y].nt; });
+    CK(hipMemcpy(b->order_d, order.data(), sizeof(int) * (size_t)b->ninst, hipMemcpyHostToDevice));
     CK(hipMemcpy(b->aexp_d, b->aexp.data(), sizeof(float) * (size_t)c0, hipMemcpyHostToDevice));
     // partials of retired or padded circles are never read, but keep them finite
     CK(hipMemset(b->P, 0, sizeof(float) * (size_t)p0));
@@ -736,10 +812,11 @@ static int batch_launch(paradmm_batch* b, const paradmm_solve_opts* o, float* ms
     a.st = b->st;
     a.t_start = b->t_start;
     a.bar = b->bar;
+    a.order = b->order_d;
     a.spin_limit = 1u << 24;
     a.barrier = b->barrier;
     void* args[] = {&a};
-    CK(hipMemsetAsync(b->bar, 0, sizeof(unsigned) * GRIDBAR_WORDS, b->stream));
+    CK(hipMemsetAsync(b->bar, 0, sizeof(unsigned) * b->bar_words, b->stream));
     const void* fn = b->T == 32 ? (const void*)k_batch_solve<32> : (const void*)k_batch_solve<64>;
     CK(hipEventRecord(b->ev0, b->stream));
     CK(hipLaunchCooperativeKernel(fn, dim3(b->grid), dim3(256), args, 0, b->stream));

@@ -311,7 +311,8 @@ template <int T, int MODE, bool QUAL, bool UNI>
 __device__ __noinline__ UnaryAcc diag_unary_body(float* X, float* U, const float* rho, const float* alpha,
                                                  const float* Nb, const float* Mb, const float* Zv,
                                                  const float* aexp, const long S, const int N, const float rho_u,
-                                                 const float alpha_u, const int t, const int I, float* srow)
+                                                 const float alpha_u, const int t, const int I, float* srow,
+                                                 const int cl)
 {
     constexpr int TT = T * T;
     constexpr bool APPLY_U = (MODE == DM_FUSED || MODE == DM_FINISH);
@@ -320,7 +321,6 @@ __device__ __noinline__ UnaryAcc diag_unary_body(float* X, float* U, const float
     constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z);
     constexpr bool PART_M = (MODE == DM_LEGACY_Z);
     UnaryAcc acc = {0.0, 0.0, 0.0};
-    const int cl = threadIdx.x;
     const int c = I * T + cl;
     if (c >= N) return acc;
     const long p[2] = {(long)t * (2 * TT) + (long)cl * T + cl, (long)t * (2 * TT) + TT + (long)cl * T + cl};
@@ -392,7 +392,8 @@ __device__ __forceinline__ void diag_unary(const DArgs& a, const int t, const in
                                            double& qbox, double& qsum)
 {
     const UnaryAcc r = diag_unary_body<T, MODE, QUAL, UNI>(a.X, a.U, a.rho, a.alpha, a.Nb, a.Mb, a.Z, a.aexp, a.stride,
-                                                           a.N, a.rho_u, a.alpha_u, t, I, &sh.srow[0][0]);
+                                                           a.N, a.rho_u, a.alpha_u, t, I, &sh.srow[0][0],
+                                                           (int)threadIdx.x);
     pr += r.pr;
     qbox = r.qbox > qbox ? r.qbox : qbox;
     qsum += r.qsum;
@@ -497,6 +498,260 @@ __device__ __forceinline__ void tile_pass(const DArgs& a, const int t, const int
             q3[2] = qsum;
         }
     }
+}
+
+
+// ---- one wavefront per 32 x 32 tile --------------------------------------
+//
+// The same pass for tile side 32 done by a single wavefront, so the waves of
+// a workgroup work on different tiles with no workgroup barrier between
+// them (the batched solver hands tiles to waves).  Lane l owns columns
+// 4 (l % 8) .. +3 of rows 4 (l / 8) .. +3: the per-lane work of the
+// 64-circle workgroup tile.  Row and column partials meet in the wave's own
+// LDS scratch.
+constexpr int WT = 32;
+constexpr int WTT = WT * WT;
+
+struct WaveShared {
+    float srow[5][WT];
+    float scol[5][WT];
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int MODE, int VAR, bool QUAL, bool UNI>
+__device__ __forceinline__ void wave_pair_tile(const DArgs& a, const int t, const int I, const int J, WaveShared& ws,
+                                               double& pr, double& qov)
+{
+    constexpr bool APPLY_U = (MODE == DM_FUSED || MODE == DM_FINISH);
+    constexpr bool PROX = (MODE <= DM_LEGACY_X);
+    constexpr bool NBUF = (MODE == DM_FUSED_NBUF || MODE == DM_LEGACY_X);
+    constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z);
+    constexpr bool PART_M = (MODE == DM_LEGACY_Z);
+    constexpr bool WRITE_U = APPLY_U;
+    constexpr bool WRITE_X = PROX;
+    constexpr bool NEED_X = APPLY_U || (VAR == 2 && WRITE_X);
+    constexpr bool NEED_U = APPLY_U || (PROX && !NBUF) || (PART && !PART_M);
+    constexpr bool NEED_A = APPLY_U;
+
+    const int N = a.N;
+    const long S = a.stride;
+    const int lane = threadIdx.x & 63;
+    const int rg = lane >> 3;         // rows 4 rg .. 4 rg + 3
+    const int c0 = (lane & 7) * 4;    // columns c0 .. c0 + 3
+    const int base = t * (2 * WTT);
+
+    float zj[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int j = min(J * WT + c0 + c, N - 1);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) zj[c][k] = a.Z[3 * j + k];
+    }
+    float cacc[4][5];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) cacc[c][q] = 0.0f;
+
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+        const int il = rg * 4 + r;
+        const int i = I * WT + il;
+        const int ic = min(i, N - 1);
+        float zi[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) zi[k] = a.Z[3 * ic + k];
+        const int pa = base + il * WT + c0;
+        const int pb = pa + WTT;
+        float racc[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float prow = 0.0f;
+        bool on = true;
+        if constexpr (VAR == 1) on = J * WT + c0 < N;
+        if constexpr (VAR == 2) on = (c0 + 3 > il) && (J * WT + max(c0, il + 1) < N);
+        if (on) {
+            float4 xa[3], xb[3], ua[3], ub[3], na[3], nbv[3], ma[3], mb[3];
+            float4 ra, rb, aa, ab;
+            if constexpr (UNI) {
+                ra = rb = make_float4(a.rho_u, a.rho_u, a.rho_u, a.rho_u);
+                if constexpr (NEED_A) aa = ab = make_float4(a.alpha_u, a.alpha_u, a.alpha_u, a.alpha_u);
+            } else {
+                ra = ld4(a.rho + pa);
+                rb = ld4(a.rho + pb);
+                if constexpr (NEED_A) { aa = ld4(a.alpha + pa); ab = ld4(a.alpha + pb); }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if constexpr (NEED_X) { xa[k] = ld4((a.X + k * S) + pa); xb[k] = ld4((a.X + k * S) + pb); }
+                if constexpr (NEED_U) { ua[k] = ld4((a.U + k * S) + pa); ub[k] = ld4((a.U + k * S) + pb); }
+                if constexpr (NBUF) { na[k] = ld4((a.Nb + k * S) + pa); nbv[k] = ld4((a.Nb + k * S) + pb); }
+                if constexpr (PART_M) { ma[k] = ld4((a.Mb + k * S) + pa); mb[k] = ld4((a.Mb + k * S) + pb); }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int jl = c0 + c;
+                bool vm = true;
+                if constexpr (VAR >= 1) vm = (i < N) && (J * WT + jl < N);
+                if constexpr (VAR == 2) vm = vm && (il < jl);
+                if constexpr (QUAL) {
+                    const double dx = (double)zi[0] - (double)zj[c][0], dy = (double)zi[1] - (double)zj[c][1];
+                    const double ov = (double)zi[2] + (double)zj[c][2] - sqrt(dx * dx + dy * dy);
+                    qov = (vm && ov > qov) ? ov : qov;
+                }
+                float xA[3], xB[3], uA[3], uB[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    if constexpr (NEED_U) { uA[k] = cmp(ua[k], c); uB[k] = cmp(ub[k], c); }
+                }
+                if constexpr (APPLY_U) {
+                    const float alA = cmp(aa, c), alB = cmp(ab, c);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float dA = cmp(xa[k], c) - zi[k];
+                        const float dB = cmp(xb[k], c) - zj[c][k];
+                        const float nuA = uA[k] + alA * dA;
+                        const float nuB = uB[k] + alB * dB;
+                        uA[k] = vm ? nuA : uA[k];
+                        uB[k] = vm ? nuB : uB[k];
+                        prow += vm ? dA * dA + dB * dB : 0.0f;
+                    }
+                }
+                if constexpr (PROX) {
+                    float nA[3], nB[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        nA[k] = NBUF ? cmp(na[k], c) : zi[k] - uA[k];
+                        nB[k] = NBUF ? cmp(nbv[k], c) : zj[c][k] - uB[k];
+                        xA[k] = nA[k];
+                        xB[k] = nB[k];
+                    }
+                    const float rA = cmp(ra, c), rB = cmp(rb, c);
+                    if (vm)
+                        pair_project(nA[0], nA[1], nA[2], nB[0], nB[1], nB[2], rA, rA, rB, rB,
+                                     xA[0], xA[1], xA[2], xB[0], xB[1], xB[2]);
+                }
+                if constexpr (PART) {
+                    const float rA = cmp(ra, c), rB = cmp(rb, c);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float mA = PART_M ? cmp(ma[k], c) : xA[k] + uA[k];
+                        const float mB = PART_M ? cmp(mb[k], c) : xB[k] + uB[k];
+                        racc[k] += vm ? rA * mA : 0.0f;
+                        cacc[c][k] += vm ? rB * mB : 0.0f;
+                    }
+                    racc[3] += vm ? rA : 0.0f;
+                    racc[4] += vm ? rA * rA : 0.0f;
+                    cacc[c][3] += vm ? rB : 0.0f;
+                    cacc[c][4] += vm ? rB * rB : 0.0f;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    if constexpr (WRITE_X) {
+                        if constexpr (VAR == 2) {
+                            cmp(xa[k], c) = vm ? xA[k] : cmp(xa[k], c);
+                            cmp(xb[k], c) = vm ? xB[k] : cmp(xb[k], c);
+                        } else {
+                            cmp(xa[k], c) = xA[k];
+                            cmp(xb[k], c) = xB[k];
+                        }
+                    }
+                    if constexpr (WRITE_U) { cmp(ua[k], c) = uA[k]; cmp(ub[k], c) = uB[k]; }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if constexpr (WRITE_X) { st4((a.X + k * S) + pa, xa[k]); st4((a.X + k * S) + pb, xb[k]); }
+                if constexpr (WRITE_U) { st4((a.U + k * S) + pa, ua[k]); st4((a.U + k * S) + pb, ub[k]); }
+            }
+        }  // on
+        if constexpr (APPLY_U) pr += (double)prow;
+        if constexpr (PART) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const float v = xsum_cg<8>(racc[q]);
+                if ((lane & 7) == 0) ws.srow[q][il] = v;
+            }
+        }
+    }
+    if constexpr (PART) {
+        // column sums across the 8 row groups of the wave
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                float v = cacc[c][q];
+                v += __shfl_xor(v, 8, 64);
+                v += __shfl_xor(v, 16, 64);
+                v += __shfl_xor(v, 32, 64);
+                if (lane < 8) ws.scol[q][c0 + c] = v;
+            }
+    }
+}
+
+// One 32 x 32 tile by one wavefront: pairs, the diagonal's unary factors,
+// the block partials, the FP64 primal partial (*rp) and, with QUAL, the
+// quality partials q3 = {max pair overlap, max container violation, sum r}.
+template <int MODE, bool QUAL, bool UNI>
+__device__ __forceinline__ void wave_tile_pass(const DArgs& a, const int t, const int I, const int J, WaveShared& ws,
+                                               double* rp, double* q3)
+{
+    constexpr bool APPLY_U = (MODE == DM_FUSED || MODE == DM_FINISH);
+    constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z);
+    const int lane = threadIdx.x & 63;
+    double pr = 0.0, qov = 0.0, qbox = 0.0, qsum = 0.0;
+    if (I == J) wave_pair_tile<MODE, 2, QUAL, UNI>(a, t, I, J, ws, pr, qov);
+    else if ((J + 1) * WT > a.N) wave_pair_tile<MODE, 1, QUAL, UNI>(a, t, I, J, ws, pr, qov);
+    else wave_pair_tile<MODE, 0, QUAL, UNI>(a, t, I, J, ws, pr, qov);
+    wave_sync();
+    if (I == J && lane < WT) {
+        const UnaryAcc r = diag_unary_body<WT, MODE, QUAL, UNI>(a.X, a.U, a.rho, a.alpha, a.Nb, a.Mb, a.Z, a.aexp,
+                                                                 a.stride, a.N, a.rho_u, a.alpha_u, t, I,
+                                                                 &ws.srow[0][0], lane);
+        pr += r.pr;
+        qbox = r.qbox > qbox ? r.qbox : qbox;
+        qsum += r.qsum;
+    }
+    wave_sync();
+    if constexpr (PART) {
+        const int N = a.N, nb = a.nb;
+        if (I != J) {
+            if (lane < WT) {
+                const int i = I * WT + lane;
+                if (i < N)
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) a.P[((long)q * nb + J) * N + i] = ws.srow[q][lane];
+            } else {
+                const int jl = lane - WT, j = J * WT + jl;
+                if (j < N)
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) a.P[((long)q * nb + I) * N + j] = ws.scol[q][jl];
+            }
+        } else if (lane < WT) {
+            const int c = I * WT + lane;
+            if (c < N)
+#pragma unroll
+                for (int q = 0; q < 5; ++q) a.P[((long)q * nb + I) * N + c] = ws.srow[q][lane] + ws.scol[q][lane];
+        }
+    }
+    if constexpr (APPLY_U) {
+        pr = wsum64d(pr);
+        if (lane == 0) *rp = pr;
+    }
+    if constexpr (QUAL) {
+        qov = wmax64d(qov);
+        qbox = wmax64d(qbox);
+        qsum = wsum64d(qsum);
+        if (lane == 0) {
+            q3[0] = qov;
+            q3[1] = qbox;
+            q3[2] = qsum;
+        }
+    }
+    wave_sync();  // the wave's scratch is reused by its next tile
 }
 
 }  // namespace pdm
