@@ -2,10 +2,11 @@
 //
 // A graph whose factors are exactly: one box and one radius factor per
 // circle and one pair factor for every unordered pair of circles (the
-// unit-square packing, dim 3) is stored as tiles of DT x DT circle pairs.
-// Tile (I, J), I <= J, holds two sub-blocks of DT*DT edge slots:
-//   A[il][jl] = the edge of circle i = I*DT+il in pair (i, j),
-//   B[il][jl] = the edge of circle j = J*DT+jl in the same pair,
+// unit-square packing, dim 3) is stored as tiles of T x T circle pairs
+// (T = 32 by default, 64 on request).
+// Tile (I, J), I <= J, holds two sub-blocks of T*T edge slots:
+//   A[il][jl] = the edge of circle i = I*T+il in pair (i, j),
+//   B[il][jl] = the edge of circle j = J*T+jl in the same pair,
 // so the two edges of every pair factor sit at the same offset in A and B
 // and a thread reads both with coalesced vector loads and no transpose.
 // Diagonal tiles (I == J) use il < jl for pairs, put the box edge of circle
@@ -18,12 +19,13 @@
 //     prox, M = X + U, and per-tile partial sums of rho*M, rho, rho^2 for
 //     every circle of the tile's two blocks; 56 algorithmic bytes per edge
 //     (read X, U, rho, alpha; write X, U).
-//   k_dense_zreduce: Z_c = sum rho M / sum rho over the circle's DT-block
+//   k_dense_zreduce: Z_c = sum rho M / sum rho over the circle's T-block
 //     partials in fixed order (atomic-free, deterministic), and the FP64
 //     dual residual.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <vector>
@@ -38,8 +40,6 @@
 
 namespace pdm {
 
-constexpr int DT = 64;        // circles per tile side
-constexpr int DTT = DT * DT;  // edge slots per sub-block
 
 // minimum waves per SIMD requested for the tile kernel (register budget)
 #ifndef PDM_PAIRTILE_WAVES
@@ -48,6 +48,7 @@ constexpr int DTT = DT * DT;  // edge slots per sub-block
 
 struct DenseState {
     int N = 0, nb = 0, nt = 0;
+    int T = 64;               // circles per tile side (32 or 64)
     int2* d_tiles = nullptr;  // [nt] (I, J)
     float* d_P = nullptr;     // [5][nb][N] partial sums
     double* d_Rp = nullptr;   // [nt] primal residual partials
@@ -57,11 +58,11 @@ struct DenseState {
     bool u_pending = false;   // U still lacks the update of the last iteration
 };
 
-// one workgroup per tile; UNI: one rho and one alpha for every edge
-template <int MODE, bool UNI>
+// one workgroup per tile of side T; UNI: one rho and one alpha for every edge
+template <int T, int MODE, bool UNI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDM_PAIRTILE_WAVES))) void k_pairtile(DArgs a)
 {
-    __shared__ TileShared<DT> sh;
+    __shared__ TileShared<T> sh;
     // launch order (a.tiles) differs from layout order: t is the layout index
     const int2 ij = a.tiles[blockIdx.x];
     const int t = ij.x * a.nb - ij.x * (ij.x - 1) / 2 + (ij.y - ij.x);
@@ -71,25 +72,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDM_PAIRTIL
         DArgs b = a;
         b.rho_u = a.uni[0];
         b.alpha_u = a.uni[1];
-        tile_pass<DT, MODE, false, UNI>(b, t, ij.x, ij.y, sh, a.Rp + t, nullptr);
+        tile_pass<T, MODE, false, UNI>(b, t, ij.x, ij.y, sh, a.Rp + t, nullptr);
     } else {
-        tile_pass<DT, MODE, false, UNI>(a, t, ij.x, ij.y, sh, a.Rp + t, nullptr);
+        tile_pass<T, MODE, false, UNI>(a, t, ij.x, ij.y, sh, a.Rp + t, nullptr);
     }
 }
 
 // Z_c = sum rho M / sum rho over the circle's nb block partials.  A
-// workgroup takes 32 circles; its 8 slices of 32 threads sum the blocks
-// K = slice, slice + 8, ... (coalesced rows of P[q][K][.]), then the slices
-// are added in fixed order.
-__global__ __launch_bounds__(256) void k_dense_zreduce(int N, int nb, const float* __restrict__ P, float* __restrict__ Z,
-                                                       float* __restrict__ Zprev, double* __restrict__ Rd)
+// workgroup takes 32 circles; its 32 slices of 32 threads sum the blocks
+// K = slice, slice + 32, ... (coalesced 128-byte rows of P[q][K][.]), then
+// the slices are added in fixed order.
+constexpr int ZR_SLICES = 32;
+
+__global__ __launch_bounds__(32 * ZR_SLICES) void k_dense_zreduce(int N, int nb, const float* __restrict__ P,
+                                                                  float* __restrict__ Z, float* __restrict__ Zprev,
+                                                                  double* __restrict__ Rd)
 {
-    __shared__ float part[8][5][32];
+    __shared__ float part[ZR_SLICES][5][32];
     const int ks = threadIdx.x >> 5, cl = threadIdx.x & 31;
     const int c = blockIdx.x * 32 + cl;
     float s[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (c < N)
-        for (int K = ks; K < nb; K += 8)
+        for (int K = ks; K < nb; K += ZR_SLICES)
 #pragma unroll
             for (int q = 0; q < 5; ++q) s[q] += P[((long)q * nb + K) * N + c];
 #pragma unroll
@@ -99,7 +103,7 @@ __global__ __launch_bounds__(256) void k_dense_zreduce(int N, int nb, const floa
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
         float t = part[0][q][cl];
-        for (int k = 1; k < 8; ++k) t += part[k][q][cl];
+        for (int k = 1; k < ZR_SLICES; ++k) t += part[k][q][cl];
         s[q] = t;
     }
     double dz2 = 0.0;
@@ -209,22 +213,37 @@ int dense_structure(graph* G, const std::vector<int>& fkind, int T, std::vector<
     return 0;
 }
 
+// Tile side.  A workgroup's 256 threads each take T*T/1024 rows of a tile,
+// one after the other: side 32 has a quarter of the serial row loop of side
+// 64 and four times the workgroups, so more loads are in flight; measured on
+// MI355X it is the faster side from N = 500 to N = 8000 (and pads less).
+// PARADMM_DENSE_TILE=64 selects side 64.
+static int dense_tile_side()
+{
+    if (const char* e = getenv("PARADMM_DENSE_TILE")) {
+        const int t = atoi(e);
+        if (t == 32 || t == 64) return t;
+    }
+    return 32;
+}
+
 int dense_try_finalize(DevGraph* s, graph* G, const std::vector<int>& fkind)
 {
     std::vector<int> pos;
     std::vector<float> aexp;
     int nb = 0, nt = 0;
-    const int rc = dense_structure(G, fkind, DT, pos, aexp, nb, nt);
+    const int T = dense_tile_side();
+    const int rc = dense_structure(G, fkind, T, pos, aexp, nb, nt);
     if (rc) return rc;
     const int N = G->num_vars, E = G->num_edges;
     const int* ev = G->edgeix_to_varlistix;
-    const long stride = (long)nt * 2 * DTT;
+    const long stride = (long)nt * 2 * T * T;
     // launch order: the diagonal tiles (the longest: pairs, then the unary
     // factors) first, the tiles of the partial last block (the shortest)
     // last, so the final round of workgroups is the cheapest
     std::vector<int2> tiles;
     tiles.reserve(nt);
-    const bool ragged = N % DT != 0;
+    const bool ragged = N % T != 0;
     for (int I = 0; I < nb; ++I) tiles.push_back(make_int2(I, I));
     for (int I = 0; I < nb; ++I)
         for (int J = I + 1; J < nb; ++J)
@@ -235,6 +254,7 @@ int dense_try_finalize(DevGraph* s, graph* G, const std::vector<int>& fkind)
     DenseState* d = new DenseState();
     s->dense = d;
     d->N = N;
+    d->T = T;
     d->nb = nb;
     d->nt = nt;
     s->stride = stride;
@@ -304,6 +324,28 @@ static const char* mode_name(int mode)
     }
 }
 
+template <int T>
+static void launch_pairtile_t(hipStream_t st, const DArgs& a, unsigned nt, int mode, bool uni)
+{
+    switch (mode) {
+    case DM_FUSED:
+        if (uni) k_pairtile<T, DM_FUSED, true><<<nt, 256, 0, st>>>(a);
+        else k_pairtile<T, DM_FUSED, false><<<nt, 256, 0, st>>>(a);
+        break;
+    case DM_FUSED_CURRENT:
+        if (uni) k_pairtile<T, DM_FUSED_CURRENT, true><<<nt, 256, 0, st>>>(a);
+        else k_pairtile<T, DM_FUSED_CURRENT, false><<<nt, 256, 0, st>>>(a);
+        break;
+    case DM_FUSED_NBUF:
+        if (uni) k_pairtile<T, DM_FUSED_NBUF, true><<<nt, 256, 0, st>>>(a);
+        else k_pairtile<T, DM_FUSED_NBUF, false><<<nt, 256, 0, st>>>(a);
+        break;
+    case DM_LEGACY_X: k_pairtile<T, DM_LEGACY_X, false><<<nt, 256, 0, st>>>(a); break;
+    case DM_LEGACY_Z: k_pairtile<T, DM_LEGACY_Z, false><<<nt, 256, 0, st>>>(a); break;
+    default: k_pairtile<T, DM_FINISH, false><<<nt, 256, 0, st>>>(a); break;
+    }
+}
+
 static int launch_pairtile(DevGraph* s, int mode)
 {
     DArgs a = dargs(s);
@@ -312,23 +354,8 @@ static int launch_pairtile(DevGraph* s, int mode)
     // streams when every edge shares one rho and one alpha
     const bool uni = s->rho_uni && s->alpha_uni && mode <= DM_FUSED_NBUF;
     prof_begin(s, mode_name(mode));
-    switch (mode) {
-    case DM_FUSED:
-        if (uni) k_pairtile<DM_FUSED, true><<<nt, 256, 0, s->stream>>>(a);
-        else k_pairtile<DM_FUSED, false><<<nt, 256, 0, s->stream>>>(a);
-        break;
-    case DM_FUSED_CURRENT:
-        if (uni) k_pairtile<DM_FUSED_CURRENT, true><<<nt, 256, 0, s->stream>>>(a);
-        else k_pairtile<DM_FUSED_CURRENT, false><<<nt, 256, 0, s->stream>>>(a);
-        break;
-    case DM_FUSED_NBUF:
-        if (uni) k_pairtile<DM_FUSED_NBUF, true><<<nt, 256, 0, s->stream>>>(a);
-        else k_pairtile<DM_FUSED_NBUF, false><<<nt, 256, 0, s->stream>>>(a);
-        break;
-    case DM_LEGACY_X: k_pairtile<DM_LEGACY_X, false><<<nt, 256, 0, s->stream>>>(a); break;
-    case DM_LEGACY_Z: k_pairtile<DM_LEGACY_Z, false><<<nt, 256, 0, s->stream>>>(a); break;
-    default: k_pairtile<DM_FINISH, false><<<nt, 256, 0, s->stream>>>(a); break;
-    }
+    if (s->dense->T == 32) launch_pairtile_t<32>(s->stream, a, nt, mode, uni);
+    else launch_pairtile_t<64>(s->stream, a, nt, mode, uni);
     prof_end(s);
     CK(hipGetLastError());
     return PARADMM_OK;
@@ -338,7 +365,8 @@ static int launch_zreduce(DevGraph* s)
 {
     DenseState* d = s->dense;
     prof_begin(s, "zreduce");
-    k_dense_zreduce<<<(unsigned)((d->N + 31) / 32), 256, 0, s->stream>>>(d->N, d->nb, d->d_P, s->d_Z, s->d_Zprev, d->d_Rd);
+    k_dense_zreduce<<<(unsigned)((d->N + 31) / 32), 32 * ZR_SLICES, 0, s->stream>>>(d->N, d->nb, d->d_P, s->d_Z,
+                                                                               s->d_Zprev, d->d_Rd);
     prof_end(s);
     CK(hipGetLastError());
     return PARADMM_OK;

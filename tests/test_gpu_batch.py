@@ -89,9 +89,10 @@ def test_batch_teacher_forced_steps(tile):
     assert np.mean(errs < 1e-5) >= 0.95
 
 
-def test_batch_equals_single_instance_solver():
-    """Tile 64 runs the same tile code as the single-instance dense layout;
-    only the order of the per-circle block sums differs."""
+def test_batch_equals_single_instance_solver(monkeypatch):
+    """Tile 64 runs the same tile code as the single-instance dense layout
+    with tile side 64; only the order of the per-circle block sums differs."""
+    monkeypatch.setenv("PARADMM_DENSE_TILE", "64")
     sizes = [70, 130]
     gs = _instances(sizes, 10.0)
     b = P.Batch(gs)
@@ -217,11 +218,13 @@ def test_batch_barrier_kinds_agree():
             assert np.array_equal(a, c)
 
 
-def test_batch_rho_policy_matches_single_instance():
+def test_batch_rho_policy_matches_single_instance(monkeypatch):
     """Learned rho on the batch (one rho per instance from the network's mean
     estimate every 50 iterations) follows the single-instance solver with the
-    same policy in its per-graph mode."""
+    same policy in its per-graph mode (both on tile side 64, so the two differ
+    only in the order of the per-circle block sums)."""
     from engine_amd import rho_train as rt
+    monkeypatch.setenv("PARADMM_DENSE_TILE", "64")
     m = rt.load_policy()
     specs = [(64, 3), (120, 5)]
     gs = [P.packing_square(n, s, rho=rt.LEARNED_START_RHO) for n, s in specs]
