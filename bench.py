@@ -112,8 +112,9 @@ def batch_bench(P, count, seed, iters, cap):
     b.close()
     for g in gs:
         g.close()
-    # the same batch with the learned rho policy (from rho = 1000, one rho per
-    # instance every 50 iterations) against configured fixed rho = 1
+    # the same batch with the learned rho policy (from rho = 1000, every 50
+    # iterations: one rho per instance, the running mean of the network's
+    # estimates, plus the edges it raises) against configured fixed rho = 1
     from engine_amd import rho_train as rt
     learned = {}
     for name, rho0, pol in (("fixed_rho1", 1.0, None), ("learned", rt.LEARNED_START_RHO, rt.load_policy())):
@@ -122,6 +123,7 @@ def batch_bench(P, count, seed, iters, cap):
         b.configure(info["tile"], info["blocks_per_cu"])
         if pol is not None:
             b.set_rho_policy(pol)
+            b.set_rho_policy_mode(rt.DEFAULT_MODE, rt.DEFAULT_AVERAGE)
         r2, ms3 = b.solve(cap, tol=1e-4, tol_feas=1e-4, check_every=50, policy_every=50 if pol is not None else 0)
         c2 = [r for r in r2 if r["converged"]]
         learned[name] = {"converged": len(c2), "ms": ms3,
@@ -149,8 +151,9 @@ def learned_bench(P, cases, cap):
     """Time to tolerance (primal, dual, overlap, container <= 1e-4) on one
     GPU: plain fixed rho = 1 (the configured rho), plain fixed rho = 0.4 N
     (the best of the teacher's grid at large N), and the learned policy
-    (committed weights, one rho per graph from the mean of the network's
-    per-edge estimates, applied every 50 iterations from rho = 1000)."""
+    (committed weights applied every 50 iterations from rho = 1000: one rho
+    per graph, the running mean of the network's estimates, plus the edges
+    the network raises above it)."""
     from engine_amd import rho_train as rt
     m = rt.load_policy()
     out = []

@@ -452,7 +452,15 @@ struct BPolicyArgs {
     const int2* iranges; // [n active] {instance, first ptile index}, then the end
     int nact;
     float* scale;        // [ninst] rho_old / rho_new
+    float* yout;         // [slots] network output per slot (raises), or null
+    int* raised;         // [ninst] edges given their own rho (raises)
+    double* hist;        // [ninst][2] running {sum of means, count}, or null
 };
+
+// an edge keeps its own network rho when that exceeds its instance's rho by
+// more than this factor (log10; 10^0.5 = 3.2); the same rule as the
+// single-instance policy mode 3
+constexpr float BATCH_RAISE_LOG10 = 0.5f;
 
 // pass 1: the network's log10 rho of every edge of one active tile, summed
 template <int T>
@@ -471,15 +479,15 @@ __global__ __launch_bounds__(256) void k_batch_policy_eval(BPolicyArgs a, const 
     const int4 td = a.tiles[tg];
     const BInst in = a.inst[td.x];
     const long base = in.s0 + (long)td.w * NS;
-    const float rho = a.inst_rho[2 * td.x];
     double acc = 0.0;
     int num = 0;
     for (int q0 = wv * 64; q0 < NS; q0 += 256) {
         const int q = q0 + l;
         const PolicySlot ps = policy_slot<T>(q, td.y, td.z, in.N);
-        float r = 0.0f, sres = 0.0f, f[MLP_NIN];
+        float r = 0.0f, sres = 0.0f, rho = 1.0f, f[MLP_NIN];
         if (ps.valid) {
             const long p = base + q;
+            rho = a.rho[p];
             const int v = in.c0 + ps.c;
             float r2 = 0.0f, dz2 = 0.0f;
             for (int k = 0; k < 3; ++k) {
@@ -494,6 +502,7 @@ __global__ __launch_bounds__(256) void k_batch_policy_eval(BPolicyArgs a, const 
         }
         edge_features(r, sres, rho, ps.type, a.frac, f);
         const float y = policy_wave_eval(w, tsw[wv], yv[wv], f, l);
+        if (ps.valid && a.yout) a.yout[base + q] = y;
         acc += ps.valid ? (double)y : 0.0;
         num += ps.valid ? 1 : 0;
     }
@@ -524,7 +533,13 @@ __global__ __launch_bounds__(256) void k_batch_policy_reduce(BPolicyArgs a)
         sum += a.part[t];
         n += a.cnt[t];
     }
-    const float rn = exp10f((float)(sum / (double)(n ? n : 1)));
+    double m = sum / (double)(n ? n : 1);
+    if (a.hist) {  // the average of this and the earlier estimates
+        a.hist[2 * i] += m;
+        a.hist[2 * i + 1] += 1.0;
+        m = a.hist[2 * i] / a.hist[2 * i + 1];
+    }
+    const float rn = exp10f((float)m);
     a.scale[i] = a.inst_rho[2 * i] / rn;
     a.inst_rho[2 * i] = rn;
 }
@@ -539,11 +554,31 @@ __global__ __launch_bounds__(256) void k_batch_policy_apply(BPolicyArgs a)
     const BInst in = a.inst[td.x];
     const long base = in.s0 + (long)td.w * NS;
     const float sc = a.scale[td.x], rn = a.inst_rho[2 * td.x];
+    if (!a.yout) {
+        for (int q = threadIdx.x; q < NS; q += 256) {
+            if (!policy_slot<T>(q, td.y, td.z, in.N).valid) continue;
+            const long p = base + q;
+            for (int k = 0; k < 3; ++k) a.U[k * a.stride + p] *= sc;
+            a.rho[p] = rn;
+        }
+        return;
+    }
+    // raises: an edge the network singles out keeps its own rho
+    const float lg = log10f(rn);
     for (int q = threadIdx.x; q < NS; q += 256) {
-        if (!policy_slot<T>(q, td.y, td.z, in.N).valid) continue;
+        const bool valid = policy_slot<T>(q, td.y, td.z, in.N).valid;
         const long p = base + q;
-        for (int k = 0; k < 3; ++k) a.U[k * a.stride + p] *= sc;
-        a.rho[p] = rn;
+        bool up = false;
+        if (valid) {
+            const float y = a.yout[p];
+            up = y - lg > BATCH_RAISE_LOG10;
+            const float r = up ? exp10f(y) : rn;
+            const float f = a.rho[p] / r;
+            for (int k = 0; k < 3; ++k) a.U[k * a.stride + p] *= f;
+            a.rho[p] = r;
+        }
+        const unsigned long long bal = __ballot(up);
+        if ((threadIdx.x & 63) == 0 && bal) atomicAdd(a.raised + td.x, (int)__popcll(bal));
     }
 }
 
@@ -589,12 +624,21 @@ struct paradmm_batch {
     double* ppart = nullptr;    // [ntiles]
     int* pcnt = nullptr;        // [ntiles]
     float* pscale = nullptr;    // [ninst]
+    int policy_mode = 2;        // 2: one rho per instance; 3: plus the edges the network raises
+    int policy_average = 0;     // running mean of each instance's estimates
+    float* yout = nullptr;      // [slots] (mode 3)
+    int* raised = nullptr;      // [ninst] (mode 3)
+    double* phist = nullptr;    // [ninst][2] (average)
+    std::vector<int> uni0;      // per instance: one rho and alpha on every edge at upload
 };
 
 static void batch_free_dev(paradmm_batch* b)
 {
     for (void* p : b->allocs) (void)hipFree(p);
     b->allocs.clear();
+    b->yout = nullptr;  // allocated on first use
+    b->raised = nullptr;
+    b->phist = nullptr;
 }
 
 static int bmalloc(paradmm_batch* b, void** p, size_t bytes)
@@ -761,6 +805,8 @@ static int batch_upload(paradmm_batch* b)
         for (int e = 1; e < G->num_edges && bi.uni; ++e)
             bi.uni = (float)G->RHOS[e] == bi.rho_u && (float)G->ALPHAS[e] == bi.alpha_u;
     }
+    b->uni0.resize(b->ninst);
+    for (int i = 0; i < b->ninst; ++i) b->uni0[i] = b->inst[i].uni;
     CK(hipMemcpyAsync(b->inst_d, b->inst.data(), sizeof(BInst) * (size_t)b->ninst, hipMemcpyHostToDevice, b->stream));
     std::vector<float> ir(2 * (size_t)b->ninst);
     for (int i = 0; i < b->ninst; ++i) {
@@ -841,6 +887,10 @@ static int batch_policy_solve(paradmm_batch* b, const paradmm_solve_opts* o, std
     float ms = 0.0f;
     *total_ms = 0.0f;
     int it = 0, first = FIRST_NBUF;
+    if (b->policy_average) {
+        if (!b->phist) RC(bmalloc(b, (void**)&b->phist, sizeof(double) * 2 * (size_t)b->ninst));
+        CK(hipMemsetAsync(b->phist, 0, sizeof(double) * 2 * (size_t)b->ninst, b->stream));
+    }
     for (;;) {
         const int end = std::min(it + K, o->max_iters);
         RC(batch_launch(b, o, &ms, it, end, first));
@@ -880,6 +930,16 @@ static int batch_policy_solve(paradmm_batch* b, const paradmm_solve_opts* o, std
         pa.iranges = b->iranges;
         pa.nact = (int)ir.size() - 1;
         pa.scale = b->pscale;
+        pa.yout = nullptr;
+        pa.raised = nullptr;
+        pa.hist = b->policy_average ? b->phist : nullptr;
+        if (b->policy_mode == 3) {
+            if (!b->yout) RC(bmalloc(b, (void**)&b->yout, sizeof(float) * (size_t)b->slots));
+            if (!b->raised) RC(bmalloc(b, (void**)&b->raised, sizeof(int) * (size_t)b->ninst));
+            CK(hipMemsetAsync(b->raised, 0, sizeof(int) * (size_t)b->ninst, b->stream));
+            pa.yout = b->yout;
+            pa.raised = b->raised;
+        }
         const unsigned npt = (unsigned)pt.size();
         const float* params = mlp_device_params(b->policy);
         if (b->T == 32) k_batch_policy_eval<32><<<npt, 256, 0, b->stream>>>(pa, params);
@@ -891,6 +951,20 @@ static int batch_policy_solve(paradmm_batch* b, const paradmm_solve_opts* o, std
         else k_batch_policy_apply<64><<<npt, 256, 0, b->stream>>>(pa);
         CK(hipGetLastError());
         CK(hipStreamSynchronize(b->stream));
+        if (b->policy_mode == 3) {
+            // an instance with raised edges leaves the uniform tile variant
+            std::vector<int> nr(b->ninst);
+            CK(hipMemcpy(nr.data(), b->raised, sizeof(int) * nr.size(), hipMemcpyDeviceToHost));
+            bool changed = false;
+            for (size_t k = 0; k + 1 < ir.size(); ++k) {
+                const int i = ir[k].x;
+                const int u = b->uni0[i] && nr[i] == 0;
+                changed |= u != b->inst[i].uni;
+                b->inst[i].uni = u;
+            }
+            if (changed)
+                CK(hipMemcpy(b->inst_d, b->inst.data(), sizeof(BInst) * (size_t)b->ninst, hipMemcpyHostToDevice));
+        }
         *total_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         first = FIRST_CURRENT;
     }
@@ -1059,6 +1133,17 @@ extern "C" int paradmm_batch_set_rho_policy(paradmm_batch* b, paradmm_rho_mlp* m
         return PARADMM_ERR_ARG;
     }
     b->policy = m;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_batch_set_rho_policy_mode(paradmm_batch* b, int mode, int average)
+{
+    if (!b || (mode != 2 && mode != 3)) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_set_rho_policy_mode: mode 2 or 3");
+        return PARADMM_ERR_ARG;
+    }
+    b->policy_mode = mode;
+    b->policy_average = average != 0;
     return PARADMM_OK;
 }
 

@@ -55,6 +55,7 @@ struct DenseState {
     double* d_Rd = nullptr;   // [N] dual residual partials
     float* d_aexp = nullptr;  // [N] radius exponent per circle
     float* d_uni = nullptr;   // [2] common rho, alpha (uniform runs)
+    int* d_tile_pos = nullptr; // [nt] layout tile index -> launch position in d_tiles
     bool u_pending = false;   // U still lacks the update of the last iteration
 };
 
@@ -64,9 +65,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDM_PAIRTIL
 {
     __shared__ TileShared<T> sh;
     // launch order (a.tiles) differs from layout order: t is the layout index
-    const int2 ij = a.tiles[blockIdx.x];
+    const int2 ij0 = a.tiles[blockIdx.x];
+    const int2 ij = make_int2(ij0.x, ij0.y & ~TILE_RAISED);
     const int t = ij.x * a.nb - ij.x * (ij.x - 1) / 2 + (ij.y - ij.x);
     if constexpr (UNI) {
+        if (ij0.y & TILE_RAISED) {
+            // a tile holding edges of their own rho (a rho policy raised
+            // them) reads the per-edge rho and alpha streams
+            tile_pass<T, MODE, false, false>(a, t, ij.x, ij.y, sh, a.Rp + t, nullptr);
+            return;
+        }
         // the common values live in device memory, so captured iterations
         // follow later changes of them
         DArgs b = a;
@@ -262,10 +270,14 @@ int dense_try_finalize(DevGraph* s, graph* G, const std::vector<int>& fkind)
     RC(dmalloc(s, (void**)&s->d_pos, sizeof(int) * (size_t)E));
     RC(dmalloc(s, (void**)&s->d_edge_var_api, sizeof(int) * (size_t)E));
     RC(dmalloc(s, (void**)&d->d_tiles, sizeof(int2) * (size_t)nt));
+    RC(dmalloc(s, (void**)&d->d_tile_pos, sizeof(int) * (size_t)nt));
     RC(dmalloc(s, (void**)&d->d_aexp, sizeof(float) * (size_t)N));
     CK(hipMemcpy(s->d_pos, pos.data(), sizeof(int) * (size_t)E, hipMemcpyHostToDevice));
     CK(hipMemcpy(s->d_edge_var_api, ev, sizeof(int) * (size_t)E, hipMemcpyHostToDevice));
     CK(hipMemcpy(d->d_tiles, tiles.data(), sizeof(int2) * (size_t)nt, hipMemcpyHostToDevice));
+    std::vector<int> tpos(nt);
+    for (int k = 0; k < nt; ++k) tpos[tile_index(tiles[k].x, tiles[k].y, nb)] = k;
+    CK(hipMemcpy(d->d_tile_pos, tpos.data(), sizeof(int) * (size_t)nt, hipMemcpyHostToDevice));
     CK(hipMemcpy(d->d_aexp, aexp.data(), sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
     return 0;
 }
@@ -277,6 +289,7 @@ int dense_alloc_state(DevGraph* s)
     RC(dmalloc(s, (void**)&d->d_Rp, sizeof(double) * (size_t)d->nt));
     RC(dmalloc(s, (void**)&d->d_Rd, sizeof(double) * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_uni, sizeof(float) * 2));
+
     CK(hipMemsetAsync(d->d_Rp, 0, sizeof(double) * (size_t)d->nt, s->stream));
     CK(hipMemsetAsync(d->d_Rd, 0, sizeof(double) * (size_t)d->N, s->stream));
     return PARADMM_OK;
@@ -385,6 +398,15 @@ int dense_iteration(DevGraph* s, int from_n)
 bool dense_steady(DevGraph* s) { return s->dense && s->dense->u_pending; }
 
 float* dense_uni_ptr(DevGraph* s) { return s->dense ? s->dense->d_uni : nullptr; }
+
+int2* dense_tile_flags(DevGraph* s, const int** tile_pos, int* nt, int* tile_slots)
+{
+    if (!s->dense) return nullptr;
+    *tile_pos = s->dense->d_tile_pos;
+    *nt = s->dense->nt;
+    *tile_slots = 2 * s->dense->T * s->dense->T;
+    return s->dense->d_tiles;
+}
 
 void dense_reset(DevGraph* s)
 {

@@ -82,6 +82,7 @@ struct DevGraph {
     bool ckpt_n_from_buffer = false;
     bool ckpt_rho_uni = false;
     float ckpt_rho_u = 0.0f;
+    int ckpt_raised = 0;
 
     // coherence
     bool host_newer = true;    // host arrays hold data the device has not seen
@@ -110,7 +111,10 @@ struct DevGraph {
     // iter_frac = iterations done / max_iters.  Set by the learned-rho module.
     int (*rho_policy)(DevGraph* s, float iter_frac) = nullptr;
     void* policy_ctx = nullptr;
-    int policy_per_type = 0;           // 0 per edge, 1 per-type mean, 2 one mean over all edges
+    int policy_per_type = 0;           // 0 per edge, 1 per-type mean, 2 one mean over all edges,
+                                       // 3 that mean plus the edges the network raises
+    float* d_policy_y = nullptr;       // per-slot network output (mode 3)
+    int policy_raised = 0;             // edges raised by the last mode-3 application
     bool policy_average = false;       // average the estimates over the applications so far
     void* d_policy_part = nullptr;     // per-type partial sums of that mean
     int* d_slot_var = nullptr;         // device slot -> variable (-1: padding), policy kernels
@@ -160,6 +164,15 @@ void dense_free(DevGraph* s);
 bool dense_steady(DevGraph* s);
 // device copy {rho_u, alpha_u} read by the uniform kernel variant (dense only)
 float* dense_uni_ptr(DevGraph* s);
+// bit of a dense launch-order tile entry's J: the tile holds edges whose rho
+// differs from the common one (a rho policy raised them), so a launch of
+// the uniform-rho variant takes that tile through the per-edge streams
+constexpr int TILE_RAISED = 1 << 30;
+
+// dense layout: the launch-order tile list (whose TILE_RAISED bits flag
+// tiles with edges of their own rho), the layout -> launch position map,
+// the tile count and the slots per tile; null for other layouts
+int2* dense_tile_flags(DevGraph* s, const int** tile_pos, int* nt, int* tile_slots);
 // writes s->rho_u, s->alpha_u to that copy (stream-ordered, synchronous)
 int sync_uniform(DevGraph* s);
 // the device state was replaced from the host: U is current

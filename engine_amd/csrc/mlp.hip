@@ -525,6 +525,10 @@ extern "C" int paradmm_rho_mlp_layout_probe(float* out256)
 
 // ------------------------------------------------------------------ rho policy on a graph
 
+// mode 3: an edge keeps its own network rho when that exceeds the graph's
+// rho by more than this factor (log10; 10^0.5 = 3.2)
+#define RHO_RAISE_LOG10 0.5f
+
 namespace pdm {
 
 struct PolicyArgs {
@@ -545,6 +549,11 @@ struct PolicyArgs {
     long nslots;
     const int* slot_var;
     const unsigned char* slot_type;
+    float* yout;  // per-slot network output (mode 3), or null
+    int* nraise;  // edges given their own rho by the last application (mode 3)
+    int* tflag;          // dense layout: launch-order tile list, .y carries TILE_RAISED (mode 3), or null
+    const int* tile_pos; // layout tile -> launch position
+    int tile_slots;
 };
 
 // residuals of the edge in device slot p (coalesced in slot order)
@@ -669,6 +678,7 @@ __global__ __launch_bounds__(256) void k_rho_policy_type_sums(PolicyArgs a, cons
         }
         edge_features(r, s, rho, type, a.frac, f);
         const float y = policy_wave_eval(w, ts, yv[wv], f, l);
+        if (valid && a.yout) a.yout[p] = y;
         if (valid) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -769,6 +779,37 @@ __global__ __launch_bounds__(256) void k_rho_policy_type_apply(PolicyArgs a, con
     a.rho[p] = rn;
 }
 
+// Graph mode with raises, pass 3: rho_e <- 10^y_e where the network asks for
+// more than 10^raise_log times the graph's rho (an edge it singles out),
+// else the graph's rho; U_e <- U_e rho_old / rho_new.  Counts the raised
+// edges (one atomic per wavefront: the count does not depend on the order).
+__global__ __launch_bounds__(256) void k_rho_policy_raise_apply(PolicyArgs a, const float* __restrict__ rho_type,
+                                                                float raise_log)
+{
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = p < a.nslots && a.slot_var[p] >= 0;
+    bool raised = false;
+    if (valid) {
+        const float g = rho_type[a.slot_type[p]];
+        const float y = a.yout[p];
+        raised = y - log10f(g) > raise_log;
+        const float rn = raised ? exp10f(y) : g;
+        const float scale = a.rho[p] / rn;
+        for (int k = 0; k < a.D; ++k) a.U[k * a.stride + p] *= scale;
+        a.rho[p] = rn;
+        if (raised && a.tflag) atomicOr(a.tflag + 2L * a.tile_pos[p / a.tile_slots] + 1, TILE_RAISED);
+    }
+    const unsigned long long b = __ballot(raised);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(a.nraise, (int)__popcll(b));
+}
+
+// clears the TILE_RAISED bits of a launch-order tile list
+__global__ __launch_bounds__(256) void k_clear_tile_flags(int nt, int* __restrict__ tiles)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < nt) tiles[2 * k + 1] &= ~TILE_RAISED;
+}
+
 // uniform rho multiplier with the scaled dual rescaled: rho *= f, U /= f
 __global__ __launch_bounds__(256) void k_scale_rho(long n, int D, long stride, float f, float* __restrict__ rho,
                                                    float* __restrict__ U)
@@ -799,6 +840,11 @@ static PolicyArgs policy_args(DevGraph* s, float frac)
     a.nslots = s->stride;
     a.slot_var = s->d_slot_var;
     a.slot_type = s->d_slot_type;
+    a.yout = nullptr;
+    a.nraise = nullptr;
+    a.tflag = nullptr;
+    a.tile_pos = nullptr;
+    a.tile_slots = 1;
     return a;
 }
 
@@ -854,21 +900,50 @@ static int apply_policy(DevGraph* s, float frac)
         double* part = hist + 6;
         int* cnt = (int*)(part + 3 * 8192);
         float* rho_type = (float*)(cnt + 3 * 8192);
+        int* nraise = (int*)(rho_type + 3);
+        const bool raises = s->policy_per_type == 3;
+        if (raises) {
+            if (!s->d_policy_y) RC(dmalloc(s, (void**)&s->d_policy_y, sizeof(float) * (size_t)a.nslots));
+            a.yout = s->d_policy_y;
+            a.nraise = nraise;
+            CK(hipMemsetAsync(nraise, 0, sizeof(int), s->stream));
+            int nt = 0, ts = 1;
+            const int* tpos = nullptr;
+            if (int2* tl = dense_tile_flags(s, &tpos, &nt, &ts)) {
+                a.tflag = (int*)tl;
+                a.tile_pos = tpos;
+                a.tile_slots = ts;
+                k_clear_tile_flags<<<(unsigned)((nt + 255) / 256), 256, 0, s->stream>>>(nt, a.tflag);
+            }
+        }
         k_rho_policy_type_sums<<<grid, 256, 0, s->stream>>>(a, mlp_device_params(m), part, cnt);
-        k_rho_policy_type_final<<<1, 256, 0, s->stream>>>((int)grid, part, cnt, rho_type, s->policy_per_type == 2,
+        k_rho_policy_type_final<<<1, 256, 0, s->stream>>>((int)grid, part, cnt, rho_type, s->policy_per_type >= 2,
                                                          s->policy_average ? hist : nullptr);
-        k_rho_policy_type_apply<<<(unsigned)((a.nslots + 255) / 256), 256, 0, s->stream>>>(a, rho_type);
+        if (raises)
+            k_rho_policy_raise_apply<<<(unsigned)((a.nslots + 255) / 256), 256, 0, s->stream>>>(a, rho_type,
+                                                                                             RHO_RAISE_LOG10);
+        else
+            k_rho_policy_type_apply<<<(unsigned)((a.nslots + 255) / 256), 256, 0, s->stream>>>(a, rho_type);
     }
     prof_end(s);
     CK(hipGetLastError());
     s->rho_device_written = true;
     s->rho_uni = false;
-    if (s->policy_per_type == 2) {
-        // one rho for every edge: the steady-state kernels may use it as a scalar
+    if (s->policy_per_type >= 2) {
+        // one rho for every edge (no edge raised): the steady-state kernels
+        // may use it as a scalar
+        int nr = 0;
         CK(hipMemcpyAsync(&s->rho_u, rho_type_ptr(s), sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        if (s->policy_per_type == 3)
+            CK(hipMemcpyAsync(&nr, rho_type_ptr(s) + 3, sizeof(int), hipMemcpyDeviceToHost, s->stream));
         CK(hipStreamSynchronize(s->stream));
-        s->rho_uni = true;
-        RC(sync_uniform(s));
+        s->policy_raised = nr;
+        // the dense tile kernel takes the raised edges' tiles through the
+        // per-edge streams (tile flags) and keeps the scalar rho elsewhere
+        if (nr == 0 || a.tflag) {
+            s->rho_uni = true;
+            RC(sync_uniform(s));
+        }
     }
     return PARADMM_OK;
 }
@@ -886,8 +961,8 @@ extern "C" int paradmm_gpu_set_rho_policy(graph* G, paradmm_rho_mlp* m)
 
 extern "C" int paradmm_gpu_set_rho_policy_mode(graph* G, int mode, int average)
 {
-    if (mode < 0 || mode > 2) {
-        pdm_set_error(PARADMM_ERR_ARG, "paradmm_gpu_set_rho_policy_mode: mode 0, 1 or 2");
+    if (mode < 0 || mode > 3) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_gpu_set_rho_policy_mode: mode 0, 1, 2 or 3");
         return PARADMM_ERR_ARG;
     }
     DevGraph* s;
@@ -978,6 +1053,7 @@ extern "C" int paradmm_gpu_state_save(graph* G)
     s->ckpt_n_from_buffer = s->n_from_buffer;
     s->ckpt_rho_uni = s->rho_uni;
     s->ckpt_rho_u = s->rho_u;
+    s->ckpt_raised = s->policy_raised;
     return PARADMM_OK;
 }
 
@@ -999,8 +1075,11 @@ extern "C" int paradmm_gpu_state_restore(graph* G)
     CK(hipStreamSynchronize(s->stream));
     if (s->layout == 2) dense_reset(s);  // the saved U was current
     s->n_from_buffer = s->ckpt_n_from_buffer;
-    s->rho_uni = s->ckpt_rho_uni;
+    // raised edges of the saved state may sit in other tiles than the
+    // current flags say: the per-edge streams until the next application
+    s->rho_uni = s->ckpt_rho_uni && s->ckpt_raised == 0;
     s->rho_u = s->ckpt_rho_u;
+    s->policy_raised = s->ckpt_raised;
     RC(sync_uniform(s));
     s->host_newer = false;
     s->dev_newer = true;

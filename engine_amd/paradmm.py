@@ -136,6 +136,7 @@ def load():
         "paradmm_batch_solve": (C.c_int, [V, C.POINTER(SolveOpts), C.POINTER(SolveResult), _DP]),
         "paradmm_batch_sync_to_host": (C.c_int, [V]),
         "paradmm_batch_set_rho_policy": (C.c_int, [V, V]),
+        "paradmm_batch_set_rho_policy_mode": (C.c_int, [V, C.c_int, C.c_int]),
         "paradmm_batch_info": (C.c_int, [V, _IP, _IP, _IP, C.POINTER(C.c_long), _IP]),
         "paradmm_rho_mlp_get_params": (C.c_int, [V, _FP, C.c_int]),
         "paradmm_rho_mlp_forward": (C.c_int, [V, _FP, C.c_int, _FP]),
@@ -382,8 +383,9 @@ class Graph:
         check(self.lib.paradmm_gpu_set_rho_policy(self.ptr, mlp.h if mlp is not None else None), "set_rho_policy")
 
     def set_rho_policy_mode(self, mode, average=False):
-        """0 per edge, 1 per-type mean, 2 one mean over all edges (True/False
-        map to 1/0); average: mean over the applications so far."""
+        """0 per edge, 1 per-type mean, 2 one mean over all edges, 3 that
+        mean plus the edges the network raises above it (True/False map to
+        1/0); average: mean over the applications so far."""
         check(self.lib.paradmm_gpu_set_rho_policy_mode(self.ptr, int(mode), 1 if average else 0),
               "set_rho_policy_mode")
 
@@ -468,6 +470,12 @@ class Batch:
         check(self.lib.paradmm_batch_set_rho_policy(self.h, mlp.h if mlp is not None else None),
               "paradmm_batch_set_rho_policy")
         self._policy = mlp
+
+    def set_rho_policy_mode(self, mode, average=False):
+        """2: one rho per instance; 3: plus the edges the network raises;
+        average: running mean of each instance's estimates."""
+        check(self.lib.paradmm_batch_set_rho_policy_mode(self.h, int(mode), 1 if average else 0),
+              "paradmm_batch_set_rho_policy_mode")
 
     def solve(self, max_iters, tol=1e-4, tol_feas=1e-4, check_every=50, policy_every=0):
         o = SolveOpts(max_iters, check_every, tol, tol, tol_feas, policy_every)

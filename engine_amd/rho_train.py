@@ -31,8 +31,9 @@ from . import paradmm as P
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(HERE, "models", "rho_policy.f32")
 # how the committed policy is applied (paradmm_gpu_set_rho_policy_mode):
-# one rho per graph, the mean of the network's estimates over all edges
-DEFAULT_MODE, DEFAULT_AVERAGE = 2, False
+# one rho per graph, the running mean of the network's estimates over the
+# applications so far, plus the edges the network raises above it
+DEFAULT_MODE, DEFAULT_AVERAGE = 3, True
 DEFAULT_MAX_ITERS = 30000
 RHO_GRID = (20.0, 35.0, 50.0, 70.0, 100.0, 150.0, 200.0, 300.0, 500.0, 800.0, 1200.0)
 # snapshot iterations of the training trajectories: dense early, then spread
@@ -78,6 +79,33 @@ def best_fixed_rho(n, seed, grid=RHO_GRID, max_iters=20000, tol=1e-4):
 
 STARTS = (("abs", LEARNED_START_RHO), ("rel", 1.0), ("rel", 0.7), ("rel", 4.0))
 
+# Raised edges.  Late in a solve (after RAISE_FROM of the iteration budget) a
+# pair edge whose primal residual |X_e - Z_v| is still above
+# RAISE_RESIDUAL is labelled RAISE_FACTOR x rho*: such edges are pairs of
+# small circles caught in a two-cycle of the nonconvex pair projection (the
+# separation direction flips every iteration, the dual never accumulates and
+# Z does not move); a higher rho on those edges alone pulls Z their way and
+# ends the cycle.  Every such row of a snapshot is kept (up to RAISE_ROWS),
+# RAISE_WEIGHT times.
+RAISE_FROM = 5000 / 30000
+RAISE_RESIDUAL = 1e-4
+RAISE_FACTOR = 10.0
+RAISE_ROWS, RAISE_WEIGHT = 2000, 3
+
+
+def _label(f, rho_star, rng, rows):
+    """Rows sampled from one snapshot's features f and their targets:
+    log10 rho*, plus log10 RAISE_FACTOR on the raised edges."""
+    take = rng.choice(f.shape[0], size=min(rows, f.shape[0]), replace=False)
+    if f.shape[0] and f[0, 5] >= RAISE_FROM:
+        hot = np.flatnonzero((f[:, 0] > np.log10(RAISE_RESIDUAL) / 5) & (f[:, 3] == 1.0))[:RAISE_ROWS]
+        if hot.size:
+            take = np.concatenate([take] + [hot] * RAISE_WEIGHT)
+    t = np.full(take.size, np.log10(rho_star), dtype=np.float32)
+    fr = f[take]
+    t[(fr[:, 0] > np.log10(RAISE_RESIDUAL) / 5) & (fr[:, 3] == 1.0) & (fr[:, 5] >= RAISE_FROM)] += np.log10(RAISE_FACTOR)
+    return fr, t
+
 
 def generate(instances, starts=STARTS, snapshots=SNAPSHOTS, max_iters=20000,
              rows_per_snapshot=1000, rng_seed=0, log=print, grid=RHO_GRID, first_weight=1, known=None):
@@ -110,9 +138,9 @@ def generate(instances, starts=STARTS, snapshots=SNAPSHOTS, max_iters=20000,
                 done = it
                 f = g.rho_features(it / max_iters)
                 rows = rows_per_snapshot * (first_weight if si == 0 and k < 3 else 1)
-                take = rng.choice(f.shape[0], size=min(rows, f.shape[0]), replace=False)
-                feats.append(f[take])
-                targets.append(np.full(take.size, np.log10(rho_star), dtype=np.float32))
+                fr, t = _label(f, rho_star, rng, rows)
+                feats.append(fr)
+                targets.append(t)
             g.close()
         log(f"# teacher N={n} seed={seed}: rho*={rho_star:g} "
             f"({teacher[f'{n}/{seed}']['iterations']}), {time.time() - t0:.1f} s")
@@ -137,9 +165,9 @@ def generate_on_policy(instances, policy, teacher, max_iters=20000, every=50, sa
             g.iterate(every)
             if it % sample_every == 0 or it <= 4 * every:
                 f = g.rho_features(it / max_iters)
-                take = rng.choice(f.shape[0], size=min(rows_per_snapshot, f.shape[0]), replace=False)
-                feats.append(f[take])
-                targets.append(np.full(take.size, np.log10(rho_star), dtype=np.float32))
+                fr, t = _label(f, rho_star, rng, rows_per_snapshot)
+                feats.append(fr)
+                targets.append(t)
             g.apply_rho_policy(policy, it / max_iters)
         g.sync_to_host()
         log(f"# on-policy N={n} seed={seed}: rho*={rho_star:g}, final rho {g.array('RHOS')[0]:.1f}")
@@ -169,11 +197,11 @@ def load_policy(path=DEFAULT_WEIGHTS):
 
 
 LEARNED_VARIANTS = {"learned": (0, False), "learned_type": (1, False), "learned_global": (2, False),
-                    "learned_global_avg": (2, True)}
+                    "learned_global_avg": (2, True), "learned_raise": (3, False), "learned_raise_avg": (3, True)}
 
 
 def compare(n, seeds, policy, fixed=(1.0, 50.0, 100.0), max_iters=20000, log=print,
-            variants=("learned_global", "learned_global_avg")):
+            variants=("learned_raise_avg",)):
     """Time to tolerance of fixed rho values and of the learned policy (the
     given application variants, from LEARNED_START_RHO)."""
     out = []

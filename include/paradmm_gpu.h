@@ -194,10 +194,13 @@ int paradmm_gpu_apply_rho_policy(graph *G, paradmm_rho_mlp *m, float iter_frac);
  *   mode 0 (default): each edge takes its own output;
  *   mode 1: each edge takes 10^(mean of the outputs over the edges of its
  *           factor type: pair, container, radius);
- *   mode 2: every edge takes 10^(mean over all edges) -- one rho per graph.
- * average = 1 (modes 1, 2): the applied mean is the average of this and
- * every earlier application's estimate since the mode was set.  Modes 1
- * and 2 reduce in a fixed order (deterministic). */
+ *   mode 2: every edge takes 10^(mean over all edges) -- one rho per graph;
+ *   mode 3: as mode 2, except that an edge whose own output exceeds that
+ *           mean by more than log10(3.2) keeps its own output (the edges
+ *           the network singles out for a higher rho).
+ * average = 1 (modes 1-3): the applied mean is the average of this and
+ * every earlier application's estimate since the mode was set.  Modes 1-3
+ * reduce in a fixed order (deterministic). */
 int paradmm_gpu_set_rho_policy_mode(graph *G, int mode, int average);
 /* rho_e *= factor on every edge, U_e /= factor (uniform change of rho) */
 int paradmm_gpu_scale_rho(graph *G, double factor);
@@ -253,6 +256,12 @@ int paradmm_batch_solve(paradmm_batch *b, const paradmm_solve_opts *opts, paradm
  * with U rescaled by rho_old / rho_new -- the per-graph mode of
  * paradmm_gpu_set_rho_policy_mode applied per instance. */
 int paradmm_batch_set_rho_policy(paradmm_batch *b, paradmm_rho_mlp *m);
+/* 2 (default): one rho per instance, the mean of the network's estimates;
+ * 3: that rho plus, per edge, the network's own rho where it exceeds the
+ * instance's by more than 10^0.5; average = 1: each instance's rho is the
+ * running mean of its estimates (paradmm_gpu_set_rho_policy_mode, per
+ * instance) */
+int paradmm_batch_set_rho_policy_mode(paradmm_batch *b, int mode, int average);
 /* Writes Z, X, U, RHOS (and M = X + U, N = Z - U) back into every graph. */
 int paradmm_batch_sync_to_host(paradmm_batch *b);
 int paradmm_batch_info(paradmm_batch *b, int *tile, int *blocks_per_cu, int *grid, long *slots, int *tiles);

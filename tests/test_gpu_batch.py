@@ -218,11 +218,13 @@ def test_batch_barrier_kinds_agree():
             assert np.array_equal(a, c)
 
 
-def test_batch_rho_policy_matches_single_instance(monkeypatch):
-    """Learned rho on the batch (one rho per instance from the network's mean
-    estimate every 50 iterations) follows the single-instance solver with the
-    same policy in its per-graph mode (both on tile side 64, so the two differ
-    only in the order of the per-circle block sums)."""
+@pytest.mark.parametrize("mode,average", [(2, False), (3, True)])
+def test_batch_rho_policy_matches_single_instance(monkeypatch, mode, average):
+    """Learned rho on the batch (every 50 iterations: one rho per instance
+    from the network's mean estimate, in mode 3 plus the edges the network
+    raises) follows the single-instance solver with the same policy and mode
+    (both on tile side 64, so the two differ only in the order of the
+    per-circle block sums)."""
     from engine_amd import rho_train as rt
     monkeypatch.setenv("PARADMM_DENSE_TILE", "64")
     m = rt.load_policy()
@@ -231,6 +233,7 @@ def test_batch_rho_policy_matches_single_instance(monkeypatch):
     b = P.Batch(gs)
     b.configure(64, 1, 1)
     b.set_rho_policy(m)
+    b.set_rho_policy_mode(mode, average)
     res, ms = b.solve(150, tol=-1.0, tol_feas=-1.0, check_every=50, policy_every=50)
     b.sync_to_host()
     for (n, s), g, r in zip(specs, gs, res):
@@ -238,13 +241,15 @@ def test_batch_rho_policy_matches_single_instance(monkeypatch):
         one = P.packing_square(n, s, rho=rt.LEARNED_START_RHO)
         one.prepare()
         one.set_rho_policy(m)
-        one.set_rho_policy_mode(2, False)
+        one.set_rho_policy_mode(mode, average)
         one.solve(150, tol=-1.0, tol_feas=-1.0, check_every=50, policy_every=50)
         one.sync_to_host()
         err = max_rel(g.array("Z"), one.array("Z"))
         rb = g.array("RHOS")
-        print(f"N={n}: batch rho {rb[0]:.2f} (one value: {np.all(rb == rb[0])}), Z vs single {err:.2e}")
-        assert np.all(rb == rb[0]) and rb[0] != rt.LEARNED_START_RHO
+        print(f"N={n} mode {mode}: batch rho {rb[0]:.2f} (one value: {np.all(rb == rb[0])}), Z vs single {err:.2e}")
+        assert rb[0] != rt.LEARNED_START_RHO
+        if mode == 2:
+            assert np.all(rb == rb[0])
         assert err < 1e-4
 
 
@@ -255,6 +260,7 @@ def test_batch_rho_policy_retires_converged_instances():
     gs = [P.packing_square(n, s, rho=rt.LEARNED_START_RHO) for n, s in specs]
     b = P.Batch(gs)
     b.set_rho_policy(m)
+    b.set_rho_policy_mode(rt.DEFAULT_MODE, rt.DEFAULT_AVERAGE)
     res, ms = b.solve(30000, tol=1e-4, tol_feas=1e-4, check_every=50, policy_every=50)
     b.sync_to_host()
     print("batch with learned rho:", [(r["iterations"], r["converged"]) for r in res], f"{ms:.0f} ms")

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from engine_amd import paradmm as P
+from gpu_helpers import max_rel
 from oracle import mlp_oracle as mo
 
 pytestmark = pytest.mark.gpu
@@ -152,6 +153,58 @@ def test_policy_global_mode_with_average():
         rho = g.array("RHOS")
         assert np.allclose(rho, 10.0 ** np.mean(est), rtol=1e-5)
         g.iterate(10)
+
+
+def test_policy_raise_mode_keeps_edges_the_network_raises():
+    """mode 3: every edge takes 10^(mean of all outputs), except an edge whose
+    own output exceeds that mean by more than 0.5 (log10), which keeps
+    10^(its output); U is rescaled by rho_old / rho_new."""
+    n = 40
+    g = P.packing_square(n, 2, rho=30.0, alpha=1.0)
+    g.iterate(25)
+    m = P.RhoMLP(params=(mo.init_params(7) * 1.5).astype(np.float32))
+    feat = g.rho_features(0.3)
+    g.sync_to_host()
+    rho0 = g.array("RHOS").copy()
+    U0 = g.array("U").reshape(-1, 3).copy()
+    y = np.clip(m.forward(feat).astype(np.float64), -1.0, 4.0)
+    mean = y.mean()
+    g.set_rho_policy_mode(3)
+    g.apply_rho_policy(m, 0.3)
+    g.sync_to_host()
+    rho = g.array("RHOS")
+    up = y - mean > 0.5 + 1e-4
+    down = y - mean < 0.5 - 1e-4
+    print(f"raised {up.sum()} of {y.size}, mean log10 rho {mean:.3f}")
+    assert up.sum() > 0 and down.sum() > 0
+    assert np.allclose(rho[up], 10.0 ** y[up], rtol=1e-5)
+    assert np.allclose(rho[down], 10.0 ** mean, rtol=1e-5)
+    U1 = g.array("U").reshape(-1, 3)
+    assert np.allclose(U1, U0 * (rho0 / rho)[:, None], rtol=1e-5, atol=1e-7)
+
+
+def test_raised_tiles_match_the_per_edge_variant():
+    """After a mode-3 application with raised edges the dense layout keeps
+    its uniform-rho kernel and sends only the tiles holding raised edges
+    through the per-edge streams; iterations match the per-edge kernel run
+    on the same state (re-uploaded from the host, which selects it)."""
+    n = 70
+    m = P.RhoMLP(params=(mo.init_params(7) * 1.5).astype(np.float32))
+    gs = []
+    for k in range(2):
+        g = P.packing_square(n, 5, rho=30.0, alpha=1.0)
+        g.iterate(25)
+        g.set_rho_policy_mode(3)
+        g.apply_rho_policy(m, 0.3)
+        g.sync_to_host()
+        gs.append(g)
+    r = gs[0].array("RHOS")
+    assert np.unique(r).size > 1  # some edges raised
+    gs[1].sync_from_host()
+    for g in gs:
+        g.iterate(5)
+        g.sync_to_host()
+    assert max_rel(gs[0].array("Z"), gs[1].array("Z")) < 1e-6
 
 
 def test_state_save_restore_and_scale():
