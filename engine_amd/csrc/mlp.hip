@@ -553,7 +553,7 @@ struct PolicyArgs {
     int* nraise;  // edges given their own rho by the last application (mode 3)
     int* tflag;          // dense layout: launch-order tile list, .y carries TILE_RAISED (mode 3), or null
     const int* tile_pos; // layout tile -> launch position
-    int tile_slots;
+    int tile_slots, ntiles;
 };
 
 // residuals of the edge in device slot p (coalesced in slot order)
@@ -659,6 +659,11 @@ __global__ __launch_bounds__(256) void k_rho_policy_type_sums(PolicyArgs a, cons
     __shared__ float yv[4][64];
     __shared__ double ssum[4][3];
     __shared__ int scnt[4][3];
+    // mode 3: the raised-tile flags of the last application are cleared here
+    // (the apply pass, launched after this one, sets the new ones)
+    if (a.tflag)
+        for (int k = blockIdx.x * 256 + threadIdx.x; k < a.ntiles; k += gridDim.x * 256)
+            a.tflag[2 * k + 1] &= ~TILE_RAISED;
     stage_weights(params, w);
     __syncthreads();
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -803,13 +808,6 @@ __global__ __launch_bounds__(256) void k_rho_policy_raise_apply(PolicyArgs a, co
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(a.nraise, (int)__popcll(b));
 }
 
-// clears the TILE_RAISED bits of a launch-order tile list
-__global__ __launch_bounds__(256) void k_clear_tile_flags(int nt, int* __restrict__ tiles)
-{
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k < nt) tiles[2 * k + 1] &= ~TILE_RAISED;
-}
-
 // uniform rho multiplier with the scaled dual rescaled: rho *= f, U /= f
 __global__ __launch_bounds__(256) void k_scale_rho(long n, int D, long stride, float f, float* __restrict__ rho,
                                                    float* __restrict__ U)
@@ -845,6 +843,7 @@ static PolicyArgs policy_args(DevGraph* s, float frac)
     a.tflag = nullptr;
     a.tile_pos = nullptr;
     a.tile_slots = 1;
+    a.ntiles = 0;
     return a;
 }
 
@@ -885,7 +884,9 @@ static int apply_policy(DevGraph* s, float frac)
     RC(ensure_slot_maps(s));
     PolicyArgs a = policy_args(s, frac);
     const long nchunks = (a.nslots + 63) / 64;
-    const unsigned grid = (unsigned)std::min<long>((nchunks + 3) / 4, 8192);
+    // every workgroup stages the network's weights once, then loops over
+    // chunks of 64 slots: about 4 workgroups per CU
+    const unsigned grid = (unsigned)std::min<long>((nchunks + 3) / 4, 1024);
     prof_begin(s, "rho_policy");
     if (!s->policy_per_type) {
         k_rho_policy<<<grid, 256, 0, s->stream>>>(a, mlp_device_params(m));
@@ -913,7 +914,7 @@ static int apply_policy(DevGraph* s, float frac)
                 a.tflag = (int*)tl;
                 a.tile_pos = tpos;
                 a.tile_slots = ts;
-                k_clear_tile_flags<<<(unsigned)((nt + 255) / 256), 256, 0, s->stream>>>(nt, a.tflag);
+                a.ntiles = nt;
             }
         }
         k_rho_policy_type_sums<<<grid, 256, 0, s->stream>>>(a, mlp_device_params(m), part, cnt);
