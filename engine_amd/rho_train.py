@@ -175,6 +175,20 @@ def generate_on_policy(instances, policy, teacher, max_iters=20000, every=50, sa
     return np.concatenate(feats).astype(np.float32), np.concatenate(targets).astype(np.float32)
 
 
+def save_rows(path, feat, target, teacher=None):
+    """Training rows (features [n, 6] float32, targets [n] float32) and the
+    teacher record in one .npz (plain arrays; the record as JSON text), so a
+    dataset is generated once and reused."""
+    np.savez(path, feat=np.asarray(feat, dtype=np.float32), target=np.asarray(target, dtype=np.float32),
+             teacher=np.array(json.dumps(teacher or {})))
+
+
+def load_rows(path):
+    """(feat, target, teacher) written by save_rows; nothing is unpickled."""
+    with np.load(path, allow_pickle=False) as z:
+        return z["feat"], z["target"], json.loads(str(z["teacher"]))
+
+
 def train(feat, target, epochs=30, batch=4096, lr=2e-3, seed=0):
     m = P.RhoMLP(seed=seed)
     loss = m.train(feat, target, epochs=epochs, batch=batch, lr=lr, seed=seed)
