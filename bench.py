@@ -287,7 +287,12 @@ def main():
             "bytes_per_launch": dom_bytes, "bytes_per_edge_update": EDGE_UPDATE_BYTES,
             "kernel_min_bytes_per_launch": kernel_bytes,
             "kernel_min_gbs": kernel_bytes / (dom_ms * 1e-3) / 1e9,
+            # frac above uses the 72 B/edge-update unit (an X pass and a Z
+            # pass); the fused kernel needs only kernel_min bytes per edge,
+            # so these are the fractions that bound it
+            "kernel_min_frac": kernel_bytes / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "traffic_gbs": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None,
+            "traffic_frac_of_copy": (traffic / (dom_ms * 1e-3) / 1e9 / copy_gbs) if traffic else None,
             "mean_launch_ms": dom_ms, "copy_kernel_gbs": copy_gbs,
         },
         "kernels_ms": {k: v[0] for k, v in prof.items()},
