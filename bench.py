@@ -303,7 +303,8 @@ def main():
     if args.batch > 0:
         line["batch"] = batch_bench(P, args.batch, args.batch_seed, args.batch_iters, args.batch_cap)
     if not args.no_learned:
-        line["time_to_tol_learned_vs_fixed"] = learned_bench(P, ((2000, 3), (2000, 4), (1000, 3), (1000, 4)), 30000)
+        line["time_to_tol_learned_vs_fixed"] = learned_bench(P, ((5000, 3), (2000, 3), (2000, 4), (1000, 3), (1000, 4)),
+                                                             30000)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_iters, threads)
