@@ -155,10 +155,13 @@ def test_policy_global_mode_with_average():
         g.iterate(10)
 
 
-def test_policy_raise_mode_keeps_edges_the_network_raises():
+@pytest.mark.parametrize("layout", ["dense", "generic"])
+def test_policy_raise_mode_keeps_edges_the_network_raises(monkeypatch, layout):
     """mode 3: every edge takes 10^(mean of all outputs), except an edge whose
     own output exceeds that mean by more than 0.5 (log10), which keeps
-    10^(its output); U is rescaled by rho_old / rho_new."""
+    10^(its output); U is rescaled by rho_old / rho_new.  Both layouts."""
+    if layout == "generic":
+        monkeypatch.setenv("PARADMM_LAYOUT", "generic")
     n = 40
     g = P.packing_square(n, 2, rho=30.0, alpha=1.0)
     g.iterate(25)
@@ -181,6 +184,11 @@ def test_policy_raise_mode_keeps_edges_the_network_raises():
     assert np.allclose(rho[down], 10.0 ** mean, rtol=1e-5)
     U1 = g.array("U").reshape(-1, 3)
     assert np.allclose(U1, U0 * (rho0 / rho)[:, None], rtol=1e-5, atol=1e-7)
+    assert g.layout() == (2 if layout == "dense" else 1)
+    # the raised rho is used by the following iterations
+    g.iterate(5)
+    g.sync_to_host()
+    assert np.all(np.isfinite(g.array("Z")))
 
 
 def test_raised_tiles_match_the_per_edge_variant():
