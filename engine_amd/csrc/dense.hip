@@ -88,8 +88,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PDM_PAIRTIL
 
 // Z_c = sum rho M / sum rho over the circle's nb block partials.  A
 // workgroup takes 32 circles; its 32 slices of 32 threads sum the blocks
-// K = slice, slice + 32, ... (coalesced 128-byte rows of P[q][K][.]), then
-// the slices are added in fixed order.
+// K = slice, slice + 32, ... (coalesced 128-byte rows of P[q][K][.], four
+// blocks in flight per thread), then the slices are added by a fixed-shape
+// tree in LDS.
 constexpr int ZR_SLICES = 32;
 
 __global__ __launch_bounds__(32 * ZR_SLICES) void k_dense_zreduce(int N, int nb, const float* __restrict__ P,
@@ -101,19 +102,23 @@ __global__ __launch_bounds__(32 * ZR_SLICES) void k_dense_zreduce(int N, int nb,
     const int c = blockIdx.x * 32 + cl;
     float s[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (c < N)
+#pragma unroll 4
         for (int K = ks; K < nb; K += ZR_SLICES)
 #pragma unroll
             for (int q = 0; q < 5; ++q) s[q] += P[((long)q * nb + K) * N + c];
 #pragma unroll
     for (int q = 0; q < 5; ++q) part[ks][q][cl] = s[q];
     __syncthreads();
+    // fixed-shape tree over the slices
+    for (int h = ZR_SLICES / 2; h > 0; h >>= 1) {
+        if (ks < h)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) part[ks][q][cl] += part[ks + h][q][cl];
+        __syncthreads();
+    }
     if (threadIdx.x >= 32 || c >= N) return;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-        float t = part[0][q][cl];
-        for (int k = 1; k < ZR_SLICES; ++k) t += part[k][q][cl];
-        s[q] = t;
-    }
+    for (int q = 0; q < 5; ++q) s[q] = part[0][q][cl];
     double dz2 = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
