@@ -1289,6 +1289,34 @@ extern "C" int paradmm_gpu_sync_from_host(graph* G)
     return get_state(G, &s, true);
 }
 
+extern "C" int paradmm_gpu_read(graph* G, int which, double* out)
+{
+    if (!out || which < PARADMM_READ_Z || which > PARADMM_READ_RHO) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_gpu_read: which is PARADMM_READ_Z, _X, _U or _RHO and out not NULL");
+        return PARADMM_ERR_ARG;
+    }
+    DevGraph* s;
+    RC(get_state(G, &s, true));
+    RC(ensure_aux(s));
+    if (which == PARADMM_READ_Z) {
+        const long n = (long)s->V * s->D;
+        k_flat_out<<<nblk(n), 256, 0, s->stream>>>(n, s->d_Z, s->d_stage);
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(out, s->d_stage, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+        CK(hipStreamSynchronize(s->stream));
+        return PARADMM_OK;
+    }
+    // the dense layout holds U one update behind between iterations
+    if (which == PARADMM_READ_U && s->layout == 2) RC(dense_finish(s));
+    const int d = which == PARADMM_READ_RHO ? 1 : s->D;
+    const float* src = which == PARADMM_READ_X ? s->d_X : which == PARADMM_READ_U ? s->d_U : s->d_rho;
+    k_edges_out<<<nblk(s->E), 256, 0, s->stream>>>(s->E, d, src, s->stride, s->d_stage, s->d_pos);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(out, s->d_stage, sizeof(double) * (size_t)d * s->E, hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    return PARADMM_OK;
+}
+
 extern "C" int paradmm_gpu_release(graph* G)
 {
     pdm_release_graph(G);

@@ -114,6 +114,7 @@ def load():
         "paradmm_gpu_iterate": (C.c_int, [G, C.c_int]),
         "paradmm_gpu_sync_to_host": (C.c_int, [G]),
         "paradmm_gpu_sync_from_host": (C.c_int, [G]),
+        "paradmm_gpu_read": (C.c_int, [G, C.c_int, _DP]),
         "paradmm_gpu_release": (C.c_int, [G]),
         "paradmm_gpu_layout": (C.c_int, [G]),
         "paradmm_gpu_residuals": (C.c_int, [G, _DP, _DP]),
@@ -353,6 +354,17 @@ class Graph:
 
     def sync_from_host(self):
         check(self.lib.paradmm_gpu_sync_from_host(self.ptr), "paradmm_gpu_sync_from_host")
+
+    _READ = {"Z": 0, "X": 1, "U": 2, "RHO": 3}
+
+    def read(self, name):
+        """One device array as float64 (paradmm_gpu_read): Z (V, D); X, U
+        (E, D) in edge order; RHO (E,).  The host arrays are not touched."""
+        w = self._READ[name]
+        shape = (self.nvar, self.ndims) if name == "Z" else (self.nedge,) if name == "RHO" else (self.nedge, self.ndims)
+        out = np.empty(shape, dtype=np.float64)
+        check(self.lib.paradmm_gpu_read(self.ptr, w, out.ctypes.data_as(_DP)), "paradmm_gpu_read")
+        return out
 
     def layout(self):
         return self.lib.paradmm_gpu_layout(self.ptr)

@@ -110,6 +110,13 @@ int paradmm_gpu_iterate(graph *G, int iters);
 int paradmm_gpu_sync_to_host(graph *G);     /* download X, M, N, U, Z, RHOS */
 int paradmm_gpu_sync_from_host(graph *G);   /* upload X, N, U, Z, RHOS, ALPHAS */
 int paradmm_gpu_release(graph *G);          /* free device state (closeG does this too) */
+/* One device array into out (double), leaving the graph's host arrays as
+ * they are: PARADMM_READ_Z num_vars x num_dims; PARADMM_READ_X, _U
+ * num_edges x num_dims in edge order; PARADMM_READ_RHO num_edges.  A read
+ * of Z after a solve moves num_vars x num_dims values instead of every edge
+ * array (paradmm_gpu_sync_to_host). */
+enum { PARADMM_READ_Z = 0, PARADMM_READ_X = 1, PARADMM_READ_U = 2, PARADMM_READ_RHO = 3 };
+int paradmm_gpu_read(graph *G, int which, double *out);
 /* 1 = generic CSR layout, 2 = dense pair-tile layout; 0 before prepare */
 int paradmm_gpu_layout(graph *G);
 /* NEW: FP64 residuals of the last device iteration (the reference has none):

@@ -300,3 +300,23 @@ def test_uniform_rho_change_seen_by_captured_iterations():
         zs.append(g.array("Z").copy())
     assert np.array_equal(zs[0], zs[1])
     assert not np.array_equal(zs[0], zs[2])
+
+
+@pytest.mark.parametrize("layout", ["dense", "generic"])
+def test_read_single_arrays_matches_full_sync(monkeypatch, layout):
+    """paradmm_gpu_read returns one device array (Z, X, U, rho) equal to what
+    a full sync to the host writes, without touching the host arrays."""
+    from engine_amd import paradmm as P
+    if layout == "generic":
+        monkeypatch.setenv("PARADMM_LAYOUT", "generic")
+    g = P.packing_square(60, 3, rho=20.0)
+    g.iterate(17)
+    z0 = g.array("Z").copy()
+    reads = {k: g.read(k) for k in ("Z", "X", "U", "RHO")}
+    assert np.array_equal(g.array("Z"), z0)  # host untouched
+    g.sync_to_host()
+    assert np.array_equal(reads["Z"].reshape(-1), g.array("Z"))
+    assert np.array_equal(reads["X"].reshape(-1), g.array("X"))
+    assert np.array_equal(reads["U"].reshape(-1), g.array("U"))
+    assert np.array_equal(reads["RHO"], g.array("RHOS"))
+    g.close()
