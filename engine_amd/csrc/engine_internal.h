@@ -6,6 +6,8 @@
 #ifndef PARADMM_ENGINE_INTERNAL_H
 #define PARADMM_ENGINE_INTERNAL_H
 
+#include <stdint.h>
+
 #include "paradmm.h"
 
 #ifdef __cplusplus
@@ -43,6 +45,9 @@ void pdm_release_graph(graph *G);
 void pdm_legacy_fail(int code);
 
 void pdm_set_error(int code, const char *fmt, ...);
+
+/* uniform double in [0, 1) from a splitmix64 stream (packing_ops.c) */
+double pdm_uniform01(uint64_t *state);
 
 #ifdef __cplusplus
 }

@@ -311,6 +311,30 @@ void initialize_X_N_Z_M_U_rand(graph *G, double X_l, double X_u, double N_l, dou
     pdm_host_written(G);
 }
 
+int paradmm_init_rand_seeded(graph *G, uint64_t seed, double X_l, double X_u, double N_l, double N_u, double Z_l,
+                             double Z_u, double M_l, double M_u, double U_l, double U_u)
+{
+    /* initialize_X_N_Z_M_U_rand with the same draws in the same order (per
+     * entry X, N, M, U, then Z), from a splitmix64 stream of the given seed
+     * instead of the time-seeded rand(): reproducible runs */
+    if (!G || G->num_edges <= 0 || G->num_vars <= 0) {
+        pdm_set_error(PARADMM_ERR_STATE, "paradmm_init_rand_seeded: need a built graph");
+        return PARADMM_ERR_STATE;
+    }
+    alloc_state(G, 0);
+    uint64_t st = seed;
+    size_t l1 = (size_t)G->num_dims * G->num_edges, l2 = (size_t)G->num_dims * G->num_vars;
+    for (size_t i = 0; i < l1; i++) {
+        G->X[i] = X_l + (X_u - X_l) * pdm_uniform01(&st);
+        G->N[i] = N_l + (N_u - N_l) * pdm_uniform01(&st);
+        G->M[i] = M_l + (M_u - M_l) * pdm_uniform01(&st);
+        G->U[i] = U_l + (U_u - U_l) * pdm_uniform01(&st);
+    }
+    for (size_t i = 0; i < l2; i++) G->Z[i] = Z_l + (Z_u - Z_l) * pdm_uniform01(&st);
+    pdm_host_written(G);
+    return PARADMM_OK;
+}
+
 void initialize_X_N_Z_M_U_optimal(graph *G, double Zinit[], double Uinit[], int maxiter)
 {
     /* core/paradmm.c:266-375: Z = Zinit, X = Z of each edge's variable, then

@@ -90,6 +90,8 @@ static double unit_draw(uint64_t *s)
     return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
 }
 
+double pdm_uniform01(uint64_t *state) { return unit_draw(state); }
+
 int paradmm_packing_init_square(graph *G, uint64_t seed)
 {
     if (!G || G->num_dims != 3 || G->num_edges <= 0) {

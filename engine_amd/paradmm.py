@@ -110,6 +110,7 @@ def load():
         "paradmm_prox_kind_of": (C.c_int, [V]),
         "paradmm_packing_square": (C.c_int, [G, C.c_int, C.c_double]),
         "paradmm_packing_init_square": (C.c_int, [G, C.c_uint64]),
+        "paradmm_init_rand_seeded": (C.c_int, [G, C.c_uint64] + [C.c_double] * 10),
         "paradmm_gpu_prepare": (C.c_int, [G]),
         "paradmm_gpu_iterate": (C.c_int, [G, C.c_int]),
         "paradmm_gpu_sync_to_host": (C.c_int, [G]),
@@ -302,6 +303,12 @@ class Graph:
 
     def init_zeros(self):
         self.lib.initialize_X_N_Z_M_U_zeros(self.ptr)
+
+    def init_rand_seeded(self, seed, X=(0.0, 1.0), N=(0.0, 1.0), Z=(0.0, 1.0), M=(0.0, 1.0), U=(0.0, 1.0)):
+        """initialize_X_N_Z_M_U_rand with a seed (paradmm_init_rand_seeded);
+        each argument is a (low, high) bound pair."""
+        check(self.lib.paradmm_init_rand_seeded(self.ptr, seed, X[0], X[1], N[0], N[1], Z[0], Z[1], M[0], M[1],
+                                                U[0], U[1]), "paradmm_init_rand_seeded")
 
     def init_square(self, seed):
         check(self.lib.paradmm_packing_init_square(self.ptr, seed), "paradmm_packing_init_square")

@@ -98,6 +98,11 @@ int paradmm_packing_square(graph *G, int ncircles, double a_exp);
  * circle; X = N = M = Z of the edge's variable, U = 0.  The reference seeds
  * from time(NULL) (core/paradmm.c:247); this is the reproducible form. */
 int paradmm_packing_init_square(graph *G, uint64_t seed);
+/* initialize_X_N_Z_M_U_rand (core/paradmm.c:229-262) with a seed: the same
+ * bounds and order of draws, from splitmix64 instead of the time-seeded
+ * rand(), so runs are reproducible */
+int paradmm_init_rand_seeded(graph *G, uint64_t seed, double X_l, double X_u, double N_l, double N_u, double Z_l,
+                             double Z_u, double M_l, double M_u, double U_l, double U_u);
 
 /* ---- device-resident iteration --------------------------------------------
  * Equivalent to the reference's GPU driver (packing/gpu.cu:250-291: copy the

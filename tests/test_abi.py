@@ -188,3 +188,36 @@ def test_snapshot_round_trip(pdm, tmp_path):
     open(path, "wb").write(b"not a snapshot at all" * 4)
     with pytest.raises(pdm.ParadmmError, match="not a snapshot"):
         g.load_state(path)
+
+
+def _splitmix_unit(seed, count):
+    """count uniforms in [0, 1) from splitmix64 (Steele, Lea, Flood 2014),
+    53-bit mantissa: the stream paradmm_init_rand_seeded draws from."""
+    x, out = seed & 0xFFFFFFFFFFFFFFFF, []
+    for _ in range(count):
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        z ^= z >> 31
+        out.append((z >> 11) * (1.0 / 9007199254740992.0))
+    return np.array(out)
+
+
+def test_rand_init_seeded_draws_in_reference_order(pdm):
+    """paradmm_init_rand_seeded: per entry X, N, M, U (each in its bounds),
+    then Z, as initialize_X_N_Z_M_U_rand draws them (core/paradmm.c:247-261);
+    the same seed gives the same state."""
+    g = pdm.Graph(3)
+    g.packing_square(5)
+    b = dict(X=(-1.0, 2.0), N=(0.0, 0.5), Z=(3.0, 4.0), M=(-2.0, -1.0), U=(10.0, 12.0))
+    g.init_rand_seeded(1234, **b)
+    l1, l2 = 3 * g.nedge, 3 * g.nvar
+    u = _splitmix_unit(1234, 4 * l1 + l2)
+    for k, name in enumerate(("X", "N", "M", "U")):
+        lo, hi = b[name]
+        assert np.array_equal(g.array(name), lo + (hi - lo) * u[k:4 * l1:4]), name
+    lo, hi = b["Z"]
+    assert np.array_equal(g.array("Z"), lo + (hi - lo) * u[4 * l1:])
+    z = g.array("Z").copy()
+    g.init_rand_seeded(1234, **b)
+    assert np.array_equal(g.array("Z"), z)
