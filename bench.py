@@ -36,31 +36,37 @@ EDGE_UPDATE_BYTES = 72
 #   generic Z pass (fused M, U update): read X, U, rho, alpha; write U = 44
 #   dense fused pair-tile pass (X and Z work of a whole iteration in one
 #   pass): read X, U, rho, alpha; write X, U                           = 56
+#   the same pass inside a captured run with alpha = 1, the X array
+#   holding M = X + U (U not stored): read M, rho; write M             = 28
+# and with one rho and one alpha on every edge (the variant without those
+# streams) 48 and 24.
 BYTES_PER_EDGE = {
     "disk_collision_3d": 28, "box_3d": 28, "max_radius_3d": 28,
     "zpass_fused": 44,
-    "pairtile_fused": 56, "unary_fused": 56, "zreduce": 0,
+    "pairtile_fused": 56, "pairtile_m": 28, "unary_fused": 56, "zreduce": 0,
 }
+UNIFORM_BYTES_PER_EDGE = {"pairtile_fused": 48, "pairtile_m": 24}
 # kernels that carry a whole iteration's edge work (their launches count
 # E edge-updates each)
-FULL_ITERATION_KERNELS = ("pairtile_fused",)
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pairtile_fused_n5000_hbm_traffic.md")
+FULL_ITERATION_KERNELS = ("pairtile_fused", "pairtile_m")
+TRAFFIC_PROFILES = {k: os.path.join(ROOT, "profiles", f"{k}_n5000_hbm_traffic.md") for k in FULL_ITERATION_KERNELS}
 
 
 def edges_per_launch(name, n):
     pairs = n * (n - 1) // 2
     return {
         "disk_collision_3d": 2 * pairs, "box_3d": n, "max_radius_3d": n,
-        "zpass_fused": n * (n + 1), "pairtile_fused": 2 * pairs, "unary_fused": 2 * n,
+        "zpass_fused": n * (n + 1), "pairtile_fused": 2 * pairs, "pairtile_m": 2 * pairs, "unary_fused": 2 * n,
     }.get(name, 0)
 
 
 def measured_traffic(kernel, n, seed):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary (copy-kernel calibrated), when it covers this kernel and workload."""
-    if kernel != "pairtile_fused" or n != 5000 or seed != 0 or not os.path.exists(TRAFFIC_PROFILE):
+    path = TRAFFIC_PROFILES.get(kernel)
+    if path is None or n != 5000 or seed != 0 or not os.path.exists(path):
         return None
-    with open(TRAFFIC_PROFILE) as fh:
+    with open(path) as fh:
         for line in fh:
             if line.startswith("traffic_bytes_per_launch:"):
                 return int(line.split(":")[1])
@@ -233,8 +239,9 @@ def main():
     else:
         dom_bytes = BYTES_PER_EDGE.get(dom, 0) * dom_edges
     # rho = 1 and alpha = 1 on every edge: the tile kernel runs its variant
-    # without the per-edge rho/alpha streams (48 B per edge instead of 56)
-    kernel_bytes = (48 if dom == "pairtile_fused" else BYTES_PER_EDGE.get(dom, 0)) * dom_edges
+    # without the per-edge rho/alpha streams (48 B per edge instead of 56, or
+    # 24 inside a captured run that keeps M in the X array)
+    kernel_bytes = UNIFORM_BYTES_PER_EDGE.get(dom, BYTES_PER_EDGE.get(dom, 0)) * dom_edges
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(dom, args.n, args.seed)
     print(f"# kernels (mean ms per launch): " + ", ".join(f"{k} {v[0]:.4f}x{v[1]}" for k, v in prof.items()),

@@ -57,6 +57,7 @@ struct DenseState {
     float* d_uni = nullptr;   // [2] common rho, alpha (uniform runs)
     int* d_tile_pos = nullptr; // [nt] layout tile index -> launch position in d_tiles
     bool u_pending = false;   // U still lacks the update of the last iteration
+    bool m_state = false;     // the X array holds M = X + U and U is stale (inside a captured run)
 };
 
 // one workgroup per tile of side T; UNI: one rho and one alpha for every edge
@@ -338,6 +339,9 @@ static const char* mode_name(int mode)
     case DM_FUSED_NBUF: return "pairtile_fused_nbuf";
     case DM_LEGACY_X: return "pairtile_legacy_x";
     case DM_LEGACY_Z: return "pairtile_legacy_z";
+    case DM_M_ENTER: return "pairtile_m_enter";
+    case DM_M: return "pairtile_m";
+    case DM_M_EXIT: return "pairtile_m_exit";
     default: return "pairtile_finish";
     }
 }
@@ -358,6 +362,18 @@ static void launch_pairtile_t(hipStream_t st, const DArgs& a, unsigned nt, int m
         if (uni) k_pairtile<T, DM_FUSED_NBUF, true><<<nt, 256, 0, st>>>(a);
         else k_pairtile<T, DM_FUSED_NBUF, false><<<nt, 256, 0, st>>>(a);
         break;
+    case DM_M_ENTER:
+        if (uni) k_pairtile<T, DM_M_ENTER, true><<<nt, 256, 0, st>>>(a);
+        else k_pairtile<T, DM_M_ENTER, false><<<nt, 256, 0, st>>>(a);
+        break;
+    case DM_M:
+        if (uni) k_pairtile<T, DM_M, true><<<nt, 256, 0, st>>>(a);
+        else k_pairtile<T, DM_M, false><<<nt, 256, 0, st>>>(a);
+        break;
+    case DM_M_EXIT:
+        if (uni) k_pairtile<T, DM_M_EXIT, true><<<nt, 256, 0, st>>>(a);
+        else k_pairtile<T, DM_M_EXIT, false><<<nt, 256, 0, st>>>(a);
+        break;
     case DM_LEGACY_X: k_pairtile<T, DM_LEGACY_X, false><<<nt, 256, 0, st>>>(a); break;
     case DM_LEGACY_Z: k_pairtile<T, DM_LEGACY_Z, false><<<nt, 256, 0, st>>>(a); break;
     default: k_pairtile<T, DM_FINISH, false><<<nt, 256, 0, st>>>(a); break;
@@ -370,7 +386,7 @@ static int launch_pairtile(DevGraph* s, int mode)
     const unsigned nt = (unsigned)s->dense->nt;
     // the steady-state modes have a variant without the per-edge rho/alpha
     // streams when every edge shares one rho and one alpha
-    const bool uni = s->rho_uni && s->alpha_uni && mode <= DM_FUSED_NBUF;
+    const bool uni = s->rho_uni && s->alpha_uni && (mode <= DM_FUSED_NBUF || mode >= DM_M_ENTER);
     prof_begin(s, mode_name(mode));
     if (s->dense->T == 32) launch_pairtile_t<32>(s->stream, a, nt, mode, uni);
     else launch_pairtile_t<64>(s->stream, a, nt, mode, uni);
@@ -390,13 +406,33 @@ static int launch_zreduce(DevGraph* s)
     return PARADMM_OK;
 }
 
-int dense_iteration(DevGraph* s, int from_n)
+int dense_iteration(DevGraph* s, int from_n, int mphase)
 {
     DenseState* d = s->dense;
-    const int mode = from_n ? DM_FUSED_NBUF : (d->u_pending ? DM_FUSED : DM_FUSED_CURRENT);
+    int mode = from_n ? DM_FUSED_NBUF : (d->u_pending ? DM_FUSED : DM_FUSED_CURRENT);
+    // a run of captured iterations with alpha = 1: enter the M state, stay,
+    // and leave it on the run's last iteration (the state between runs is
+    // always X and U)
+    if (mphase == DENSE_M_ENTER) {
+        if (mode != DM_FUSED || d->m_state) {
+            pdm_set_error(PARADMM_ERR_STATE, "dense: the M state is entered from a pending U update only");
+            return PARADMM_ERR_STATE;
+        }
+        mode = DM_M_ENTER;
+    } else if (mphase == DENSE_M_MID || mphase == DENSE_M_EXIT) {
+        if (!d->m_state) {
+            pdm_set_error(PARADMM_ERR_STATE, "dense: not in the M state");
+            return PARADMM_ERR_STATE;
+        }
+        mode = mphase == DENSE_M_MID ? DM_M : DM_M_EXIT;
+    } else if (d->m_state) {
+        pdm_set_error(PARADMM_ERR_STATE, "dense: a plain iteration in the M state");
+        return PARADMM_ERR_STATE;
+    }
     RC(launch_pairtile(s, mode));
     RC(launch_zreduce(s));
     d->u_pending = true;
+    d->m_state = mode == DM_M_ENTER || mode == DM_M;
     return PARADMM_OK;
 }
 
@@ -415,13 +451,20 @@ int2* dense_tile_flags(DevGraph* s, const int** tile_pos, int* nt, int* tile_slo
 
 void dense_reset(DevGraph* s)
 {
-    if (s->dense) s->dense->u_pending = false;
+    if (s->dense) {
+        s->dense->u_pending = false;
+        s->dense->m_state = false;
+    }
 }
 
 int dense_finish(DevGraph* s)
 {
     DenseState* d = s->dense;
     if (!d || !d->u_pending) return PARADMM_OK;
+    if (d->m_state) {
+        pdm_set_error(PARADMM_ERR_STATE, "dense: finish inside the M state");
+        return PARADMM_ERR_STATE;
+    }
     RC(launch_pairtile(s, DM_FINISH));
     d->u_pending = false;
     return PARADMM_OK;

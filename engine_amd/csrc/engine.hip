@@ -1038,9 +1038,20 @@ static int device_pass(DevGraph* s, int pass)
 
 // One fused iteration: X pass (N = Z - U inline, or from the N buffer), then
 // the Z pass with M and the U update fused and residual partials.
-static int fused_iteration(DevGraph* s, int from_n)
+// Dense layout with alpha = 1 on every edge: the middle iterations of a run
+// keep M = X + U in the X array and do not store U (DENSE_M_*).
+static bool mrun_ok(const DevGraph* s) { return s->layout == 2 && s->alpha_uni && s->alpha_u == 1.0f; }
+
+// place of iteration i of a run of n in the M state sequence
+static int mphase_of(const DevGraph* s, int i, int n)
 {
-    if (s->layout == 2) return dense_iteration(s, from_n);
+    if (!mrun_ok(s) || n < 2) return DENSE_PLAIN;
+    return i == 0 ? DENSE_M_ENTER : i == n - 1 ? DENSE_M_EXIT : DENSE_M_MID;
+}
+
+static int fused_iteration(DevGraph* s, int from_n, int mphase = DENSE_PLAIN)
+{
+    if (s->layout == 2) return dense_iteration(s, from_n, mphase);
     RC(launch_xpass_generic(s, from_n));
     ZArgs a = zargs(s);
     prof_begin(s, "zpass_fused");
@@ -1065,7 +1076,7 @@ static int run_fused(DevGraph* s, int iters)
     const int GI = s->graph_iters;
     // host-callback factors synchronise with the host every iteration: no capture
     if (iters >= GI && GI > 1 && !s->host) {
-        const int variant = (s->rho_uni && s->alpha_uni) ? 1 : 0;
+        const int variant = ((s->rho_uni && s->alpha_uni) ? 1 : 0) | (mrun_ok(s) ? 2 : 0);
         if (s->gexec && s->graph_variant != variant) {
             // recorded with the other kernel variant: record again
             CK(hipGraphExecDestroy(s->gexec));
@@ -1077,7 +1088,7 @@ static int run_fused(DevGraph* s, int iters)
             s->graph_variant = variant;
             CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
             int rc = PARADMM_OK;
-            for (int i = 0; i < GI && !rc; ++i) rc = fused_iteration(s, 0);
+            for (int i = 0; i < GI && !rc; ++i) rc = fused_iteration(s, 0, mphase_of(s, i, GI));
             hipGraph_t gr = nullptr;
             hipError_t e = hipStreamEndCapture(s->stream, &gr);
             if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
@@ -1484,11 +1495,13 @@ extern "C" int paradmm_gpu_profile_kernels(graph* G, int iters, int max_kernels,
         RC(fused_iteration(s, 1));
         s->n_from_buffer = false;
     }
+    if (s->layout == 2 && !dense_steady(s)) RC(fused_iteration(s, 0));
     s->prof = true;
     s->prof_recs.clear();
     s->prof_used = 0;
     int rc = PARADMM_OK;
-    for (int i = 0; i < iters && !rc; ++i) rc = fused_iteration(s, 0);
+    // the kernel sequence of a captured run of iters iterations
+    for (int i = 0; i < iters && !rc; ++i) rc = fused_iteration(s, 0, mphase_of(s, i, iters));
     s->prof = false;
     RC(rc);
     CK(hipStreamSynchronize(s->stream));

@@ -155,7 +155,10 @@ void free_dev(DevGraph* s);
 // 0 when the dense layout was built, >0 on error.
 int dense_try_finalize(DevGraph* s, graph* G, const std::vector<int>& fkind);
 int dense_alloc_state(DevGraph* s);
-int dense_iteration(DevGraph* s, int from_n);
+// mphase: DENSE_PLAIN, or the place of the iteration in a captured run with
+// alpha = 1 (enter, middle, exit of the M state)
+enum { DENSE_PLAIN = 0, DENSE_M_ENTER = 1, DENSE_M_MID = 2, DENSE_M_EXIT = 3 };
+int dense_iteration(DevGraph* s, int from_n, int mphase = DENSE_PLAIN);
 int dense_legacy_xpass(DevGraph* s);
 int dense_legacy_zpass(DevGraph* s);
 int dense_residual_reduce(DevGraph* s);

@@ -32,8 +32,24 @@ enum {
     DM_FUSED_NBUF = 2,     // N from the N buffer, prox, partials       (after upload)
     DM_LEGACY_X = 3,       // N from the N buffer, prox                 (updateX)
     DM_LEGACY_Z = 4,       // partials of the stored M                   (updateZ)
-    DM_FINISH = 5          // U update only (+ primal residual)
+    DM_FINISH = 5,         // U update only (+ primal residual)
+    // alpha = 1 on every edge: U_{k+1} = U_k + X_{k+1} - Z_{k+1} = M_k - Z_{k+1}
+    // with M_k = X_{k+1} + U_k, so between the first and the last iteration
+    // of a run the X array can hold M alone and U is not stored: 24 instead
+    // of 48 bytes per edge per iteration
+    DM_M_ENTER = 6,        // as DM_FUSED, but the X array receives M = X + U and U is not stored
+    DM_M = 7,              // X array holds M: U = M - Z, N = Z - U, prox, partials, X array receives the new M
+    DM_M_EXIT = 8          // X array holds M: U = M - Z, N = Z - U, prox, partials, X and U stored (DM_FUSED's state)
 };
+
+// U + alpha (X - Z) of one edge component, d = X - Z.  For alpha = 1 it is
+// rounded as (X + U) - Z, the value the M state computes from its stored
+// M = X + U, so a captured run (which keeps M) and plain iterations give
+// the same bits.
+__device__ __forceinline__ float u_update(float u, float x, float z, float alpha, float d)
+{
+    return alpha == 1.0f ? (x + u) - z : u + alpha * d;
+}
 
 // One instance as the tile code sees it: pointers already offset to the
 // instance's first slot / circle / partial.
@@ -119,18 +135,20 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
                                           double& pr, double& qov)
 {
     using G = TG<T>;
-    constexpr bool APPLY_U = (MODE == DM_FUSED || MODE == DM_FINISH);
-    constexpr bool PROX = (MODE <= DM_LEGACY_X);
+    constexpr bool MIN = (MODE == DM_M || MODE == DM_M_EXIT);      // the X array holds M
+    constexpr bool MOUT = (MODE == DM_M_ENTER || MODE == DM_M);    // the X array receives M
+    constexpr bool APPLY_U = (MODE == DM_FUSED || MODE == DM_FINISH || MODE == DM_M_ENTER);
+    constexpr bool PROX = (MODE <= DM_LEGACY_X || MODE >= DM_M_ENTER);
     constexpr bool NBUF = (MODE == DM_FUSED_NBUF || MODE == DM_LEGACY_X);
-    constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z);
+    constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z || MODE >= DM_M_ENTER);
     constexpr bool PART_M = (MODE == DM_LEGACY_Z);
-    constexpr bool WRITE_U = APPLY_U;
+    constexpr bool WRITE_U = (MODE == DM_FUSED || MODE == DM_FINISH || MODE == DM_M_EXIT);
     constexpr bool WRITE_X = PROX;
     // a diagonal tile's vector stores also cover its unary slots (processed
     // afterwards by diag_unary, which reads their old X) and its padding:
     // X is loaded there so masked slots are written back unchanged
-    constexpr bool NEED_X = APPLY_U || (VAR == 2 && WRITE_X);
-    constexpr bool NEED_U = APPLY_U || (PROX && !NBUF) || (PART && !PART_M);
+    constexpr bool NEED_X = APPLY_U || (VAR == 2 && WRITE_X) || MIN;
+    constexpr bool NEED_U = (APPLY_U || (PROX && !NBUF) || (PART && !PART_M)) && !MIN;
     constexpr bool NEED_A = APPLY_U;
 
     const int N = a.N;
@@ -177,6 +195,9 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
         if (on) {
         float4 xa[3], xb[3], ua[3], ub[3], na[3], nbv[3], ma[3], mb[3];
         float4 ra, rb, aa, ab;
+        if constexpr (MIN)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) ua[k] = ub[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if constexpr (UNI) {
             ra = rb = make_float4(a.rho_u, a.rho_u, a.rho_u, a.rho_u);
             if constexpr (NEED_A) aa = ab = make_float4(a.alpha_u, a.alpha_u, a.alpha_u, a.alpha_u);
@@ -209,6 +230,7 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 if constexpr (NEED_U) { uA[k] = cmp(ua[k], c); uB[k] = cmp(ub[k], c); }
+                if constexpr (MIN) { uA[k] = cmp(xa[k], c) - zi[k]; uB[k] = cmp(xb[k], c) - zj[c][k]; }
             }
             if constexpr (APPLY_U) {
                 const float alA = cmp(aa, c), alB = cmp(ab, c);
@@ -216,8 +238,9 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
                 for (int k = 0; k < 3; ++k) {
                     const float dA = cmp(xa[k], c) - zi[k];
                     const float dB = cmp(xb[k], c) - zj[c][k];
-                    const float nuA = uA[k] + alA * dA;
-                    const float nuB = uB[k] + alB * dB;
+                    // alpha = 1: (X + U) - Z, the rounding of the M state's M - Z
+                    const float nuA = u_update(uA[k], cmp(xa[k], c), zi[k], alA, dA);
+                    const float nuB = u_update(uB[k], cmp(xb[k], c), zj[c][k], alB, dB);
                     uA[k] = vm ? nuA : uA[k];
                     uB[k] = vm ? nuB : uB[k];
                     prow += vm ? dA * dA + dB * dB : 0.0f;
@@ -254,12 +277,14 @@ __device__ __forceinline__ void pair_tile(const DArgs& a, const int t, const int
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 if constexpr (WRITE_X) {
-                    if constexpr (VAR == 2) {  // masked slots keep their X
-                        cmp(xa[k], c) = vm ? xA[k] : cmp(xa[k], c);
-                        cmp(xb[k], c) = vm ? xB[k] : cmp(xb[k], c);
+                    const float oA = MOUT ? xA[k] + uA[k] : xA[k];
+                    const float oB = MOUT ? xB[k] + uB[k] : xB[k];
+                    if constexpr (VAR == 2) {  // masked slots keep their X (or M)
+                        cmp(xa[k], c) = vm ? oA : cmp(xa[k], c);
+                        cmp(xb[k], c) = vm ? oB : cmp(xb[k], c);
                     } else {
-                        cmp(xa[k], c) = xA[k];
-                        cmp(xb[k], c) = xB[k];
+                        cmp(xa[k], c) = oA;
+                        cmp(xb[k], c) = oB;
                     }
                 }
                 if constexpr (WRITE_U) { cmp(ua[k], c) = uA[k]; cmp(ub[k], c) = uB[k]; }
@@ -315,11 +340,14 @@ __device__ __noinline__ UnaryAcc diag_unary_body(float* X, float* U, const float
                                                  const int cl)
 {
     constexpr int TT = T * T;
-    constexpr bool APPLY_U = (MODE == DM_FUSED || MODE == DM_FINISH);
-    constexpr bool PROX = (MODE <= DM_LEGACY_X);
+    constexpr bool MIN = (MODE == DM_M || MODE == DM_M_EXIT);
+    constexpr bool MOUT = (MODE == DM_M_ENTER || MODE == DM_M);
+    constexpr bool APPLY_U = (MODE == DM_FUSED || MODE == DM_FINISH || MODE == DM_M_ENTER);
+    constexpr bool PROX = (MODE <= DM_LEGACY_X || MODE >= DM_M_ENTER);
     constexpr bool NBUF = (MODE == DM_FUSED_NBUF || MODE == DM_LEGACY_X);
-    constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z);
+    constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z || MODE >= DM_M_ENTER);
     constexpr bool PART_M = (MODE == DM_LEGACY_Z);
+    constexpr bool WRITE_U = (MODE == DM_FUSED || MODE == DM_FINISH || MODE == DM_M_EXIT);
     UnaryAcc acc = {0.0, 0.0, 0.0};
     const int c = I * T + cl;
     if (c >= N) return acc;
@@ -343,19 +371,26 @@ __device__ __noinline__ UnaryAcc diag_unary_body(float* X, float* U, const float
         float u[3], x[3], n[3], m[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            u[k] = U[k * S + q];
-            if constexpr (APPLY_U) x[k] = X[k * S + q];
+            if constexpr (MIN) {
+                x[k] = X[k * S + q];  // M
+                u[k] = x[k] - z[k];
+            } else {
+                u[k] = U[k * S + q];
+                if constexpr (APPLY_U) x[k] = X[k * S + q];
+            }
         }
         if constexpr (APPLY_U) {
             const float al = UNI ? alpha_u : alpha[q];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const float d = x[k] - z[k];
-                u[k] += al * d;
+                u[k] = u_update(u[k], x[k], z[k], al, d);
                 prs += d * d;
-                U[k * S + q] = u[k];
             }
         }
+        if constexpr (WRITE_U)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) U[k * S + q] = u[k];
         if constexpr (PROX) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) n[k] = NBUF ? Nb[k * S + q] : z[k] - u[k];
@@ -367,7 +402,7 @@ __device__ __noinline__ UnaryAcc diag_unary_body(float* X, float* U, const float
                 x[2] = max_alpha_radius(n[2], r, aexp[c]);
             }
 #pragma unroll
-            for (int k = 0; k < 3; ++k) X[k * S + q] = x[k];
+            for (int k = 0; k < 3; ++k) X[k * S + q] = MOUT ? x[k] + u[k] : x[k];
         }
         if constexpr (PART) {
 #pragma unroll
@@ -426,8 +461,8 @@ template <int T, int MODE, bool QUAL, bool UNI = false>
 __device__ __forceinline__ void tile_pass(const DArgs& a, const int t, const int I, const int J, TileShared<T>& sh,
                                           double* rp, double* q3)
 {
-    constexpr bool APPLY_U = (MODE == DM_FUSED || MODE == DM_FINISH);
-    constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z);
+    constexpr bool APPLY_U = (MODE == DM_FUSED || MODE == DM_FINISH || MODE == DM_M_ENTER);
+    constexpr bool PART = (MODE <= DM_FUSED_NBUF || MODE == DM_LEGACY_Z || MODE >= DM_M_ENTER);
     const int tid = threadIdx.x;
     double pr = 0.0, qov = 0.0, qbox = 0.0, qsum = 0.0;
     if (I == J) {
@@ -613,8 +648,8 @@ __device__ __forceinline__ void wave_pair_tile(const DArgs& a, const int t, cons
                     for (int k = 0; k < 3; ++k) {
                         const float dA = cmp(xa[k], c) - zi[k];
                         const float dB = cmp(xb[k], c) - zj[c][k];
-                        const float nuA = uA[k] + alA * dA;
-                        const float nuB = uB[k] + alB * dB;
+                        const float nuA = u_update(uA[k], cmp(xa[k], c), zi[k], alA, dA);
+                        const float nuB = u_update(uB[k], cmp(xb[k], c), zj[c][k], alB, dB);
                         uA[k] = vm ? nuA : uA[k];
                         uB[k] = vm ? nuB : uB[k];
                         prow += vm ? dA * dA + dB * dB : 0.0f;
