@@ -298,6 +298,9 @@ def main():
             # pass); the fused kernel needs only kernel_min bytes per edge,
             # so these are the fractions that bound it
             "kernel_min_frac": kernel_bytes / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "note": ("frac counts 72 B per edge-update (X and U read and written by an X pass and a Z pass); "
+                     "the dominant kernel moves kernel_min bytes per edge (M alone inside captured runs), so "
+                     "kernel_min_frac and traffic_frac_of_copy are the fractions that bound it"),
             "traffic_gbs": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None,
             "traffic_frac_of_copy": (traffic / (dom_ms * 1e-3) / 1e9 / copy_gbs) if traffic else None,
             "mean_launch_ms": dom_ms, "copy_kernel_gbs": copy_gbs,
