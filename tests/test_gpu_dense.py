@@ -320,3 +320,52 @@ def test_read_single_arrays_matches_full_sync(monkeypatch, layout):
     assert np.array_equal(reads["U"].reshape(-1), g.array("U"))
     assert np.array_equal(reads["RHO"], g.array("RHOS"))
     g.close()
+
+
+def _per_edge_rho_alpha1(nc, seed):
+    g, o = square_pair(nc, seed, rho=10.0)
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(2.0, 30.0, size=o.nedge)
+    o.array("RHO")[:] = r
+    g.array("RHOS")[:] = r
+    o.array("ALPHA")[:] = 1.0
+    g.array("ALPHAS")[:] = 1.0
+    return g, o
+
+
+def test_m_state_runs_with_per_edge_rho_vs_oracle():
+    """alpha = 1 with a different rho on every edge: a captured run of 16
+    iterations keeps M = X + U in the X array (the per-edge rho variant)
+    and must follow the FP64 oracle; every step starts from the oracle's
+    state (uploaded), so only the run itself is compared.  The states are
+    taken after 400 iterations: earlier, a pair whose overlap test flips
+    between fp32 and FP64 makes any fp32 run part from the oracle (the
+    plain kernel as much as this one)."""
+    g, o = _per_edge_rho_alpha1(80, 4)
+    o.iterate(400)
+    errs = []
+    for _ in range(3):
+        for n in ("X", "M", "N", "U", "Z"):
+            g.array(n)[:] = o.array(OR_NAME[n])
+        g.sync_from_host()
+        g.iterate(16)
+        g.sync_to_host()
+        o.iterate(16)
+        errs.append(max_rel(g.array("Z"), o.array("Z")))
+    print("M-state runs with per-edge rho, Z error after 16 iterations:", " ".join(f"{e:.1e}" for e in errs))
+    assert g.layout() == 2
+    assert max(errs) < 1e-5
+
+
+def test_m_state_runs_match_eager_with_per_edge_rho():
+    """Bit-for-bit: 32 iterations as two captured runs (the M state, per-edge
+    rho) against 32 single iterations (the plain fused kernel)."""
+    g1, _ = _per_edge_rho_alpha1(70, 9)
+    g2, _ = _per_edge_rho_alpha1(70, 9)
+    g1.iterate(32)
+    for _ in range(32):
+        g2.iterate(1)
+    g1.sync_to_host()
+    g2.sync_to_host()
+    for n in ("Z", "U", "X"):
+        assert np.array_equal(g1.array(n), g2.array(n)), n
