@@ -456,8 +456,11 @@ namespace pdm {
 void free_dev(DevGraph* s)
 {
     if (!s) return;
-    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-    if (s->ghandle) (void)hipGraphDestroy(s->ghandle);
+    for (auto& r : s->runs) {
+        if (r.exec) (void)hipGraphExecDestroy(r.exec);
+        if (r.graph) (void)hipGraphDestroy(r.graph);
+    }
+    s->runs.clear();
     if (s->host) {
         if (s->host->h_buf) (void)hipHostFree(s->host->h_buf);
         delete s->host;
@@ -1060,45 +1063,99 @@ static int fused_iteration(DevGraph* s, int from_n, int mphase = DENSE_PLAIN)
     return PARADMM_OK;
 }
 
+// kernel variant a captured run depends on: the uniform rho/alpha kernels,
+// and the M state (dense, alpha = 1)
+static int run_variant(const DevGraph* s) { return ((s->rho_uni && s->alpha_uni) ? 1 : 0) | (mrun_ok(s) ? 2 : 0); }
+
+// The captured run of n iterations for the current variant (recorded on
+// first use; runs of another variant are dropped; at most 4 are kept).
+// Recording needs the steady state (U one update behind), which it also
+// leaves.
+static int captured_run(DevGraph* s, int n, hipGraphExec_t* out)
+{
+    const int variant = run_variant(s);
+    for (auto it = s->runs.begin(); it != s->runs.end();) {
+        if (it->variant != variant) {
+            CK(hipGraphExecDestroy(it->exec));
+            CK(hipGraphDestroy(it->graph));
+            it = s->runs.erase(it);
+        } else if (it->iters == n) {
+            *out = it->exec;
+            return PARADMM_OK;
+        } else {
+            ++it;
+        }
+    }
+    if (s->runs.size() >= 4) {  // the oldest run other than the main length
+        auto victim = s->runs.begin();
+        if (victim->iters == s->graph_iters && s->runs.size() > 1) ++victim;
+        CK(hipGraphExecDestroy(victim->exec));
+        CK(hipGraphDestroy(victim->graph));
+        s->runs.erase(victim);
+    }
+    DevGraph::CapturedRun r;
+    r.iters = n;
+    r.variant = variant;
+    CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = PARADMM_OK;
+    for (int i = 0; i < n && !rc; ++i) rc = fused_iteration(s, 0, mphase_of(s, i, n));
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(s->stream, &gr);
+    if (rc) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    r.graph = gr;
+    e = hipGraphInstantiate(&r.exec, gr, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(gr);
+        return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    s->runs.push_back(r);
+    *out = r.exec;
+    return PARADMM_OK;
+}
+
+// Brings the state to the one captured runs start from (eager iterations
+// if needed); returns the iterations that took.
+static int steady_start(DevGraph* s, int iters, int* used)
+{
+    *used = 0;
+    if (iters > 0 && s->n_from_buffer) {
+        RC(fused_iteration(s, 1));
+        s->n_from_buffer = false;
+        ++*used;
+    }
+    if (s->layout == 2 && !dense_steady(s) && iters > *used) {
+        // captured runs assume a pending U update
+        RC(fused_iteration(s, 0));
+        ++*used;
+    }
+    return PARADMM_OK;
+}
+
 static int run_fused(DevGraph* s, int iters)
 {
     if (iters <= 0) return PARADMM_OK;
-    if (s->n_from_buffer) {
-        RC(fused_iteration(s, 1));
-        s->n_from_buffer = false;
-        --iters;
-    }
-    if (s->layout == 2 && !dense_steady(s) && iters > 0) {
-        // the captured iterations assume a pending U update
-        RC(fused_iteration(s, 0));
-        --iters;
-    }
-    const int GI = s->graph_iters;
+    int used = 0;
+    RC(steady_start(s, iters, &used));
+    iters -= used;
     // host-callback factors synchronise with the host every iteration: no capture
-    if (iters >= GI && GI > 1 && !s->host) {
-        const int variant = ((s->rho_uni && s->alpha_uni) ? 1 : 0) | (mrun_ok(s) ? 2 : 0);
-        if (s->gexec && s->graph_variant != variant) {
-            // recorded with the other kernel variant: record again
-            CK(hipGraphExecDestroy(s->gexec));
-            CK(hipGraphDestroy(s->ghandle));
-            s->gexec = nullptr;
-            s->ghandle = nullptr;
+    if (!s->host && iters >= 2) {
+        const int GI = s->graph_iters;
+        hipGraphExec_t ex = nullptr;
+        if (iters >= GI) {
+            RC(captured_run(s, GI, &ex));
+            while (iters >= GI) {
+                CK(hipGraphLaunch(ex, s->stream));
+                iters -= GI;
+            }
         }
-        if (!s->gexec) {
-            s->graph_variant = variant;
-            CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-            int rc = PARADMM_OK;
-            for (int i = 0; i < GI && !rc; ++i) rc = fused_iteration(s, 0, mphase_of(s, i, GI));
-            hipGraph_t gr = nullptr;
-            hipError_t e = hipStreamEndCapture(s->stream, &gr);
-            if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
-            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
-            s->ghandle = gr;
-            CK(hipGraphInstantiate(&s->gexec, gr, nullptr, nullptr, 0));
-        }
-        while (iters >= GI) {
-            CK(hipGraphLaunch(s->gexec, s->stream));
-            iters -= GI;
+        if (iters >= 2) {
+            RC(captured_run(s, iters, &ex));
+            CK(hipGraphLaunch(ex, s->stream));
+            iters = 0;
         }
     }
     for (int i = 0; i < iters; ++i) RC(fused_iteration(s, 0));
@@ -1439,9 +1496,14 @@ extern "C" int paradmm_gpu_time_iterations(graph* G, int iters, double* ms)
 {
     DevGraph* s;
     RC(get_state(G, &s, true));
-    // build the captured graph outside the timed region
-    if (s->n_from_buffer) RC(run_fused(s, 1));
-    if (!s->gexec && iters >= s->graph_iters) RC(run_fused(s, s->graph_iters));
+    // reach the steady state and record the runs outside the timed region
+    int used = 0;
+    RC(steady_start(s, 2, &used));
+    if (!s->host) {
+        hipGraphExec_t ex = nullptr;
+        if (iters >= s->graph_iters) RC(captured_run(s, s->graph_iters, &ex));
+        if (iters % s->graph_iters >= 2) RC(captured_run(s, iters % s->graph_iters, &ex));
+    }
     CK(hipStreamSynchronize(s->stream));
     CK(hipEventRecord(s->ev0, s->stream));
     RC(run_fused(s, iters));

@@ -95,14 +95,19 @@ struct DevGraph {
     // per-edge change on the device): the dense kernels then skip those streams
     bool rho_uni = false, alpha_uni = false;
     float rho_u = 0.0f, alpha_u = 0.0f;
-    int graph_variant = -1;  // kernel variant the captured graph was recorded with
     bool iters_done = false;
     bool dense_unary_stale = false;
 
-    // captured iterations
-    int graph_iters = 16;
-    hipGraph_t ghandle = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    // captured runs of iterations: a run of graph_iters is replayed as often
+    // as it fits, the rest of a call is one run of its own length; runs are
+    // kept per (length, kernel variant), a few at a time
+    struct CapturedRun {
+        int iters = 0, variant = 0;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    int graph_iters = 64;
+    std::vector<CapturedRun> runs;
 
     DenseState* dense = nullptr;
     HostProx* host = nullptr;  // generic layout only
