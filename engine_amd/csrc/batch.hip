@@ -100,6 +100,7 @@ struct BArgs {
     // iteration it0, nothing else
     int it0, it_end, first_mode, finish_only;
     const float* inst_rho;  // [ninst][2]: common rho, alpha of each instance
+    int alpha1;             // alpha = 1 on every edge: block tiles keep M = X + U between checks
 };
 
 enum { FIRST_NBUF = 0, FIRST_CURRENT = 1, FIRST_FUSED = 2 };
@@ -243,7 +244,7 @@ __device__ __forceinline__ void phase_tiles_block(const BArgs& a, const BLists& 
         const DArgs v = inst_view(a, in, i);
         // in the steady state, instances whose edges share one rho and alpha
         // skip those streams
-        if constexpr (MODE == DM_FUSED && !QUAL) {
+        if constexpr ((MODE == DM_FUSED || MODE >= DM_M_ENTER) && !QUAL) {
             if (in.uni) tile_pass<64, MODE, QUAL, true>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
             else tile_pass<64, MODE, QUAL, false>(v, lt, td.y, td.z, sh, a.Rp + t, a.Q + 3L * t);
         } else {
@@ -298,7 +299,7 @@ template <int T, int MODE, bool QUAL>
 __device__ __forceinline__ void phase_tiles(const BArgs& a, const BArgs& sa, const BLists& L, TileShared<64>& sh,
                                             WaveShared* ws)
 {
-    constexpr bool STEADY = MODE == DM_FUSED && !QUAL;
+    constexpr bool STEADY = (MODE == DM_FUSED || MODE == DM_M) && !QUAL;
     if constexpr (T == 32) {
         if constexpr (STEADY) phase_tiles_wave<MODE, QUAL>(a, L, ws[threadIdx.x >> 6]);
         else phase_tiles_wave_call<MODE, QUAL>(sa, L, ws[threadIdx.x >> 6]);
@@ -371,6 +372,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     __syncthreads();
     if (a.it0 == 0 && !a.finish_only && blockIdx.x == 0 && threadIdx.x == 0) *a.t_start = wall_clock64();
     unsigned gen = 0;
+    bool mst = false;  // the X array holds M (alpha = 1, block tiles)
     rebuild_lists(a, L);
     if (a.finish_only) {
         // the pending U update of iteration it0 (its primal residual) and the
@@ -391,7 +393,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         else if (first && a.first_mode == FIRST_CURRENT)
             phase_tiles<T, DM_FUSED_CURRENT, false>(a, sa, L, sh, ws);
         else if (check) phase_tiles<T, DM_FUSED, true>(a, sa, L, sh, ws);
-        else phase_tiles<T, DM_FUSED, false>(a, sa, L, sh, ws);
+        else {
+            bool done = false;
+            if constexpr (T == 64) {
+                if (a.alpha1) {
+                    // between checks the tiles keep M = X + U (alpha = 1);
+                    // the iteration before a check, the final pass or the
+                    // launch's end stores X and U again
+                    const bool next_split =
+                        it + 1 >= a.it_end || it + 1 >= a.max_iters || (it + 1) % a.check_every == 0;
+                    if (!mst && !next_split) {
+                        phase_tiles<T, DM_M_ENTER, false>(a, sa, L, sh, ws);
+                        mst = true;
+                    } else if (!mst) {
+                        phase_tiles<T, DM_FUSED, false>(a, sa, L, sh, ws);
+                    } else if (next_split) {
+                        phase_tiles<T, DM_M_EXIT, false>(a, sa, L, sh, ws);
+                        mst = false;
+                    } else {
+                        phase_tiles<T, DM_M, false>(a, sa, L, sh, ws);
+                    }
+                    done = true;
+                }
+            }
+            if (!done) phase_tiles<T, DM_FUSED, false>(a, sa, L, sh, ws);
+        }
         if (!phase_barrier(a, grid, gen)) return;
         if (check) {
             phase_check(a, L, it, final);
@@ -630,6 +656,7 @@ struct paradmm_batch {
     int* raised = nullptr;      // [ninst] (mode 3)
     double* phist = nullptr;    // [ninst][2] (average)
     std::vector<int> uni0;      // per instance: one rho and alpha on every edge at upload
+    int alpha1 = 0;             // alpha = 1 on every edge of every instance
 };
 
 static void batch_free_dev(paradmm_batch* b)
@@ -807,6 +834,11 @@ static int batch_upload(paradmm_batch* b)
     }
     b->uni0.resize(b->ninst);
     for (int i = 0; i < b->ninst; ++i) b->uni0[i] = b->inst[i].uni;
+    b->alpha1 = 1;
+    for (int i = 0; i < b->ninst && b->alpha1; ++i) {
+        graph* G = b->graphs[i];
+        for (int e = 0; e < G->num_edges && b->alpha1; ++e) b->alpha1 = (float)G->ALPHAS[e] == 1.0f;
+    }
     CK(hipMemcpyAsync(b->inst_d, b->inst.data(), sizeof(BInst) * (size_t)b->ninst, hipMemcpyHostToDevice, b->stream));
     std::vector<float> ir(2 * (size_t)b->ninst);
     for (int i = 0; i < b->ninst; ++i) {
@@ -837,6 +869,7 @@ static int batch_launch(paradmm_batch* b, const paradmm_solve_opts* o, float* ms
     a.first_mode = first_mode;
     a.finish_only = finish_only;
     a.inst_rho = b->inst_rho;
+    a.alpha1 = b->alpha1;
     a.tol_p = o->tol_primal;
     a.tol_d = o->tol_dual;
     a.tol_f = o->tol_feas;

@@ -89,21 +89,28 @@ def test_batch_teacher_forced_steps(tile):
     assert np.mean(errs < 1e-5) >= 0.95
 
 
-def test_batch_equals_single_instance_solver(monkeypatch):
+@pytest.mark.parametrize("iters,sizes", [(3, [70, 130]), (40, [70, 100])])
+def test_batch_equals_single_instance_solver(monkeypatch, iters, sizes):
     """Tile 64 runs the same tile code as the single-instance dense layout
-    with tile side 64; only the order of the per-circle block sums differs."""
+    with tile side 64, and between checks the same M state (X array holding
+    X + U, alpha = 1); only the order of the per-circle block sums differs,
+    and not at all with two blocks (N <= 128): then 40 iterations agree bit
+    for bit.  (With three blocks the 4e-7 of the first iterations grows to
+    about 6e-4 by iteration 40, as any rounding difference does early in a
+    packing.)"""
     monkeypatch.setenv("PARADMM_DENSE_TILE", "64")
-    sizes = [70, 130]
     gs = _instances(sizes, 10.0)
     b = P.Batch(gs)
     b.configure(64, 2)
-    b.solve(3, tol=-1.0, tol_feas=-1.0, check_every=1000)
+    b.solve(iters, tol=-1.0, tol_feas=-1.0, check_every=1000)
     b.sync_to_host()
     for k, n in enumerate(sizes):
         s = P.packing_square(n, 11 + k, rho=10.0)
-        s.iterate(3)
+        s.iterate(iters)
         s.sync_to_host()
-        assert max_rel(gs[k].array("Z"), s.array("Z")) < 1e-6
+        err = max_rel(gs[k].array("Z"), s.array("Z"))
+        print(f"N={n}, {iters} iterations: batch vs single {err:.1e}")
+        assert err < 1e-6 if iters == 3 else err == 0.0
 
 
 def test_batch_retires_instances_independently():
