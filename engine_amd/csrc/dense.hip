@@ -58,6 +58,17 @@ struct DenseState {
     int* d_tile_pos = nullptr; // [nt] layout tile index -> launch position in d_tiles
     bool u_pending = false;   // U still lacks the update of the last iteration
     bool m_state = false;     // the X array holds M = X + U and U is stale (inside a captured run)
+
+    // contact state (one rho, alpha = 1; see the contact kernels below):
+    // the pair slots of X and U are stale, the contact lists hold the M of
+    // the pairs whose prox was active, every other pair edge has M = Z
+    bool c_state = false;
+    float4* d_L = nullptr;     // [2][nt][T*T][2] contact lists
+    int* d_cnt = nullptr;      // [2][nt] list lengths
+    int* d_cur = nullptr;      // [1] the list the last iteration wrote
+    float* d_Pc = nullptr;     // [CQ][nb][N] circle partials of the contact iteration
+    double* d_Rpc = nullptr;   // [N] primal residual per circle
+    float* d_Zpp = nullptr;    // [N][3] Z two iterations back
 };
 
 // one workgroup per tile of side T; UNI: one rho and one alpha for every edge
@@ -154,6 +165,504 @@ __global__ __launch_bounds__(256) void k_dense_res_final(int nt, const double* _
     if (threadIdx.x == 0) {
         out[0] = sp[0];
         out[1] = sq[0];
+    }
+}
+
+// ------------------------------------------------------------------ contact state
+//
+// With alpha = 1 the U update is U_{k+1} = M_k - Z_{k+1}.  A pair factor
+// whose prox is inactive returns X = N = Z_k - U_k, so M_k = X + U = Z_k:
+// its edges carry no state of their own, and at the next iteration
+// U = Z_k - Z_{k+1}, N = Z_{k+1} - U, the same for every inactive pair of a
+// circle.  So only the pairs whose prox was active (the contacts, about 1%
+// of all pairs for N = 5000 circles) need per-edge values: a tile keeps a
+// list of them, each entry the M of both edges and the pair's slot, and an
+// iteration reads that list, tests every pair of the tile against the
+// circles' common N, runs the prox on the colliding ones and writes the new
+// list.  The box and radius edges stay in their X and U slots.
+//
+// Per circle the tile sums, over its pair edges that are in either list,
+//   a = M - Z_k                    (consensus: Z_{k+1} = Z_k + sum a / (N + 1)),
+//   g = (X - Z_k) - (Z_k - Z_{k-1})  and |g|^2,
+// where a and g are 0 on every other edge.  The primal residual
+// sum |X - Z_{k+1}|^2 is then, per circle, |g - e|^2 summed over all N + 1
+// edges with e = Z_{k+1} - 2 Z_k + Z_{k-1}: S2 - 2 e.S1 + (N + 1)|e|^2.
+// The sums run in 2^-32 fixed point (64-bit LDS atomics), so they do not
+// depend on the order the lanes add them: runs are bitwise reproducible.
+//
+// The per-pair test replaces 24 bytes of M per edge read and written by
+// ~10 VALU operations; the state is Z, Z_prev, Z_prev2 and the lists.
+
+constexpr int CT = 32;        // tile side of the contact kernels
+constexpr int CTT = CT * CT;
+constexpr int CQ = 7;         // per-circle sums: a (3), g (3), |g|^2
+constexpr double CFIX = 4294967296.0;  // 2^32
+constexpr double CFIX_INV = 1.0 / 4294967296.0;
+
+struct CArgs {
+    int N, nb, nt;
+    long stride;
+    const int2* tiles;     // launch order
+    float* X;
+    float* U;              // box and radius slots of the diagonal tiles
+    float* Z;
+    float* Zp;
+    float* Zpp;
+    const float* aexp;
+    const float* uni;      // {rho, alpha}
+    int* cur;
+    float4* L;             // [2][nt][CTT][2]
+    int* cnt;              // [2][nt]
+    float* Pc;             // [CQ][nb][N]
+    double* Rpc;
+    double* Rd;
+};
+
+__device__ __forceinline__ void cadd(unsigned long long* acc, double v)
+{
+    atomicAdd(acc, (unsigned long long)__double2ll_rn(v * CFIX));
+}
+
+// Exclusive position of this lane's n items among the workgroup's (lanes in
+// thread order) and the workgroup total.  Contains one __syncthreads.
+__device__ __forceinline__ int block_offset(int n, int (&wsum)[4], int& total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int s = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(s, o, 64);
+        s += lane >= o ? y : 0;
+    }
+    if (lane == 63) wsum[w] = s;
+    __syncthreads();
+    int base = 0;
+    total = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        base += q < w ? wsum[q] : 0;
+        total += wsum[q];
+    }
+    return base + s - n;
+}
+
+// Shared scratch of one wavefront's tile.  Q circle sums: a (3), and with
+// the residual terms g (3), |g|^2.
+template <int Q>
+struct ContactWave {
+    unsigned long long acc[2 * CT][Q];  // fixed-point circle sums: rows (block I), then columns (block J)
+    float zc[2 * CT][6];                // Z_k and c = Z_k - Z_{k-1} of the tile's 64 circles
+    unsigned short queue[CTT];          // colliding pairs not in the previous list (slot)
+    unsigned mask[CT];                  // pairs in the previous list, one word per row
+};
+
+// (I, J) of layout tile t: row I starts at I nb - I (I - 1) / 2.
+__device__ __forceinline__ void tile_of(long t, int nb, int& I, int& J)
+{
+    const double b = 2.0 * nb + 1.0;
+    I = (int)((b - sqrt(b * b - 8.0 * (double)t)) * 0.5);
+    while (I > 0 && (long)I * nb - (long)I * (I - 1) / 2 > t) --I;
+    while ((long)(I + 1) * nb - (long)(I + 1) * I / 2 <= t) ++I;
+    J = I + (int)(t - ((long)I * nb - (long)I * (I - 1) / 2));
+}
+
+// One contact iteration: one wavefront per layout tile, four tiles per
+// workgroup, no workgroup barrier.
+//   pass 1: lane l tests the 16 pairs of rows 4 (l / 8) .. +3 and columns
+//     4 (l % 8) .. +3 with the circles' common N; the colliding ones not in
+//     the previous list go to a queue (lane order, then pair order);
+//   pass 2: the previous list (its order), then the queue, one pair per
+//     lane: its N (Z - U from its list entry, or the common one), the
+//     prox, the new list (in that order) and the circle sums.
+// RES: also the residual sums (the last iteration before residuals are read).
+template <bool RES>
+__global__ __launch_bounds__(256) void k_contact_tile(CArgs a)
+{
+    constexpr int Q = RES ? CQ : 3;
+    __shared__ ContactWave<Q> shw[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long t = (long)blockIdx.x * 4 + w;
+    if (t >= a.nt) return;  // whole wavefronts only
+    ContactWave<Q>& sh = shw[w];
+    const int N = a.N, nb = a.nb;
+    int I, J;
+    tile_of(t, nb, I, J);
+    const int par = *a.cur;
+    const long tp = (long)par * a.nt + t;
+    const float4* Lp = a.L + tp * (2 * CTT);
+    float4* Ln = a.L + ((long)(par ^ 1) * a.nt + t) * (2 * CTT);
+    const int cp = a.cnt[tp];
+    const float rho = a.uni[0];
+
+    {   // circle table: lanes 0..31 the rows' circles, 32..63 the columns'
+        const int cc = min(lane < CT ? I * CT + lane : J * CT + (lane - CT), N - 1);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float z = a.Z[3 * cc + k];
+            sh.zc[lane][k] = z;
+            sh.zc[lane][3 + k] = z - a.Zp[3 * cc + k];
+        }
+    }
+    for (int k = lane; k < 2 * CT * Q; k += 64) (&sh.acc[0][0])[k] = 0ull;
+    if (lane < CT) sh.mask[lane] = 0u;
+    wave_sync();
+    for (int e = lane; e < cp; e += 64) {
+        const int idx = __float_as_int(Lp[2 * e].w);
+        atomicOr(&sh.mask[idx >> 5], 1u << (idx & 31));
+    }
+    wave_sync();
+
+    // pass 1
+    const int r0 = (lane >> 3) * 4, c0 = (lane & 7) * 4;
+    // N of a circle's pairs off the lists: Z_k - (Z_{k-1} - Z_k) = Z_k + c
+    float nr[4][3], nc[4][3];
+    unsigned exb = 0u;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            nr[r][k] = sh.zc[r0 + r][k] + sh.zc[r0 + r][3 + k];
+            nc[r][k] = sh.zc[CT + c0 + r][k] + sh.zc[CT + c0 + r][3 + k];
+        }
+        exb |= ((sh.mask[r0 + r] >> c0) & 0xFu) << (4 * r);
+    }
+    if (cp == 0) exb = 0u;
+    const bool interior = I != J && (J + 1) * CT <= N;
+    unsigned qm = 0u;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool hit = pair_hits(nr[r][0], nr[r][1], nr[r][2], nc[c][0], nc[c][1], nc[c][2]);
+            qm |= hit ? 1u << (4 * r + c) : 0u;
+        }
+    qm &= ~exb;
+    if (!interior) {  // pairs of the tile: below the diagonal and past N are not
+        unsigned vmask = 0u;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int rl = r0 + r, cl = c0 + c;
+                const bool vm = (I * CT + rl < N) && (J * CT + cl < N) && (I != J || rl < cl);
+                vmask |= vm ? 1u << (4 * r + c) : 0u;
+            }
+        qm &= vmask;
+    }
+    int qn;
+    {
+        const int n = __popc(qm);
+        int s = n;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(s, o, 64);
+            s += lane >= o ? y : 0;
+        }
+        qn = __shfl(s, 63, 64);
+        int off = s - n;
+        unsigned m = qm;
+        while (m) {
+            const int b = __ffs(m) - 1;
+            m &= m - 1u;
+            sh.queue[off++] = (unsigned short)((r0 + (b >> 2)) * CT + c0 + (b & 3));
+        }
+    }
+    wave_sync();
+
+    // pass 2
+    int base = 0;
+    const int total = cp + qn;
+    for (int q0 = 0; q0 < total; q0 += 64) {
+        const int q = q0 + lane;
+        bool act = false;
+        int idx = 0;
+        float mA[3], mB[3];
+        if (q < total) {
+            const bool ex = q < cp;
+            float4 pa, pb;
+            if (ex) {
+                pa = Lp[2 * q];
+                pb = Lp[2 * q + 1];
+                idx = __float_as_int(pa.w);
+            } else {
+                idx = sh.queue[q - cp];
+            }
+            const int rl = idx >> 5, cl = idx & 31;
+            const int cs = I == J ? cl : CT + cl;
+            float zA[3], zB[3], cA[3], cB[3], uA[3], uB[3], nA[3], nB[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                zA[k] = sh.zc[rl][k];
+                cA[k] = sh.zc[rl][3 + k];
+                zB[k] = sh.zc[CT + cl][k];
+                cB[k] = sh.zc[CT + cl][3 + k];
+                uA[k] = -cA[k];  // Z_{k-1} - Z_k, exactly
+                uB[k] = -cB[k];
+                nA[k] = zA[k] + cA[k];
+                nB[k] = zB[k] + cB[k];
+            }
+            if (ex) {
+                const float ma[3] = {pa.x, pa.y, pa.z}, mb[3] = {pb.x, pb.y, pb.z};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    uA[k] = ma[k] - zA[k];
+                    uB[k] = mb[k] - zB[k];
+                    nA[k] = zA[k] - uA[k];
+                    nB[k] = zB[k] - uB[k];
+                }
+            }
+            float xA[3] = {nA[0], nA[1], nA[2]}, xB[3] = {nB[0], nB[1], nB[2]};
+            act = pair_project(nA[0], nA[1], nA[2], nB[0], nB[1], nB[2], rho, rho, rho, rho, xA[0], xA[1], xA[2],
+                               xB[0], xB[1], xB[2]);
+            if constexpr (RES) {
+                double gA2 = 0.0, gB2 = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float gA = (xA[k] - zA[k]) - cA[k];
+                    const float gB = (xB[k] - zB[k]) - cB[k];
+                    cadd(&sh.acc[rl][3 + k], gA);
+                    cadd(&sh.acc[cs][3 + k], gB);
+                    gA2 += (double)gA * gA;
+                    gB2 += (double)gB * gB;
+                }
+                cadd(&sh.acc[rl][6], gA2);
+                cadd(&sh.acc[cs][6], gB2);
+            }
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    mA[k] = xA[k] + uA[k];
+                    mB[k] = xB[k] + uB[k];
+                    cadd(&sh.acc[rl][k], mA[k] - zA[k]);
+                    cadd(&sh.acc[cs][k], mB[k] - zB[k]);
+                }
+            }
+        }
+        const unsigned long long bal = __ballot(act);
+        if (act) {
+            const int slot = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
+            Ln[2 * slot] = make_float4(mA[0], mA[1], mA[2], __int_as_float(idx));
+            Ln[2 * slot + 1] = make_float4(mB[0], mB[1], mB[2], 0.0f);
+        }
+        base += (int)__popcll(bal);
+    }
+    if (lane == 0) a.cnt[(long)(par ^ 1) * a.nt + t] = base;
+
+    // the diagonal tile's box and radius edges: the plain U update, prox
+    // and X, U stores of the fused iteration
+    if (I == J && lane < CT && I * CT + lane < N) {
+        const int cc = I * CT + lane;
+        const long S = a.stride;
+        float z[3], cz[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            z[k] = sh.zc[lane][k];
+            cz[k] = sh.zc[lane][3 + k];
+        }
+        const long p0 = t * (2 * CTT) + (long)lane * CT + lane;
+        double g2 = 0.0;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const long q = p0 + side * CTT;
+            float x[3], u[3], n[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                x[k] = a.X[k * S + q];
+                u[k] = (x[k] + a.U[k * S + q]) - z[k];
+                n[k] = z[k] - u[k];
+            }
+            if (side == 0) {
+                box3(x[0], x[1], x[2], n[0], n[1], n[2]);
+            } else {
+                x[0] = n[0];
+                x[1] = n[1];
+                x[2] = max_alpha_radius(n[2], rho, a.aexp[cc]);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a.X[k * S + q] = x[k];
+                a.U[k * S + q] = u[k];
+                cadd(&sh.acc[lane][k], (x[k] + u[k]) - z[k]);
+                if constexpr (RES) {
+                    const float g = (x[k] - z[k]) - cz[k];
+                    cadd(&sh.acc[lane][3 + k], g);
+                    g2 += (double)g * g;
+                }
+            }
+        }
+        if constexpr (RES) cadd(&sh.acc[lane][6], g2);
+    }
+    wave_sync();
+    // circle partials: Pc[q][K][c] for the rows (K = J) and columns (K = I)
+    const bool on = !(I == J && lane >= CT);
+    const int circ = lane < CT ? I * CT + lane : J * CT + (lane - CT);
+    const int K = lane < CT ? J : I;
+    if (on && circ < N)
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            a.Pc[((long)q * nb + K) * N + circ] = (float)((double)(long long)sh.acc[lane][q] * CFIX_INV);
+}
+
+// Z_{k+1} = Z_k + sum a / (N + 1) over the circle's nb block partials in
+// fixed order, the Z history shift, the list flip and, RES, the per-circle
+// primal and dual residuals.  Workgroup = the 32 circles of one tile block.
+template <bool RES>
+__global__ __launch_bounds__(32 * ZR_SLICES) void k_contact_zreduce(CArgs a)
+{
+    constexpr int Q = RES ? CQ : 3;
+    __shared__ float part[ZR_SLICES][Q][32];
+    const int ks = threadIdx.x >> 5, cl = threadIdx.x & 31;
+    const int c = blockIdx.x * CT + cl;
+    const int N = a.N, nb = a.nb;
+    float s[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) s[q] = 0.0f;
+    if (c < N)
+#pragma unroll 4
+        for (int K = ks; K < nb; K += ZR_SLICES)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) s[q] += a.Pc[((long)q * nb + K) * N + c];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) part[ks][q][cl] = s[q];
+    __syncthreads();
+    for (int h = ZR_SLICES / 2; h > 0; h >>= 1) {
+        if (ks < h)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) part[ks][q][cl] += part[ks + h][q][cl];
+        __syncthreads();
+    }
+    if (threadIdx.x >= 32 || c >= N) return;
+    const float deg = (float)(N + 1);
+    double p = 0.0, dz2 = 0.0;
+    if constexpr (RES) p = (double)part[0][6][cl];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float zk = a.Z[3 * c + k], zp = a.Zp[3 * c + k];
+        const float zn = zk + part[0][k][cl] / deg;
+        if constexpr (RES) {
+            const float dz = zn - zk;
+            const double e = (double)(dz - (zk - zp));
+            p += -2.0 * e * (double)part[0][3 + k][cl] + (double)deg * e * e;
+            dz2 += (double)dz * dz;
+        }
+        a.Zpp[3 * c + k] = zp;
+        a.Zp[3 * c + k] = zk;
+        a.Z[3 * c + k] = zn;
+    }
+    if constexpr (RES) {
+        const float rho = a.uni[0];
+        a.Rpc[c] = p;
+        a.Rd[c] = (double)rho * rho * (double)deg * dz2;
+    }
+    if (c == 0) *a.cur ^= 1;
+}
+
+// Enters the contact state from the steady X, U state (U one update behind,
+// Z = Z_k, Z_prev = Z_{k-1}): list 0 receives every pair with
+// M = X + U != Z_{k-1} on either edge, exactly, so the first contact
+// iteration computes the same U = M - Z_k as the fused one.
+__global__ __launch_bounds__(256) void k_contact_enter(CArgs a)
+{
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x;
+    const int2 ij0 = a.tiles[blockIdx.x];
+    const int I = ij0.x, J = ij0.y & ~TILE_RAISED;
+    const long t = (long)I * a.nb - (long)I * (I - 1) / 2 + (J - I);
+    const int N = a.N;
+    const long S = a.stride;
+    float4* L0 = a.L + t * (2 * CTT);
+    const int il = tid >> 3, c0 = (tid & 7) * 4;
+    const int i = I * CT + il;
+    const int ic = min(i, N - 1);
+    float zpi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zpi[k] = a.Zp[3 * ic + k];
+    unsigned exm = 0u;
+    float mA[4][3], mB[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int jl = c0 + c, j = J * CT + jl;
+        bool vm = (i < N) && (j < N);
+        if (I == J) vm = vm && (il < jl);
+        if (!vm) continue;
+        const long pa = t * (2 * CTT) + (long)il * CT + jl, pb = pa + CTT;
+        bool ex = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            mA[c][k] = a.X[k * S + pa] + a.U[k * S + pa];
+            mB[c][k] = a.X[k * S + pb] + a.U[k * S + pb];
+            ex = ex || mA[c][k] != zpi[k] || mB[c][k] != a.Zp[3 * j + k];
+        }
+        exm |= ex ? 1u << c : 0u;
+    }
+    int total = 0;
+    int slot = block_offset(__popc(exm), wsum, total);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        if ((exm >> c) & 1u) {
+            L0[2 * slot] = make_float4(mA[c][0], mA[c][1], mA[c][2], __int_as_float(il * CT + c0 + c));
+            L0[2 * slot + 1] = make_float4(mB[c][0], mB[c][1], mB[c][2], 0.0f);
+            ++slot;
+        }
+    if (tid == 0) a.cnt[t] = total;
+}
+
+// Leaves the contact state after iteration k (Z_prev = Z_k, Z_prev2 =
+// Z_{k-1}, list cur = L_k, the other = L_{k-1}): every pair edge gets
+//   U_k = M_{k-1} - Z_k,  X_k = M_k - U_k
+// with M = Z of its circle off the lists: the steady X, U state.
+__global__ __launch_bounds__(256) void k_contact_leave(CArgs a)
+{
+    __shared__ unsigned short map[2][CTT];
+    __shared__ unsigned mask[2][CT];
+    const int tid = threadIdx.x;
+    const int2 ij0 = a.tiles[blockIdx.x];
+    const int I = ij0.x, J = ij0.y & ~TILE_RAISED;
+    const long t = (long)I * a.nb - (long)I * (I - 1) / 2 + (J - I);
+    const int N = a.N;
+    const long S = a.stride;
+    const int par = *a.cur;
+    const float4* Ls[2] = {a.L + ((long)par * a.nt + t) * (2 * CTT), a.L + ((long)(par ^ 1) * a.nt + t) * (2 * CTT)};
+    const int cn[2] = {a.cnt[(long)par * a.nt + t], a.cnt[(long)(par ^ 1) * a.nt + t]};
+    if (tid < CT) mask[0][tid] = mask[1][tid] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+        for (int e = tid; e < cn[b]; e += 256) {
+            const int idx = __float_as_int(Ls[b][2 * e].w);
+            map[b][idx] = (unsigned short)e;
+            atomicOr(&mask[b][idx >> 5], 1u << (idx & 31));
+        }
+    __syncthreads();
+    const int il = tid >> 3, c0 = (tid & 7) * 4;
+    const int i = I * CT + il;
+    const int ic = min(i, N - 1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int jl = c0 + c, j = J * CT + jl;
+        bool vm = (i < N) && (j < N);
+        if (I == J) vm = vm && (il < jl);
+        if (!vm) continue;
+        const int idx = il * CT + jl;
+        const bool in_k = (mask[0][il] >> jl) & 1u, in_k1 = (mask[1][il] >> jl) & 1u;
+        float4 ek[2] = {}, ek1[2] = {};
+        if (in_k) { ek[0] = Ls[0][2 * map[0][idx]]; ek[1] = Ls[0][2 * map[0][idx] + 1]; }
+        if (in_k1) { ek1[0] = Ls[1][2 * map[1][idx]]; ek1[1] = Ls[1][2 * map[1][idx] + 1]; }
+        const long pa = t * (2 * CTT) + (long)idx, pb = pa + CTT;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int v = side == 0 ? ic : j;
+            const long q = side == 0 ? pa : pb;
+            const float mk[3] = {ek[side].x, ek[side].y, ek[side].z};
+            const float mk1[3] = {ek1[side].x, ek1[side].y, ek1[side].z};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float zk = a.Zp[3 * v + k];
+                const float u = (in_k1 ? mk1[k] : a.Zpp[3 * v + k]) - zk;
+                a.U[k * S + q] = u;
+                a.X[k * S + q] = (in_k ? mk[k] : zk) - u;
+            }
+        }
     }
 }
 
@@ -406,9 +915,121 @@ static int launch_zreduce(DevGraph* s)
     return PARADMM_OK;
 }
 
+// ---- contact state, host side
+
+static CArgs cargs(DevGraph* s)
+{
+    DenseState* d = s->dense;
+    CArgs a;
+    a.N = d->N;
+    a.nb = d->nb;
+    a.nt = d->nt;
+    a.stride = s->stride;
+    a.tiles = d->d_tiles;
+    a.X = s->d_X;
+    a.U = s->d_U;
+    a.Z = s->d_Z;
+    a.Zp = s->d_Zprev;
+    a.Zpp = d->d_Zpp;
+    a.aexp = d->d_aexp;
+    a.uni = d->d_uni;
+    a.cur = d->d_cur;
+    a.L = d->d_L;
+    a.cnt = d->d_cnt;
+    a.Pc = d->d_Pc;
+    a.Rpc = d->d_Rpc;
+    a.Rd = d->d_Rd;
+    return a;
+}
+
+bool dense_contact_ok(DevGraph* s)
+{
+    const DenseState* d = s->dense;
+    if (!d || d->T != CT) return false;
+    if (!(s->rho_uni && s->alpha_uni && s->alpha_u == 1.0f && s->policy_raised == 0)) return false;
+    const char* e = getenv("PARADMM_CONTACT");
+    return !(e && e[0] == '0');
+}
+
+bool dense_in_contact(DevGraph* s) { return s->dense && s->dense->c_state; }
+
+static int contact_alloc(DevGraph* s)
+{
+    DenseState* d = s->dense;
+    if (d->d_L) return PARADMM_OK;
+    RC(dmalloc(s, (void**)&d->d_L, sizeof(float4) * 2 * 2 * (size_t)d->nt * CTT));
+    RC(dmalloc(s, (void**)&d->d_cnt, sizeof(int) * 2 * (size_t)d->nt));
+    RC(dmalloc(s, (void**)&d->d_cur, sizeof(int)));
+    RC(dmalloc(s, (void**)&d->d_Pc, sizeof(float) * CQ * (size_t)d->nb * d->N));
+    RC(dmalloc(s, (void**)&d->d_Rpc, sizeof(double) * (size_t)d->N));
+    RC(dmalloc(s, (void**)&d->d_Zpp, sizeof(float) * 3 * (size_t)d->N));
+    return PARADMM_OK;
+}
+
+// res: also the residual sums (the last iteration of a run)
+static int contact_iteration(DevGraph* s, bool res)
+{
+    DenseState* d = s->dense;
+    const CArgs a = cargs(s);
+    const unsigned tiles_grid = (unsigned)((d->nt + 3) / 4);
+    prof_begin(s, res ? "contact_tile_res" : "contact_tile");
+    if (res) k_contact_tile<true><<<tiles_grid, 256, 0, s->stream>>>(a);
+    else k_contact_tile<false><<<tiles_grid, 256, 0, s->stream>>>(a);
+    prof_end(s);
+    CK(hipGetLastError());
+    prof_begin(s, res ? "contact_zreduce_res" : "contact_zreduce");
+    if (res) k_contact_zreduce<true><<<(unsigned)d->nb, 32 * ZR_SLICES, 0, s->stream>>>(a);
+    else k_contact_zreduce<false><<<(unsigned)d->nb, 32 * ZR_SLICES, 0, s->stream>>>(a);
+    prof_end(s);
+    CK(hipGetLastError());
+    return PARADMM_OK;
+}
+
+int dense_contact_enter(DevGraph* s)
+{
+    DenseState* d = s->dense;
+    if (d->c_state) return PARADMM_OK;
+    if (!d->u_pending || d->m_state || !dense_contact_ok(s)) {
+        pdm_set_error(PARADMM_ERR_STATE, "dense: the contact state is entered from the steady state with one rho "
+                                         "and alpha = 1");
+        return PARADMM_ERR_STATE;
+    }
+    RC(contact_alloc(s));
+    CK(hipMemsetAsync(d->d_cur, 0, sizeof(int), s->stream));
+    const CArgs a = cargs(s);
+    prof_begin(s, "contact_enter");
+    k_contact_enter<<<(unsigned)d->nt, 256, 0, s->stream>>>(a);
+    prof_end(s);
+    CK(hipGetLastError());
+    d->c_state = true;
+    return contact_iteration(s, true);
+}
+
+int dense_contact_leave(DevGraph* s)
+{
+    DenseState* d = s->dense;
+    if (!d || !d->c_state) return PARADMM_OK;
+    const CArgs a = cargs(s);
+    prof_begin(s, "contact_leave");
+    k_contact_leave<<<(unsigned)d->nt, 256, 0, s->stream>>>(a);
+    prof_end(s);
+    CK(hipGetLastError());
+    d->c_state = false;
+    return PARADMM_OK;
+}
+
 int dense_iteration(DevGraph* s, int from_n, int mphase)
 {
     DenseState* d = s->dense;
+    if (d->c_state || mphase == DENSE_CONTACT) {
+        // in the contact state every iteration is a contact iteration; only
+        // the middle ones of a captured run skip the residual sums
+        if (!d->c_state || from_n) {
+            pdm_set_error(PARADMM_ERR_STATE, "dense: a contact iteration outside the contact state");
+            return PARADMM_ERR_STATE;
+        }
+        return contact_iteration(s, mphase != DENSE_CONTACT);
+    }
     int mode = from_n ? DM_FUSED_NBUF : (d->u_pending ? DM_FUSED : DM_FUSED_CURRENT);
     // a run of captured iterations with alpha = 1: enter the M state, stay,
     // and leave it on the run's last iteration (the state between runs is
@@ -454,12 +1075,14 @@ void dense_reset(DevGraph* s)
     if (s->dense) {
         s->dense->u_pending = false;
         s->dense->m_state = false;
+        s->dense->c_state = false;
     }
 }
 
 int dense_finish(DevGraph* s)
 {
     DenseState* d = s->dense;
+    if (d && d->c_state) RC(dense_contact_leave(s));
     if (!d || !d->u_pending) return PARADMM_OK;
     if (d->m_state) {
         pdm_set_error(PARADMM_ERR_STATE, "dense: finish inside the M state");
@@ -470,18 +1093,28 @@ int dense_finish(DevGraph* s)
     return PARADMM_OK;
 }
 
-int dense_legacy_xpass(DevGraph* s) { return launch_pairtile(s, DM_LEGACY_X); }
+int dense_legacy_xpass(DevGraph* s)
+{
+    RC(dense_contact_leave(s));
+    return launch_pairtile(s, DM_LEGACY_X);
+}
 
 int dense_legacy_zpass(DevGraph* s)
 {
+    RC(dense_contact_leave(s));
     RC(launch_pairtile(s, DM_LEGACY_Z));
     return launch_zreduce(s);
 }
 
 int dense_residual_reduce(DevGraph* s)
 {
-    RC(dense_finish(s));
     DenseState* d = s->dense;
+    if (d->c_state) {  // per-circle residuals of the last contact iteration
+        k_dense_res_final<<<1, 256, 0, s->stream>>>(d->N, d->d_Rpc, d->N, d->d_Rd, s->d_res);
+        CK(hipGetLastError());
+        return PARADMM_OK;
+    }
+    RC(dense_finish(s));
     k_dense_res_final<<<1, 256, 0, s->stream>>>(d->nt, d->d_Rp, d->N, d->d_Rd, s->d_res);
     CK(hipGetLastError());
     return PARADMM_OK;

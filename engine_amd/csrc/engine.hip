@@ -1048,6 +1048,8 @@ static bool mrun_ok(const DevGraph* s) { return s->layout == 2 && s->alpha_uni &
 // place of iteration i of a run of n in the M state sequence
 static int mphase_of(const DevGraph* s, int i, int n)
 {
+    // contact state: the last iteration of a run also sums the residuals
+    if (s->layout == 2 && dense_in_contact(const_cast<DevGraph*>(s))) return i == n - 1 ? DENSE_PLAIN : DENSE_CONTACT;
     if (!mrun_ok(s) || n < 2) return DENSE_PLAIN;
     return i == 0 ? DENSE_M_ENTER : i == n - 1 ? DENSE_M_EXIT : DENSE_M_MID;
 }
@@ -1064,8 +1066,12 @@ static int fused_iteration(DevGraph* s, int from_n, int mphase = DENSE_PLAIN)
 }
 
 // kernel variant a captured run depends on: the uniform rho/alpha kernels,
-// and the M state (dense, alpha = 1)
-static int run_variant(const DevGraph* s) { return ((s->rho_uni && s->alpha_uni) ? 1 : 0) | (mrun_ok(s) ? 2 : 0); }
+// the M state (dense, alpha = 1) and the contact state
+static int run_variant(const DevGraph* s)
+{
+    const bool contact = s->layout == 2 && dense_in_contact(const_cast<DevGraph*>(s));
+    return ((s->rho_uni && s->alpha_uni) ? 1 : 0) | (mrun_ok(s) ? 2 : 0) | (contact ? 4 : 0);
+}
 
 // The captured run of n iterations for the current variant (recorded on
 // first use; runs of another variant are dropped; at most 4 are kept).
@@ -1132,6 +1138,14 @@ static int steady_start(DevGraph* s, int iters, int* used)
         RC(fused_iteration(s, 0));
         ++*used;
     }
+    if (s->layout == 2) {
+        // one rho and alpha = 1: the contact state (entering it runs an iteration)
+        if (dense_in_contact(s) && !dense_contact_ok(s)) RC(dense_contact_leave(s));
+        if (iters > *used && !dense_in_contact(s) && dense_contact_ok(s)) {
+            RC(dense_contact_enter(s));
+            ++*used;
+        }
+    }
     return PARADMM_OK;
 }
 
@@ -1196,8 +1210,10 @@ extern "C" void pdm_host_written(graph* G)
 }
 
 // Device state for G, finalised and current with respect to the graph
-// structure.  Uploads everything when the host was written since.
-static int get_state(graph* G, DevGraph** out, bool upload_if_new)
+// structure.  Uploads everything when the host was written since.  The
+// dense contact state is left (per-edge X and U restored) unless the caller
+// works on it (keep_contact).
+static int get_state(graph* G, DevGraph** out, bool upload_if_new, bool keep_contact = false)
 {
     if (!G) {
         pdm_set_error(PARADMM_ERR_ARG, "graph is NULL");
@@ -1222,6 +1238,7 @@ static int get_state(graph* G, DevGraph** out, bool upload_if_new)
         s->m_valid = false;
         if (s->layout == 2) dense_reset(s);
     }
+    if (!keep_contact && s->layout == 2) RC(dense_contact_leave(s));
     *out = s;
     return PARADMM_OK;
 }
@@ -1325,7 +1342,7 @@ extern "C" int paradmm_gpu_iterate(graph* G, int iters)
         return PARADMM_ERR_ARG;
     }
     DevGraph* s;
-    RC(get_state(G, &s, true));
+    RC(get_state(G, &s, true, true));
     RC(run_fused(s, iters));
     if (iters > 0) {
         s->dev_newer = true;
@@ -1364,7 +1381,7 @@ extern "C" int paradmm_gpu_read(graph* G, int which, double* out)
         return PARADMM_ERR_ARG;
     }
     DevGraph* s;
-    RC(get_state(G, &s, true));
+    RC(get_state(G, &s, true, which == PARADMM_READ_Z));
     RC(ensure_aux(s));
     if (which == PARADMM_READ_Z) {
         const long n = (long)s->V * s->D;
@@ -1416,7 +1433,7 @@ extern "C" int paradmm_gpu_residuals(graph* G, double* primal2, double* dual2)
 extern "C" int paradmm_gpu_square_quality(graph* G, double* max_overlap, double* max_box, double* sum_r)
 {
     DevGraph* s;
-    RC(get_state(G, &s, true));
+    RC(get_state(G, &s, true, true));
     double q[3];
     RC(square_quality(s, q));
     if (max_overlap) *max_overlap = q[0];
@@ -1432,7 +1449,7 @@ extern "C" int paradmm_gpu_solve(graph* G, const paradmm_solve_opts* o, paradmm_
         return PARADMM_ERR_ARG;
     }
     DevGraph* s;
-    RC(get_state(G, &s, true));
+    RC(get_state(G, &s, true, true));
     if (o->tol_feas >= 0 && s->D != 3) {
         pdm_set_error(PARADMM_ERR_SHAPE, "feasibility check needs the dim-3 packing");
         return PARADMM_ERR_SHAPE;
@@ -1495,7 +1512,7 @@ extern "C" int paradmm_gpu_solve(graph* G, const paradmm_solve_opts* o, paradmm_
 extern "C" int paradmm_gpu_time_iterations(graph* G, int iters, double* ms)
 {
     DevGraph* s;
-    RC(get_state(G, &s, true));
+    RC(get_state(G, &s, true, true));
     // reach the steady state and record the runs outside the timed region
     int used = 0;
     RC(steady_start(s, 2, &used));
@@ -1552,12 +1569,9 @@ extern "C" int paradmm_gpu_profile_kernels(graph* G, int iters, int max_kernels,
                                            int* count, int* nkernels)
 {
     DevGraph* s;
-    RC(get_state(G, &s, true));
-    if (s->n_from_buffer) {
-        RC(fused_iteration(s, 1));
-        s->n_from_buffer = false;
-    }
-    if (s->layout == 2 && !dense_steady(s)) RC(fused_iteration(s, 0));
+    RC(get_state(G, &s, true, true));
+    int used = 0;
+    RC(steady_start(s, 2, &used));
     s->prof = true;
     s->prof_recs.clear();
     s->prof_used = 0;

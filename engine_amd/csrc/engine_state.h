@@ -162,7 +162,9 @@ int dense_try_finalize(DevGraph* s, graph* G, const std::vector<int>& fkind);
 int dense_alloc_state(DevGraph* s);
 // mphase: DENSE_PLAIN, or the place of the iteration in a captured run with
 // alpha = 1 (enter, middle, exit of the M state)
-enum { DENSE_PLAIN = 0, DENSE_M_ENTER = 1, DENSE_M_MID = 2, DENSE_M_EXIT = 3 };
+// or DENSE_CONTACT: a contact-state iteration that skips the residual sums
+// (in the contact state every other phase is a contact iteration with them)
+enum { DENSE_PLAIN = 0, DENSE_M_ENTER = 1, DENSE_M_MID = 2, DENSE_M_EXIT = 3, DENSE_CONTACT = 4 };
 int dense_iteration(DevGraph* s, int from_n, int mphase = DENSE_PLAIN);
 int dense_legacy_xpass(DevGraph* s);
 int dense_legacy_zpass(DevGraph* s);
@@ -186,6 +188,15 @@ int sync_uniform(DevGraph* s);
 // the device state was replaced from the host: U is current
 void dense_reset(DevGraph* s);
 // applies the pending U update (and its primal residual) so U is current
+// (leaves the contact state first)
 int dense_finish(DevGraph* s);
+// contact state (one rho, alpha = 1, tile side 32): only the pairs whose
+// prox is active keep per-edge values; PARADMM_CONTACT=0 disables it
+bool dense_contact_ok(DevGraph* s);
+bool dense_in_contact(DevGraph* s);
+// from the steady state: enters the contact state and runs one iteration in it
+int dense_contact_enter(DevGraph* s);
+// back to the steady X, U state (no-op outside the contact state)
+int dense_contact_leave(DevGraph* s);
 
 }  // namespace pdm

@@ -38,6 +38,15 @@ __device__ __forceinline__ float max_alpha_radius(float n, float rho, float a)
     return r;
 }
 
+// the collision test of pair_project: |c1 - c2|^2 < (r1 + r2)^2
+__device__ __forceinline__ bool pair_hits(float c1x, float c1y, float r1, float c2x, float c2y, float r2)
+{
+    const float dx = c1x - c2x, dy = c1y - c2y;
+    const float d2 = dx * dx + dy * dy;
+    const float s = r1 + r2;
+    return d2 < s * s;
+}
+
 // No-overlap projection shared by the dim-2 and dim-3 pair operators
 // (packing/host.c:83-132).  Along the centre line circle 1 sits at 0 and
 // circle 2 at dist; when dist^2 < (r1+r2)^2 the tuple (x1, x2, r1, r2) is
@@ -50,10 +59,10 @@ __device__ __forceinline__ bool pair_project(float c1x, float c1y, float r1, flo
                                              float& o1x, float& o1y, float& or1,
                                              float& o2x, float& o2y, float& or2)
 {
+    if (!pair_hits(c1x, c1y, r1, c2x, c2y, r2)) return false;
     const float dx = c1x - c2x, dy = c1y - c2y;
     const float d2 = dx * dx + dy * dy;
     const float s = r1 + r2;
-    if (!(d2 < s * s)) return false;
     const float dist = sqrtf(d2);
     const float ix1 = 1.0f / wx1, ir1 = 1.0f / wr1, ix2 = 1.0f / wx2, ir2 = 1.0f / wr2;
     const float lam = (dist - s) / (ix1 + ir1 + ix2 + ir2);
