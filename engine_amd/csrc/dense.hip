@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_dense_res_final(int nt, const double* _
 // where a and g are 0 on every other edge.  The primal residual
 // sum |X - Z_{k+1}|^2 is then, per circle, |g - e|^2 summed over all N + 1
 // edges with e = Z_{k+1} - 2 Z_k + Z_{k-1}: S2 - 2 e.S1 + (N + 1)|e|^2.
-// The sums run in 2^-32 fixed point (64-bit LDS atomics), so they do not
+// The sums run in 2^-31 fixed point (64-bit LDS atomics), so they do not
 // depend on the order the lanes add them: runs are bitwise reproducible.
 //
 // The per-pair test replaces 24 bytes of M per edge read and written by
@@ -196,8 +196,7 @@ __global__ __launch_bounds__(256) void k_dense_res_final(int nt, const double* _
 constexpr int CT = 32;        // tile side of the contact kernels
 constexpr int CTT = CT * CT;
 constexpr int CQ = 7;         // per-circle sums: a (3), g (3), |g|^2
-constexpr double CFIX = 4294967296.0;  // 2^32
-constexpr double CFIX_INV = 1.0 / 4294967296.0;
+constexpr double CFIX_INV = 1.0 / 2147483648.0;  // 2^-31
 
 struct CArgs {
     int N, nb, nt;
@@ -218,9 +217,14 @@ struct CArgs {
     double* Rd;
 };
 
-__device__ __forceinline__ void cadd(unsigned long long* acc, double v)
+// acc += v in 2^-31 fixed point: v = k + f with k = rint(v) and f = v - k
+// exact (|f| <= 1/2), so v 2^31 = k 2^31 + f 2^31 needs no FP64 (|v| < 2^31)
+__device__ __forceinline__ void cadd(unsigned long long* acc, float v)
 {
-    atomicAdd(acc, (unsigned long long)__double2ll_rn(v * CFIX));
+    const float k = rintf(v);
+    const float f = v - k;
+    const long long q = (long long)(int)k * 2147483648LL + (long long)(int)(f * 2147483648.0f);
+    atomicAdd(acc, (unsigned long long)q);
 }
 
 // Exclusive position of this lane's n items among the workgroup's (lanes in
@@ -256,14 +260,17 @@ struct ContactWave {
     unsigned mask[CT];                  // pairs in the previous list, one word per row
 };
 
-// (I, J) of layout tile t: row I starts at I nb - I (I - 1) / 2.
-__device__ __forceinline__ void tile_of(long t, int nb, int& I, int& J)
+// (I, J) of layout tile t: row I starts at I nb - I (I - 1) / 2 (the
+// float estimate is corrected by the loops; nb < 1449 for any dense layout,
+// so every product fits in 32 bits)
+__device__ __forceinline__ void tile_of(int t, int nb, int& I, int& J)
 {
-    const double b = 2.0 * nb + 1.0;
-    I = (int)((b - sqrt(b * b - 8.0 * (double)t)) * 0.5);
-    while (I > 0 && (long)I * nb - (long)I * (I - 1) / 2 > t) --I;
-    while ((long)(I + 1) * nb - (long)(I + 1) * I / 2 <= t) ++I;
-    J = I + (int)(t - ((long)I * nb - (long)I * (I - 1) / 2));
+    const float b = 2.0f * nb + 1.0f;
+    I = (int)((b - sqrtf(fmaxf(b * b - 8.0f * (float)t, 0.0f))) * 0.5f);
+    I = max(0, min(I, nb - 1));
+    while (I > 0 && I * nb - I * (I - 1) / 2 > t) --I;
+    while (I + 1 < nb && (I + 1) * nb - (I + 1) * I / 2 <= t) ++I;
+    J = I + (t - (I * nb - I * (I - 1) / 2));
 }
 
 // One contact iteration: one wavefront per layout tile, four tiles per
@@ -281,7 +288,7 @@ __global__ __launch_bounds__(256) void k_contact_tile(CArgs a)
     constexpr int Q = RES ? CQ : 3;
     __shared__ ContactWave<Q> shw[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const long t = (long)blockIdx.x * 4 + w;
+    const int t = blockIdx.x * 4 + w;
     if (t >= a.nt) return;  // whole wavefronts only
     ContactWave<Q>& sh = shw[w];
     const int N = a.N, nb = a.nb;
@@ -415,15 +422,15 @@ __global__ __launch_bounds__(256) void k_contact_tile(CArgs a)
             act = pair_project(nA[0], nA[1], nA[2], nB[0], nB[1], nB[2], rho, rho, rho, rho, xA[0], xA[1], xA[2],
                                xB[0], xB[1], xB[2]);
             if constexpr (RES) {
-                double gA2 = 0.0, gB2 = 0.0;
+                float gA2 = 0.0f, gB2 = 0.0f;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const float gA = (xA[k] - zA[k]) - cA[k];
                     const float gB = (xB[k] - zB[k]) - cB[k];
                     cadd(&sh.acc[rl][3 + k], gA);
                     cadd(&sh.acc[cs][3 + k], gB);
-                    gA2 += (double)gA * gA;
-                    gB2 += (double)gB * gB;
+                    gA2 += gA * gA;
+                    gB2 += gB * gB;
                 }
                 cadd(&sh.acc[rl][6], gA2);
                 cadd(&sh.acc[cs][6], gB2);
@@ -459,8 +466,8 @@ __global__ __launch_bounds__(256) void k_contact_tile(CArgs a)
             z[k] = sh.zc[lane][k];
             cz[k] = sh.zc[lane][3 + k];
         }
-        const long p0 = t * (2 * CTT) + (long)lane * CT + lane;
-        double g2 = 0.0;
+        const long p0 = (long)t * (2 * CTT) + (long)lane * CT + lane;
+        float g2 = 0.0f;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const long q = p0 + side * CTT;
@@ -486,7 +493,7 @@ __global__ __launch_bounds__(256) void k_contact_tile(CArgs a)
                 if constexpr (RES) {
                     const float g = (x[k] - z[k]) - cz[k];
                     cadd(&sh.acc[lane][3 + k], g);
-                    g2 += (double)g * g;
+                    g2 += g * g;
                 }
             }
         }
@@ -505,33 +512,37 @@ __global__ __launch_bounds__(256) void k_contact_tile(CArgs a)
 
 // Z_{k+1} = Z_k + sum a / (N + 1) over the circle's nb block partials in
 // fixed order, the Z history shift, the list flip and, RES, the per-circle
-// primal and dual residuals.  Workgroup = the 32 circles of one tile block.
+// primal and dual residuals.  Workgroup = CZ_CIRC circles (one tile block:
+// 128-byte rows of the partials) x CZ_SLICES slices of the blocks, added by
+// a fixed-shape tree.
+constexpr int CZ_CIRC = 32, CZ_SLICES = 32;
+
 template <bool RES>
-__global__ __launch_bounds__(32 * ZR_SLICES) void k_contact_zreduce(CArgs a)
+__global__ __launch_bounds__(CZ_CIRC * CZ_SLICES) void k_contact_zreduce(CArgs a)
 {
     constexpr int Q = RES ? CQ : 3;
-    __shared__ float part[ZR_SLICES][Q][32];
-    const int ks = threadIdx.x >> 5, cl = threadIdx.x & 31;
-    const int c = blockIdx.x * CT + cl;
+    __shared__ float part[CZ_SLICES][Q][CZ_CIRC];
+    const int ks = threadIdx.x / CZ_CIRC, cl = threadIdx.x % CZ_CIRC;
+    const int c = blockIdx.x * CZ_CIRC + cl;
     const int N = a.N, nb = a.nb;
     float s[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) s[q] = 0.0f;
     if (c < N)
 #pragma unroll 4
-        for (int K = ks; K < nb; K += ZR_SLICES)
+        for (int K = ks; K < nb; K += CZ_SLICES)
 #pragma unroll
             for (int q = 0; q < Q; ++q) s[q] += a.Pc[((long)q * nb + K) * N + c];
 #pragma unroll
     for (int q = 0; q < Q; ++q) part[ks][q][cl] = s[q];
     __syncthreads();
-    for (int h = ZR_SLICES / 2; h > 0; h >>= 1) {
+    for (int h = CZ_SLICES / 2; h > 0; h >>= 1) {
         if (ks < h)
 #pragma unroll
             for (int q = 0; q < Q; ++q) part[ks][q][cl] += part[ks + h][q][cl];
         __syncthreads();
     }
-    if (threadIdx.x >= 32 || c >= N) return;
+    if (threadIdx.x >= CZ_CIRC || c >= N) return;
     const float deg = (float)(N + 1);
     double p = 0.0, dz2 = 0.0;
     if constexpr (RES) p = (double)part[0][6][cl];
@@ -978,8 +989,9 @@ static int contact_iteration(DevGraph* s, bool res)
     prof_end(s);
     CK(hipGetLastError());
     prof_begin(s, res ? "contact_zreduce_res" : "contact_zreduce");
-    if (res) k_contact_zreduce<true><<<(unsigned)d->nb, 32 * ZR_SLICES, 0, s->stream>>>(a);
-    else k_contact_zreduce<false><<<(unsigned)d->nb, 32 * ZR_SLICES, 0, s->stream>>>(a);
+    const unsigned zgrid = (unsigned)((d->N + CZ_CIRC - 1) / CZ_CIRC);
+    if (res) k_contact_zreduce<true><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a);
+    else k_contact_zreduce<false><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a);
     prof_end(s);
     CK(hipGetLastError());
     return PARADMM_OK;

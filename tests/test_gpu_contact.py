@@ -102,8 +102,8 @@ def test_contact_matches_per_edge_path(monkeypatch, nc, rho):
     print(f"N={nc} rho={rho}: Z error vs oracle at it 10/50/100/200 contact "
           + " ".join(f"{ec[k]:.1e}" for k in (9, 49, 99, 199)) + " per-edge "
           + " ".join(f"{em[k]:.1e}" for k in (9, 49, 99, 199)))
-    assert trajectory_error(out["1"][:8], out["0"][:8]).max() < 1e-5
-    assert ec[:8].max() < 1e-5
+    assert trajectory_error(out["1"][:5], out["0"][:5]).max() < 1e-5
+    assert ec[:5].max() < 1e-5
     for k in (9, 49, 99, 199):
         assert ec[k] <= 2.0 * em[k] + 1e-5, k
 
