@@ -63,10 +63,13 @@ struct DenseState {
     // the pair slots of X and U are stale, the contact lists hold the M of
     // the pairs whose prox was active, every other pair edge has M = Z
     bool c_state = false;
-    float4* d_L = nullptr;     // [2][nt][T*T][2] contact lists
-    int* d_cnt = nullptr;      // [2][nt] list lengths
-    int* d_cur = nullptr;      // [1] the list the last iteration wrote
-    float* d_Pc = nullptr;     // [CQ][nb][N] circle partials of the contact iteration
+    float4* d_L = nullptr;     // [2][nts][cs*cs][2] contact lists
+    int* d_cnt = nullptr;      // [2][nts] list lengths
+    int cs = 64;               // contact tile side (32 or 64)
+    int nbs = 0, nts = 0;      // contact tiling: blocks of cs circles, tiles
+    int c_par = 0;             // the list the last iteration wrote (kept on the host, passed to the kernels)
+    unsigned* d_Lmask = nullptr; // [2][nts][cs][cs/32] the pairs of each list, per tile row
+    float* d_Pc = nullptr;     // [CQ][nbs][N] circle partials of the contact iteration
     double* d_Rpc = nullptr;   // [N] primal residual per circle
     float* d_Zpp = nullptr;    // [N][3] Z two iterations back
 };
@@ -175,14 +178,16 @@ __global__ __launch_bounds__(256) void k_dense_res_final(int nt, const double* _
 // its edges carry no state of their own, and at the next iteration
 // U = Z_k - Z_{k+1}, N = Z_{k+1} - U, the same for every inactive pair of a
 // circle.  So only the pairs whose prox was active (the contacts, about 1%
-// of all pairs for N = 5000 circles) need per-edge values: a tile keeps a
-// list of them, each entry the M of both edges and the pair's slot, and an
-// iteration reads that list, tests every pair of the tile against the
-// circles' common N, runs the prox on the colliding ones and writes the new
-// list.  The box and radius edges stay in their X and U slots.
+// of all pairs for N = 5000 circles) need per-edge values.  The contact
+// state covers the pairs with contact tiles of CS x CS circles (its own
+// tiling, independent of the dense layout's); a contact tile keeps a list
+// of its active pairs, each entry the M of both edges and the pair's slot,
+// and an iteration reads that list, tests every pair of the tile against
+// the circles' common N, runs the prox on the colliding ones and writes the
+// new list.  The box and radius edges stay in their dense X and U slots.
 //
-// Per circle the tile sums, over its pair edges that are in either list,
-//   a = M - Z_k                    (consensus: Z_{k+1} = Z_k + sum a / (N + 1)),
+// Per circle a tile sums, over its pair edges that are in either list,
+//   a = M - Z_k                      (consensus: Z_{k+1} = Z_k + sum a / (N + 1)),
 //   g = (X - Z_k) - (Z_k - Z_{k-1})  and |g|^2,
 // where a and g are 0 on every other edge.  The primal residual
 // sum |X - Z_{k+1}|^2 is then, per circle, |g - e|^2 summed over all N + 1
@@ -193,26 +198,26 @@ __global__ __launch_bounds__(256) void k_dense_res_final(int nt, const double* _
 // The per-pair test replaces 24 bytes of M per edge read and written by
 // ~10 VALU operations; the state is Z, Z_prev, Z_prev2 and the lists.
 
-constexpr int CT = 32;        // tile side of the contact kernels
-constexpr int CTT = CT * CT;
 constexpr int CQ = 7;         // per-circle sums: a (3), g (3), |g|^2
 constexpr double CFIX_INV = 1.0 / 2147483648.0;  // 2^-31
 
 struct CArgs {
-    int N, nb, nt;
+    int N;
+    int nbs, nts;          // contact tiling: blocks of CS circles, tiles
+    int nb, T;             // dense layout: blocks, tile side (box and radius slots)
     long stride;
-    const int2* tiles;     // launch order
     float* X;
-    float* U;              // box and radius slots of the diagonal tiles
+    float* U;
     float* Z;
     float* Zp;
     float* Zpp;
     const float* aexp;
     const float* uni;      // {rho, alpha}
-    int* cur;
-    float4* L;             // [2][nt][CTT][2]
-    int* cnt;              // [2][nt]
-    float* Pc;             // [CQ][nb][N]
+    int par;               // the list the last iteration wrote
+    float4* L;             // [2][nts][CS*CS][2]
+    unsigned* Lmask;       // [2][nts][CS][CS/32] pairs of each list, per tile row
+    int* cnt;              // [2][nts]
+    float* Pc;             // [CQ][nbs][N]
     double* Rpc;
     double* Rd;
 };
@@ -227,7 +232,36 @@ __device__ __forceinline__ void cadd(unsigned long long* acc, float v)
     atomicAdd(acc, (unsigned long long)q);
 }
 
-// Exclusive position of this lane's n items among the workgroup's (lanes in
+// (I, J) of tile t of a triangular tiling with nb blocks: row I starts at
+// I nb - I (I - 1) / 2 (the float estimate is corrected by the loops)
+__device__ __forceinline__ void tri_tile(int t, int nb, int& I, int& J)
+{
+    const float b = 2.0f * nb + 1.0f;
+    I = (int)((b - sqrtf(fmaxf(b * b - 8.0f * (float)t, 0.0f))) * 0.5f);
+    I = max(0, min(I, nb - 1));
+    while (I > 0 && I * nb - I * (I - 1) / 2 > t) --I;
+    while (I + 1 < nb && (I + 1) * nb - (I + 1) * I / 2 <= t) ++I;
+    J = I + (t - (I * nb - I * (I - 1) / 2));
+}
+
+// dense-layout slot of the edge of circle i in pair (i, j), i < j (the
+// edge of j is T*T further)
+__device__ __forceinline__ long dense_pair_slot(int i, int j, int nb, int T)
+{
+    const int I = i / T, J = j / T;
+    const long t = (long)I * nb - (long)I * (I - 1) / 2 + (J - I);
+    return t * (2L * T * T) + (long)(i % T) * T + (j % T);
+}
+
+// box slot of circle c (its radius slot is T*T further)
+__device__ __forceinline__ long dense_unary_slot(int c, int nb, int T)
+{
+    const int I = c / T;
+    const long t = (long)I * nb - (long)I * (I - 1) / 2;
+    return t * (2L * T * T) + (long)(c % T) * T + (c % T);
+}
+
+// Exclusive position of this thread's n items among the workgroup's (in
 // thread order) and the workgroup total.  Contains one __syncthreads.
 __device__ __forceinline__ int block_offset(int n, int (&wsum)[4], int& total)
 {
@@ -250,114 +284,220 @@ __device__ __forceinline__ int block_offset(int n, int (&wsum)[4], int& total)
     return base + s - n;
 }
 
-// Shared scratch of one wavefront's tile.  Q circle sums: a (3), and with
-// the residual terms g (3), |g|^2.
-template <int Q>
+// Orders one wavefront's LDS accesses across its lanes.  A wavefront's LDS
+// operations execute in issue order, so a wavefront-scope fence (a compiler
+// ordering point, no instruction) suffices; s_waitcnt 0 would also wait for
+// the wavefront's list stores to reach memory.
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// Shared scratch of one contact tile (one wavefront).  The pairs to project
+// form one stream per tile: the previous list's pairs, then the colliding
+// pairs found by the tests, which pass through a ring in LDS.
+constexpr int CRING = 2048;  // > 63 pending + the 1024 pairs one test step can add
+
+template <int CS, int Q>
 struct ContactWave {
-    unsigned long long acc[2 * CT][Q];  // fixed-point circle sums: rows (block I), then columns (block J)
-    float zc[2 * CT][6];                // Z_k and c = Z_k - Z_{k-1} of the tile's 64 circles
-    unsigned short queue[CTT];          // colliding pairs not in the previous list (slot)
-    unsigned mask[CT];                  // pairs in the previous list, one word per row
+    unsigned long long acc[2 * CS][Q];  // fixed-point circle sums: rows (block P), then columns (block Q)
+    float zc[2 * CS][6];                // Z_k and c = Z_k - Z_{k-1} of the tile's 2 CS circles
+    unsigned short ring[CRING];         // colliding pairs not in the previous list (slot)
+    unsigned mask[CS][CS / 32];         // pairs in the previous list, per row
+    unsigned nmask[CS][CS / 32];        // pairs in the new list
 };
 
-// (I, J) of layout tile t: row I starts at I nb - I (I - 1) / 2 (the
-// float estimate is corrected by the loops; nb < 1449 for any dense layout,
-// so every product fits in 32 bits)
-__device__ __forceinline__ void tile_of(int t, int nb, int& I, int& J)
+// One pair of the contact tile through the prox: N from its list entry (ex)
+// or the circles' common N, the circle sums; returns whether the prox was
+// active, with the new M of both edges.
+template <int CS, int Q>
+__device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bool ex, float4 pa, float4 pb,
+                                             bool diag, float rho, float (&mA)[3], float (&mB)[3])
 {
-    const float b = 2.0f * nb + 1.0f;
-    I = (int)((b - sqrtf(fmaxf(b * b - 8.0f * (float)t, 0.0f))) * 0.5f);
-    I = max(0, min(I, nb - 1));
-    while (I > 0 && I * nb - I * (I - 1) / 2 > t) --I;
-    while (I + 1 < nb && (I + 1) * nb - (I + 1) * I / 2 <= t) ++I;
-    J = I + (t - (I * nb - I * (I - 1) / 2));
+    const int rl = idx / CS, cl = idx % CS;
+    const int cs = diag ? cl : CS + cl;
+    float zA[3], zB[3], cA[3], cB[3], uA[3], uB[3], nA[3], nB[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        zA[k] = sh.zc[rl][k];
+        cA[k] = sh.zc[rl][3 + k];
+        zB[k] = sh.zc[CS + cl][k];
+        cB[k] = sh.zc[CS + cl][3 + k];
+    }
+    if (ex) {
+        const float ma[3] = {pa.x, pa.y, pa.z}, mb[3] = {pb.x, pb.y, pb.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            uA[k] = ma[k] - zA[k];
+            uB[k] = mb[k] - zB[k];
+            nA[k] = zA[k] - uA[k];
+            nB[k] = zB[k] - uB[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            uA[k] = -cA[k];  // Z_{k-1} - Z_k, exactly
+            uB[k] = -cB[k];
+            nA[k] = zA[k] + cA[k];  // Z_k - (Z_{k-1} - Z_k), exactly
+            nB[k] = zB[k] + cB[k];
+        }
+    }
+    float xA[3] = {nA[0], nA[1], nA[2]}, xB[3] = {nB[0], nB[1], nB[2]};
+    const bool act = pair_project(nA[0], nA[1], nA[2], nB[0], nB[1], nB[2], rho, rho, rho, rho, xA[0], xA[1], xA[2],
+                                  xB[0], xB[1], xB[2]);
+    if constexpr (Q == CQ) if (act || ex) {  // g = 0 on the other edges
+        float gA2 = 0.0f, gB2 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float gA = (xA[k] - zA[k]) - cA[k];
+            const float gB = (xB[k] - zB[k]) - cB[k];
+            cadd(&sh.acc[rl][3 + k], gA);
+            cadd(&sh.acc[cs][3 + k], gB);
+            gA2 += gA * gA;
+            gB2 += gB * gB;
+        }
+        cadd(&sh.acc[rl][6], gA2);
+        cadd(&sh.acc[cs][6], gB2);
+    }
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            mA[k] = xA[k] + uA[k];
+            mB[k] = xB[k] + uB[k];
+            cadd(&sh.acc[rl][k], mA[k] - zA[k]);
+            cadd(&sh.acc[cs][k], mB[k] - zB[k]);
+        }
+    }
+    return act;
 }
 
-// One contact iteration: one wavefront per layout tile, four tiles per
-// workgroup, no workgroup barrier.
-//   pass 1: lane l tests the 16 pairs of rows 4 (l / 8) .. +3 and columns
-//     4 (l % 8) .. +3 with the circles' common N; the colliding ones not in
-//     the previous list go to a queue (lane order, then pair order);
-//   pass 2: the previous list (its order), then the queue, one pair per
-//     lane: its N (Z - U from its list entry, or the common one), the
-//     prox, the new list (in that order) and the circle sums.
-// RES: also the residual sums (the last iteration before residuals are read).
-template <bool RES>
-__global__ __launch_bounds__(256) void k_contact_tile(CArgs a)
+// One round of the tile's pair stream: items d .. d + 63 (item < cp: entry
+// of the previous list, else ring slot item - cp); the active pairs are
+// appended to the new list in lane order.
+template <int CS, int Q>
+__device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, int d, int avail, int cp, const float4* Lp,
+                                              float4 pre0, float4 pre1, float4* Ln, int& base, bool diag, float rho)
+{
+    constexpr int CSS = CS * CS;
+    const int lane = threadIdx.x;
+    const int item = d + lane;
+    bool act = false;
+    int idx = 0;
+    float mA[3], mB[3];
+    if (item < avail) {
+        const bool ex = item < cp;
+        float4 pa = pre0, pb = pre1;
+        if (ex) {
+            if (d > 0) {
+                pa = Lp[2 * item];
+                pb = Lp[2 * item + 1];
+            }
+            idx = __float_as_int(pa.w) & (CSS - 1);
+        } else {
+            idx = sh.ring[(item - cp) & (CRING - 1)];
+        }
+        act = contact_pair(sh, idx, ex, pa, pb, diag, rho, mA, mB);
+    }
+    const unsigned long long bal = __ballot(act);
+    if (act) {
+        const int slot = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
+        Ln[2 * slot] = make_float4(mA[0], mA[1], mA[2], __int_as_float(idx));
+        Ln[2 * slot + 1] = make_float4(mB[0], mB[1], mB[2], 0.0f);
+        atomicOr(&sh.nmask[idx / CS][(idx % CS) >> 5], 1u << (idx & 31));
+    }
+    base += (int)__popcll(bal);
+}
+
+// One contact iteration of one contact tile (P, Q) of side CS: a wavefront
+// of 64 lanes, lane l owning rows RL (l / 8) .. +RL-1 and columns
+// RL (l % 8) .. +RL-1 (RL = CS / 8).  The tile's pair stream is the
+// previous list's pairs, then, two rows of every lane at a time, the pairs
+// that collide with the circles' common N and are not in that list (lane
+// order, then pair order); whenever 64 items are pending they go through
+// the prox as one round.  The new list is written in stream order.  RES:
+// also the residual sums (the last iteration before residuals are read).
+template <int CS, bool RES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_contact_tile(CArgs a)
 {
     constexpr int Q = RES ? CQ : 3;
-    __shared__ ContactWave<Q> shw[4];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int t = blockIdx.x * 4 + w;
-    if (t >= a.nt) return;  // whole wavefronts only
-    ContactWave<Q>& sh = shw[w];
-    const int N = a.N, nb = a.nb;
-    int I, J;
-    tile_of(t, nb, I, J);
-    const int par = *a.cur;
-    const long tp = (long)par * a.nt + t;
-    const float4* Lp = a.L + tp * (2 * CTT);
-    float4* Ln = a.L + ((long)(par ^ 1) * a.nt + t) * (2 * CTT);
-    const int cp = a.cnt[tp];
+    constexpr int CSS = CS * CS, RL = CS / 8, W = CS / 32;
+    __shared__ ContactWave<CS, Q> sh;
+    const int lane = threadIdx.x;
+    const int t = blockIdx.x;
+    const int N = a.N, nbs = a.nbs;
+    int P, Qb;
+    tri_tile(t, nbs, P, Qb);
+    const bool diag = P == Qb;
+    const long tp = (long)a.par * a.nts + t, tn = (long)(a.par ^ 1) * a.nts + t;
+    const float4* Lp = a.L + tp * (2 * CSS);
+    float4* Ln = a.L + tn * (2 * CSS);
+    const int cp = min(a.cnt[tp], CSS);
     const float rho = a.uni[0];
 
-    {   // circle table: lanes 0..31 the rows' circles, 32..63 the columns'
-        const int cc = min(lane < CT ? I * CT + lane : J * CT + (lane - CT), N - 1);
+    // circle table: entries 0..CS-1 the rows' circles, CS..2CS-1 the columns'
+    for (int e = lane; e < 2 * CS; e += 64) {
+        const int cc = min((e < CS ? P * CS + e : Qb * CS + (e - CS)), N - 1);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const float z = a.Z[3 * cc + k];
-            sh.zc[lane][k] = z;
-            sh.zc[lane][3 + k] = z - a.Zp[3 * cc + k];
+            sh.zc[e][k] = z;
+            sh.zc[e][3 + k] = z - a.Zp[3 * cc + k];
         }
     }
-    for (int k = lane; k < 2 * CT * Q; k += 64) (&sh.acc[0][0])[k] = 0ull;
-    if (lane < CT) sh.mask[lane] = 0u;
-    wave_sync();
-    for (int e = lane; e < cp; e += 64) {
-        const int idx = __float_as_int(Lp[2 * e].w);
-        atomicOr(&sh.mask[idx >> 5], 1u << (idx & 31));
+    for (int e = lane; e < CS * W; e += 64) {
+        (&sh.mask[0][0])[e] = a.Lmask[tp * CS * W + e];
+        (&sh.nmask[0][0])[e] = 0u;
     }
-    wave_sync();
+    for (int k = lane; k < 2 * CS * Q; k += 64) (&sh.acc[0][0])[k] = 0ull;
+    wave_lds_fence();
+    // the first 64 entries of the previous list, in flight during the tests
+    float4 pre0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pre1 = pre0;
+    if (lane < cp) {
+        pre0 = Lp[2 * lane];
+        pre1 = Lp[2 * lane + 1];
+    }
 
-    // pass 1
-    const int r0 = (lane >> 3) * 4, c0 = (lane & 7) * 4;
-    // N of a circle's pairs off the lists: Z_k - (Z_{k-1} - Z_k) = Z_k + c
-    float nr[4][3], nc[4][3];
-    unsigned exb = 0u;
+    const int r0 = (lane >> 3) * RL, c0 = (lane & 7) * RL;
+    const bool interior = !diag && (Qb + 1) * CS <= N;
+    float nc[RL][3];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int c = 0; c < RL; ++c)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            nr[r][k] = sh.zc[r0 + r][k] + sh.zc[r0 + r][3 + k];
-            nc[r][k] = sh.zc[CT + c0 + r][k] + sh.zc[CT + c0 + r][3 + k];
-        }
-        exb |= ((sh.mask[r0 + r] >> c0) & 0xFu) << (4 * r);
+        for (int k = 0; k < 3; ++k) nc[c][k] = sh.zc[CS + c0 + c][k] + sh.zc[CS + c0 + c][3 + k];
+    int base = 0;           // entries of the new list so far
+    int done = 0;           // stream items taken through the prox
+    int avail = cp;         // stream items so far (previous list + ring)
+    while (avail - done >= 64) {
+        contact_round(sh, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
+        done += 64;
     }
-    if (cp == 0) exb = 0u;
-    const bool interior = I != J && (J + 1) * CT <= N;
-    unsigned qm = 0u;
+#pragma unroll 1
+    for (int h = 0; h < RL / 2; ++h) {
+        // the tests of rows r0 + 2h, r0 + 2h + 1
+        unsigned qm = 0u;
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < 2; ++r) {
+            const int rl = r0 + 2 * h + r;
+            float nr[3];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const bool hit = pair_hits(nr[r][0], nr[r][1], nr[r][2], nc[c][0], nc[c][1], nc[c][2]);
-            qm |= hit ? 1u << (4 * r + c) : 0u;
-        }
-    qm &= ~exb;
-    if (!interior) {  // pairs of the tile: below the diagonal and past N are not
-        unsigned vmask = 0u;
+            for (int k = 0; k < 3; ++k) nr[k] = sh.zc[rl][k] + sh.zc[rl][3 + k];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int rl = r0 + r, cl = c0 + c;
-                const bool vm = (I * CT + rl < N) && (J * CT + cl < N) && (I != J || rl < cl);
-                vmask |= vm ? 1u << (4 * r + c) : 0u;
+            for (int c = 0; c < RL; ++c) {
+                const bool hit = pair_hits(nr[0], nr[1], nr[2], nc[c][0], nc[c][1], nc[c][2]);
+                qm |= hit ? 1u << (RL * r + c) : 0u;
             }
-        qm &= vmask;
-    }
-    int qn;
-    {
+            const unsigned ex = (sh.mask[rl][c0 >> 5] >> (c0 & 31)) & ((1u << RL) - 1u);
+            qm &= ~(ex << (RL * r));
+        }
+        if (!interior) {  // pairs of the tile: not below the diagonal, not past N
+            unsigned vmask = 0u;
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int c = 0; c < RL; ++c) {
+                    const int rl = r0 + 2 * h + r, cl = c0 + c;
+                    const bool vm = (P * CS + rl < N) && (Qb * CS + cl < N) && (!diag || rl < cl);
+                    vmask |= vm ? 1u << (RL * r + c) : 0u;
+                }
+            qm &= vmask;
+        }
         const int n = __popc(qm);
         int s = n;
 #pragma unroll
@@ -365,156 +505,86 @@ __global__ __launch_bounds__(256) void k_contact_tile(CArgs a)
             const int y = __shfl_up(s, o, 64);
             s += lane >= o ? y : 0;
         }
-        qn = __shfl(s, 63, 64);
-        int off = s - n;
+        int off = avail - cp + s - n;
+        avail += __shfl(s, 63, 64);
         unsigned m = qm;
         while (m) {
             const int b = __ffs(m) - 1;
             m &= m - 1u;
-            sh.queue[off++] = (unsigned short)((r0 + (b >> 2)) * CT + c0 + (b & 3));
+            sh.ring[(off++) & (CRING - 1)] = (unsigned short)((r0 + 2 * h + b / RL) * CS + c0 + b % RL);
+        }
+        wave_lds_fence();
+        while (avail - done >= 64) {
+            contact_round(sh, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
+            done += 64;
         }
     }
-    wave_sync();
-
-    // pass 2
-    int base = 0;
-    const int total = cp + qn;
-    for (int q0 = 0; q0 < total; q0 += 64) {
-        const int q = q0 + lane;
-        bool act = false;
-        int idx = 0;
-        float mA[3], mB[3];
-        if (q < total) {
-            const bool ex = q < cp;
-            float4 pa, pb;
-            if (ex) {
-                pa = Lp[2 * q];
-                pb = Lp[2 * q + 1];
-                idx = __float_as_int(pa.w);
-            } else {
-                idx = sh.queue[q - cp];
-            }
-            const int rl = idx >> 5, cl = idx & 31;
-            const int cs = I == J ? cl : CT + cl;
-            float zA[3], zB[3], cA[3], cB[3], uA[3], uB[3], nA[3], nB[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                zA[k] = sh.zc[rl][k];
-                cA[k] = sh.zc[rl][3 + k];
-                zB[k] = sh.zc[CT + cl][k];
-                cB[k] = sh.zc[CT + cl][3 + k];
-                uA[k] = -cA[k];  // Z_{k-1} - Z_k, exactly
-                uB[k] = -cB[k];
-                nA[k] = zA[k] + cA[k];
-                nB[k] = zB[k] + cB[k];
-            }
-            if (ex) {
-                const float ma[3] = {pa.x, pa.y, pa.z}, mb[3] = {pb.x, pb.y, pb.z};
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    uA[k] = ma[k] - zA[k];
-                    uB[k] = mb[k] - zB[k];
-                    nA[k] = zA[k] - uA[k];
-                    nB[k] = zB[k] - uB[k];
-                }
-            }
-            float xA[3] = {nA[0], nA[1], nA[2]}, xB[3] = {nB[0], nB[1], nB[2]};
-            act = pair_project(nA[0], nA[1], nA[2], nB[0], nB[1], nB[2], rho, rho, rho, rho, xA[0], xA[1], xA[2],
-                               xB[0], xB[1], xB[2]);
-            if constexpr (RES) {
-                float gA2 = 0.0f, gB2 = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float gA = (xA[k] - zA[k]) - cA[k];
-                    const float gB = (xB[k] - zB[k]) - cB[k];
-                    cadd(&sh.acc[rl][3 + k], gA);
-                    cadd(&sh.acc[cs][3 + k], gB);
-                    gA2 += gA * gA;
-                    gB2 += gB * gB;
-                }
-                cadd(&sh.acc[rl][6], gA2);
-                cadd(&sh.acc[cs][6], gB2);
-            }
-            if (act) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    mA[k] = xA[k] + uA[k];
-                    mB[k] = xB[k] + uB[k];
-                    cadd(&sh.acc[rl][k], mA[k] - zA[k]);
-                    cadd(&sh.acc[cs][k], mB[k] - zB[k]);
-                }
-            }
-        }
-        const unsigned long long bal = __ballot(act);
-        if (act) {
-            const int slot = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
-            Ln[2 * slot] = make_float4(mA[0], mA[1], mA[2], __int_as_float(idx));
-            Ln[2 * slot + 1] = make_float4(mB[0], mB[1], mB[2], 0.0f);
-        }
-        base += (int)__popcll(bal);
-    }
-    if (lane == 0) a.cnt[(long)(par ^ 1) * a.nt + t] = base;
+    if (avail > done) contact_round(sh, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
+    if (lane == 0) a.cnt[tn] = base;
 
     // the diagonal tile's box and radius edges: the plain U update, prox
     // and X, U stores of the fused iteration
-    if (I == J && lane < CT && I * CT + lane < N) {
-        const int cc = I * CT + lane;
+    for (int e = lane; diag && e < CS; e += 64) {
+        const int cc = P * CS + e;
+        if (cc >= N) break;
         const long S = a.stride;
         float z[3], cz[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            z[k] = sh.zc[lane][k];
-            cz[k] = sh.zc[lane][3 + k];
+            z[k] = sh.zc[e][k];
+            cz[k] = sh.zc[e][3 + k];
         }
-        const long p0 = (long)t * (2 * CTT) + (long)lane * CT + lane;
+        const long p0 = dense_unary_slot(cc, a.nb, a.T);
+        const long TT = (long)a.T * a.T;
         float g2 = 0.0f;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
-            const long q = p0 + side * CTT;
-            float x[3], u[3], n[3];
+            const long q = p0 + side * TT;
+            float x[3], u[3], nn[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 x[k] = a.X[k * S + q];
                 u[k] = (x[k] + a.U[k * S + q]) - z[k];
-                n[k] = z[k] - u[k];
+                nn[k] = z[k] - u[k];
             }
             if (side == 0) {
-                box3(x[0], x[1], x[2], n[0], n[1], n[2]);
+                box3(x[0], x[1], x[2], nn[0], nn[1], nn[2]);
             } else {
-                x[0] = n[0];
-                x[1] = n[1];
-                x[2] = max_alpha_radius(n[2], rho, a.aexp[cc]);
+                x[0] = nn[0];
+                x[1] = nn[1];
+                x[2] = max_alpha_radius(nn[2], rho, a.aexp[cc]);
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 a.X[k * S + q] = x[k];
                 a.U[k * S + q] = u[k];
-                cadd(&sh.acc[lane][k], (x[k] + u[k]) - z[k]);
+                cadd(&sh.acc[e][k], (x[k] + u[k]) - z[k]);
                 if constexpr (RES) {
                     const float g = (x[k] - z[k]) - cz[k];
-                    cadd(&sh.acc[lane][3 + k], g);
+                    cadd(&sh.acc[e][3 + k], g);
                     g2 += g * g;
                 }
             }
         }
-        if constexpr (RES) cadd(&sh.acc[lane][6], g2);
+        if constexpr (RES) cadd(&sh.acc[e][6], g2);
     }
-    wave_sync();
-    // circle partials: Pc[q][K][c] for the rows (K = J) and columns (K = I)
-    const bool on = !(I == J && lane >= CT);
-    const int circ = lane < CT ? I * CT + lane : J * CT + (lane - CT);
-    const int K = lane < CT ? J : I;
-    if (on && circ < N)
+    wave_lds_fence();
+    for (int e = lane; e < CS * W; e += 64) a.Lmask[tn * CS * W + e] = (&sh.nmask[0][0])[e];
+    // circle partials: Pc[q][K][c], rows into K = Q, columns into K = P
+    for (int e = lane; e < (diag ? CS : 2 * CS); e += 64) {
+        const int circ = e < CS ? P * CS + e : Qb * CS + (e - CS);
+        const int K = e < CS ? Qb : P;
+        if (circ < N)
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-            a.Pc[((long)q * nb + K) * N + circ] = (float)((double)(long long)sh.acc[lane][q] * CFIX_INV);
+            for (int q = 0; q < Q; ++q)
+                a.Pc[((long)q * nbs + K) * N + circ] = (float)((double)(long long)sh.acc[e][q] * CFIX_INV);
+    }
 }
 
-// Z_{k+1} = Z_k + sum a / (N + 1) over the circle's nb block partials in
-// fixed order, the Z history shift, the list flip and, RES, the per-circle
-// primal and dual residuals.  Workgroup = CZ_CIRC circles (one tile block:
-// 128-byte rows of the partials) x CZ_SLICES slices of the blocks, added by
-// a fixed-shape tree.
+// Z_{k+1} = Z_k + sum a / (N + 1) over the circle's nbs block partials in
+// fixed order, the Z history shift and, RES, the per-circle primal and dual
+// residuals.  Workgroup = CZ_CIRC circles (128-byte rows of the partials) x
+// CZ_SLICES slices of the blocks, added by a fixed-shape tree.
 constexpr int CZ_CIRC = 32, CZ_SLICES = 32;
 
 template <bool RES>
@@ -524,15 +594,15 @@ __global__ __launch_bounds__(CZ_CIRC * CZ_SLICES) void k_contact_zreduce(CArgs a
     __shared__ float part[CZ_SLICES][Q][CZ_CIRC];
     const int ks = threadIdx.x / CZ_CIRC, cl = threadIdx.x % CZ_CIRC;
     const int c = blockIdx.x * CZ_CIRC + cl;
-    const int N = a.N, nb = a.nb;
+    const int N = a.N, nbs = a.nbs;
     float s[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) s[q] = 0.0f;
     if (c < N)
 #pragma unroll 4
-        for (int K = ks; K < nb; K += CZ_SLICES)
+        for (int K = ks; K < nbs; K += CZ_SLICES)
 #pragma unroll
-            for (int q = 0; q < Q; ++q) s[q] += a.Pc[((long)q * nb + K) * N + c];
+            for (int q = 0; q < Q; ++q) s[q] += a.Pc[((long)q * nbs + K) * N + c];
 #pragma unroll
     for (int q = 0; q < Q; ++q) part[ks][q][cl] = s[q];
     __syncthreads();
@@ -565,101 +635,125 @@ __global__ __launch_bounds__(CZ_CIRC * CZ_SLICES) void k_contact_zreduce(CArgs a
         a.Rpc[c] = p;
         a.Rd[c] = (double)rho * rho * (double)deg * dz2;
     }
-    if (c == 0) *a.cur ^= 1;
 }
 
 // Enters the contact state from the steady X, U state (U one update behind,
-// Z = Z_k, Z_prev = Z_{k-1}): list 0 receives every pair with
-// M = X + U != Z_{k-1} on either edge, exactly, so the first contact
-// iteration computes the same U = M - Z_k as the fused one.
+// Z = Z_k, Z_prev = Z_{k-1}): list 0 of contact tile blockIdx.x receives
+// every pair with M = X + U != Z_{k-1} on either edge, exactly, so the first
+// contact iteration computes the same U = M - Z_k as the fused one.  256
+// threads; thread: row tid / (256 / CS), CS^2 / 256 consecutive columns.
+template <int CS>
 __global__ __launch_bounds__(256) void k_contact_enter(CArgs a)
 {
+    constexpr int CSS = CS * CS, W = CS / 32, TPR = 256 / CS, CPT = CSS / 256;
     __shared__ int wsum[4];
+    __shared__ unsigned emask[CS][W];
     const int tid = threadIdx.x;
-    const int2 ij0 = a.tiles[blockIdx.x];
-    const int I = ij0.x, J = ij0.y & ~TILE_RAISED;
-    const long t = (long)I * a.nb - (long)I * (I - 1) / 2 + (J - I);
+    const int t = blockIdx.x;
     const int N = a.N;
-    const long S = a.stride;
-    float4* L0 = a.L + t * (2 * CTT);
-    const int il = tid >> 3, c0 = (tid & 7) * 4;
-    const int i = I * CT + il;
+    const long S = a.stride, TT = (long)a.T * a.T;
+    int P, Qb;
+    tri_tile(t, a.nbs, P, Qb);
+    if (tid < CS * W) (&emask[0][0])[tid] = 0u;
+    __syncthreads();
+    float4* L0 = a.L + (long)t * (2 * CSS);
+    const int il = tid / TPR, c0 = (tid % TPR) * CPT;
+    const int i = P * CS + il;
     const int ic = min(i, N - 1);
     float zpi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) zpi[k] = a.Zp[3 * ic + k];
     unsigned exm = 0u;
-    float mA[4][3], mB[4][3];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int jl = c0 + c, j = J * CT + jl;
-        bool vm = (i < N) && (j < N);
-        if (I == J) vm = vm && (il < jl);
+#pragma unroll 4
+    for (int c = 0; c < CPT; ++c) {
+        const int jl = c0 + c, j = Qb * CS + jl;
+        const bool vm = (i < N) && (j < N) && (P != Qb || il < jl);
         if (!vm) continue;
-        const long pa = t * (2 * CTT) + (long)il * CT + jl, pb = pa + CTT;
+        const long pa = dense_pair_slot(i, j, a.nb, a.T), pb = pa + TT;
         bool ex = false;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            mA[c][k] = a.X[k * S + pa] + a.U[k * S + pa];
-            mB[c][k] = a.X[k * S + pb] + a.U[k * S + pb];
-            ex = ex || mA[c][k] != zpi[k] || mB[c][k] != a.Zp[3 * j + k];
+            const float mA = a.X[k * S + pa] + a.U[k * S + pa];
+            const float mB = a.X[k * S + pb] + a.U[k * S + pb];
+            ex = ex || mA != zpi[k] || mB != a.Zp[3 * j + k];
         }
-        exm |= ex ? 1u << c : 0u;
+        if (ex) {
+            exm |= 1u << c;
+            atomicOr(&emask[il][jl >> 5], 1u << (jl & 31));
+        }
     }
     int total = 0;
     int slot = block_offset(__popc(exm), wsum, total);
+    while (exm) {
+        const int c = __ffs(exm) - 1;
+        exm &= exm - 1u;
+        const int jl = c0 + c, j = Qb * CS + jl;
+        const long pa = dense_pair_slot(i, j, a.nb, a.T), pb = pa + TT;
+        float mA[3], mB[3];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-        if ((exm >> c) & 1u) {
-            L0[2 * slot] = make_float4(mA[c][0], mA[c][1], mA[c][2], __int_as_float(il * CT + c0 + c));
-            L0[2 * slot + 1] = make_float4(mB[c][0], mB[c][1], mB[c][2], 0.0f);
-            ++slot;
+        for (int k = 0; k < 3; ++k) {
+            mA[k] = a.X[k * S + pa] + a.U[k * S + pa];
+            mB[k] = a.X[k * S + pb] + a.U[k * S + pb];
         }
+        L0[2 * slot] = make_float4(mA[0], mA[1], mA[2], __int_as_float(il * CS + jl));
+        L0[2 * slot + 1] = make_float4(mB[0], mB[1], mB[2], 0.0f);
+        ++slot;
+    }
     if (tid == 0) a.cnt[t] = total;
+    if (tid < CS * W) a.Lmask[(long)t * CS * W + tid] = (&emask[0][0])[tid];
 }
 
 // Leaves the contact state after iteration k (Z_prev = Z_k, Z_prev2 =
-// Z_{k-1}, list cur = L_k, the other = L_{k-1}): every pair edge gets
+// Z_{k-1}, list par = L_k, the other = L_{k-1}): every pair edge gets
 //   U_k = M_{k-1} - Z_k,  X_k = M_k - U_k
-// with M = Z of its circle off the lists: the steady X, U state.
+// with M = Z of its circle off the lists: the steady X, U state.  256
+// threads per contact tile, as k_contact_enter.
+template <int CS>
 __global__ __launch_bounds__(256) void k_contact_leave(CArgs a)
 {
-    __shared__ unsigned short map[2][CTT];
-    __shared__ unsigned mask[2][CT];
+    constexpr int CSS = CS * CS, W = CS / 32, TPR = 256 / CS, CPT = CSS / 256;
+    __shared__ unsigned short map[2][CSS];
+    __shared__ unsigned mask[2][CS][W];
     const int tid = threadIdx.x;
-    const int2 ij0 = a.tiles[blockIdx.x];
-    const int I = ij0.x, J = ij0.y & ~TILE_RAISED;
-    const long t = (long)I * a.nb - (long)I * (I - 1) / 2 + (J - I);
+    const int t = blockIdx.x;
     const int N = a.N;
-    const long S = a.stride;
-    const int par = *a.cur;
-    const float4* Ls[2] = {a.L + ((long)par * a.nt + t) * (2 * CTT), a.L + ((long)(par ^ 1) * a.nt + t) * (2 * CTT)};
-    const int cn[2] = {a.cnt[(long)par * a.nt + t], a.cnt[(long)(par ^ 1) * a.nt + t]};
-    if (tid < CT) mask[0][tid] = mask[1][tid] = 0u;
+    const long S = a.stride, TT = (long)a.T * a.T;
+    int P, Qb;
+    tri_tile(t, a.nbs, P, Qb);
+    const long lt[2] = {(long)a.par * a.nts + t, (long)(a.par ^ 1) * a.nts + t};
+    const float4* Ls[2] = {a.L + lt[0] * (2 * CSS), a.L + lt[1] * (2 * CSS)};
+    const int cn[2] = {min(a.cnt[lt[0]], CSS), min(a.cnt[lt[1]], CSS)};
+    if (tid < 2 * CS * W) (&mask[0][0][0])[tid] = 0u;
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < 2; ++b)
         for (int e = tid; e < cn[b]; e += 256) {
-            const int idx = __float_as_int(Ls[b][2 * e].w);
+            const int idx = __float_as_int(Ls[b][2 * e].w) & (CSS - 1);
             map[b][idx] = (unsigned short)e;
-            atomicOr(&mask[b][idx >> 5], 1u << (idx & 31));
+            atomicOr(&mask[b][idx / CS][(idx % CS) >> 5], 1u << (idx & 31));
         }
     __syncthreads();
-    const int il = tid >> 3, c0 = (tid & 7) * 4;
-    const int i = I * CT + il;
+    const int il = tid / TPR, c0 = (tid % TPR) * CPT;
+    const int i = P * CS + il;
     const int ic = min(i, N - 1);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int jl = c0 + c, j = J * CT + jl;
-        bool vm = (i < N) && (j < N);
-        if (I == J) vm = vm && (il < jl);
+#pragma unroll 2
+    for (int c = 0; c < CPT; ++c) {
+        const int jl = c0 + c, j = Qb * CS + jl;
+        const bool vm = (i < N) && (j < N) && (P != Qb || il < jl);
         if (!vm) continue;
-        const int idx = il * CT + jl;
-        const bool in_k = (mask[0][il] >> jl) & 1u, in_k1 = (mask[1][il] >> jl) & 1u;
+        const int idx = il * CS + jl;
+        const bool in_k = (mask[0][il][jl >> 5] >> (jl & 31)) & 1u;
+        const bool in_k1 = (mask[1][il][jl >> 5] >> (jl & 31)) & 1u;
         float4 ek[2] = {}, ek1[2] = {};
-        if (in_k) { ek[0] = Ls[0][2 * map[0][idx]]; ek[1] = Ls[0][2 * map[0][idx] + 1]; }
-        if (in_k1) { ek1[0] = Ls[1][2 * map[1][idx]]; ek1[1] = Ls[1][2 * map[1][idx] + 1]; }
-        const long pa = t * (2 * CTT) + (long)idx, pb = pa + CTT;
+        if (in_k) {
+            ek[0] = Ls[0][2 * map[0][idx]];
+            ek[1] = Ls[0][2 * map[0][idx] + 1];
+        }
+        if (in_k1) {
+            ek1[0] = Ls[1][2 * map[1][idx]];
+            ek1[1] = Ls[1][2 * map[1][idx] + 1];
+        }
+        const long pa = dense_pair_slot(i, j, a.nb, a.T), pb = pa + TT;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const int v = side == 0 ? ic : j;
@@ -933,10 +1027,11 @@ static CArgs cargs(DevGraph* s)
     DenseState* d = s->dense;
     CArgs a;
     a.N = d->N;
+    a.nbs = d->nbs;
+    a.nts = d->nts;
     a.nb = d->nb;
-    a.nt = d->nt;
+    a.T = d->T;
     a.stride = s->stride;
-    a.tiles = d->d_tiles;
     a.X = s->d_X;
     a.U = s->d_U;
     a.Z = s->d_Z;
@@ -944,8 +1039,9 @@ static CArgs cargs(DevGraph* s)
     a.Zpp = d->d_Zpp;
     a.aexp = d->d_aexp;
     a.uni = d->d_uni;
-    a.cur = d->d_cur;
+    a.par = d->c_par;
     a.L = d->d_L;
+    a.Lmask = d->d_Lmask;
     a.cnt = d->d_cnt;
     a.Pc = d->d_Pc;
     a.Rpc = d->d_Rpc;
@@ -956,7 +1052,7 @@ static CArgs cargs(DevGraph* s)
 bool dense_contact_ok(DevGraph* s)
 {
     const DenseState* d = s->dense;
-    if (!d || d->T != CT) return false;
+    if (!d) return false;
     if (!(s->rho_uni && s->alpha_uni && s->alpha_u == 1.0f && s->policy_raised == 0)) return false;
     const char* e = getenv("PARADMM_CONTACT");
     return !(e && e[0] == '0');
@@ -964,14 +1060,33 @@ bool dense_contact_ok(DevGraph* s)
 
 bool dense_in_contact(DevGraph* s) { return s->dense && s->dense->c_state; }
 
+// Contact tile side: 64 (a quarter of the per-pair overhead of 32) once
+// the tiles are enough to fill the GPU, 32 below N = 4000 (the smaller
+// tiles spread a smaller packing over more wavefronts); PARADMM_CONTACT_TILE
+// = 32 or 64 overrides.
+static int contact_tile_side(int N)
+{
+    if (const char* e = getenv("PARADMM_CONTACT_TILE")) {
+        const int v = atoi(e);
+        if (v == 32 || v == 64) return v;
+    }
+    return N >= 4000 ? 64 : 32;
+}
+
 static int contact_alloc(DevGraph* s)
 {
     DenseState* d = s->dense;
     if (d->d_L) return PARADMM_OK;
-    RC(dmalloc(s, (void**)&d->d_L, sizeof(float4) * 2 * 2 * (size_t)d->nt * CTT));
-    RC(dmalloc(s, (void**)&d->d_cnt, sizeof(int) * 2 * (size_t)d->nt));
-    RC(dmalloc(s, (void**)&d->d_cur, sizeof(int)));
-    RC(dmalloc(s, (void**)&d->d_Pc, sizeof(float) * CQ * (size_t)d->nb * d->N));
+    d->cs = contact_tile_side(d->N);
+    d->nbs = (d->N + d->cs - 1) / d->cs;
+    d->nts = d->nbs * (d->nbs + 1) / 2;
+    const size_t nts = (size_t)d->nts, css = (size_t)d->cs * d->cs, words = (size_t)d->cs * (d->cs / 32);
+    RC(dmalloc(s, (void**)&d->d_L, sizeof(float4) * 2 * 2 * nts * css));
+    RC(dmalloc(s, (void**)&d->d_cnt, sizeof(int) * 2 * nts));
+    RC(dmalloc(s, (void**)&d->d_Lmask, sizeof(unsigned) * 2 * nts * words));
+    CK(hipMemsetAsync(d->d_cnt, 0, sizeof(int) * 2 * nts, s->stream));
+    CK(hipMemsetAsync(d->d_Lmask, 0, sizeof(unsigned) * 2 * nts * words, s->stream));
+    RC(dmalloc(s, (void**)&d->d_Pc, sizeof(float) * CQ * (size_t)d->nbs * d->N));
     RC(dmalloc(s, (void**)&d->d_Rpc, sizeof(double) * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_Zpp, sizeof(float) * 3 * (size_t)d->N));
     return PARADMM_OK;
@@ -982,10 +1097,15 @@ static int contact_iteration(DevGraph* s, bool res)
 {
     DenseState* d = s->dense;
     const CArgs a = cargs(s);
-    const unsigned tiles_grid = (unsigned)((d->nt + 3) / 4);
+    const unsigned g = (unsigned)d->nts;
     prof_begin(s, res ? "contact_tile_res" : "contact_tile");
-    if (res) k_contact_tile<true><<<tiles_grid, 256, 0, s->stream>>>(a);
-    else k_contact_tile<false><<<tiles_grid, 256, 0, s->stream>>>(a);
+    if (d->cs == 64) {
+        if (res) k_contact_tile<64, true><<<g, 64, 0, s->stream>>>(a);
+        else k_contact_tile<64, false><<<g, 64, 0, s->stream>>>(a);
+    } else {
+        if (res) k_contact_tile<32, true><<<g, 64, 0, s->stream>>>(a);
+        else k_contact_tile<32, false><<<g, 64, 0, s->stream>>>(a);
+    }
     prof_end(s);
     CK(hipGetLastError());
     prof_begin(s, res ? "contact_zreduce_res" : "contact_zreduce");
@@ -994,7 +1114,15 @@ static int contact_iteration(DevGraph* s, bool res)
     else k_contact_zreduce<false><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a);
     prof_end(s);
     CK(hipGetLastError());
+    d->c_par ^= 1;
     return PARADMM_OK;
+}
+
+int dense_contact_parity(DevGraph* s) { return s->dense ? s->dense->c_par : 0; }
+
+void dense_contact_set_parity(DevGraph* s, int p)
+{
+    if (s->dense) s->dense->c_par = p & 1;
 }
 
 int dense_contact_enter(DevGraph* s)
@@ -1007,10 +1135,11 @@ int dense_contact_enter(DevGraph* s)
         return PARADMM_ERR_STATE;
     }
     RC(contact_alloc(s));
-    CK(hipMemsetAsync(d->d_cur, 0, sizeof(int), s->stream));
+    d->c_par = 0;  // the entry writes list 0
     const CArgs a = cargs(s);
     prof_begin(s, "contact_enter");
-    k_contact_enter<<<(unsigned)d->nt, 256, 0, s->stream>>>(a);
+    if (d->cs == 64) k_contact_enter<64><<<(unsigned)d->nts, 256, 0, s->stream>>>(a);
+    else k_contact_enter<32><<<(unsigned)d->nts, 256, 0, s->stream>>>(a);
     prof_end(s);
     CK(hipGetLastError());
     d->c_state = true;
@@ -1023,7 +1152,8 @@ int dense_contact_leave(DevGraph* s)
     if (!d || !d->c_state) return PARADMM_OK;
     const CArgs a = cargs(s);
     prof_begin(s, "contact_leave");
-    k_contact_leave<<<(unsigned)d->nt, 256, 0, s->stream>>>(a);
+    if (d->cs == 64) k_contact_leave<64><<<(unsigned)d->nts, 256, 0, s->stream>>>(a);
+    else k_contact_leave<32><<<(unsigned)d->nts, 256, 0, s->stream>>>(a);
     prof_end(s);
     CK(hipGetLastError());
     d->c_state = false;

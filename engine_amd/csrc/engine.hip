@@ -1066,11 +1066,20 @@ static int fused_iteration(DevGraph* s, int from_n, int mphase = DENSE_PLAIN)
 }
 
 // kernel variant a captured run depends on: the uniform rho/alpha kernels,
-// the M state (dense, alpha = 1) and the contact state
+// the M state (dense, alpha = 1) and the contact state with its start parity
+// (RUN_PARITY: runs that differ only there are both kept)
+constexpr int RUN_PARITY = 8;
+
+static bool in_contact(const DevGraph* s)
+{
+    return s->layout == 2 && dense_in_contact(const_cast<DevGraph*>(s));
+}
+
 static int run_variant(const DevGraph* s)
 {
-    const bool contact = s->layout == 2 && dense_in_contact(const_cast<DevGraph*>(s));
-    return ((s->rho_uni && s->alpha_uni) ? 1 : 0) | (mrun_ok(s) ? 2 : 0) | (contact ? 4 : 0);
+    const bool contact = in_contact(s);
+    const int par = contact && dense_contact_parity(const_cast<DevGraph*>(s)) ? RUN_PARITY : 0;
+    return ((s->rho_uni && s->alpha_uni) ? 1 : 0) | (mrun_ok(s) ? 2 : 0) | (contact ? 4 : 0) | par;
 }
 
 // The captured run of n iterations for the current variant (recorded on
@@ -1081,11 +1090,11 @@ static int captured_run(DevGraph* s, int n, hipGraphExec_t* out)
 {
     const int variant = run_variant(s);
     for (auto it = s->runs.begin(); it != s->runs.end();) {
-        if (it->variant != variant) {
+        if ((it->variant | RUN_PARITY) != (variant | RUN_PARITY)) {
             CK(hipGraphExecDestroy(it->exec));
             CK(hipGraphDestroy(it->graph));
             it = s->runs.erase(it);
-        } else if (it->iters == n) {
+        } else if (it->iters == n && it->variant == variant) {
             *out = it->exec;
             return PARADMM_OK;
         } else {
@@ -1102,11 +1111,15 @@ static int captured_run(DevGraph* s, int n, hipGraphExec_t* out)
     DevGraph::CapturedRun r;
     r.iters = n;
     r.variant = variant;
+    // recording runs nothing: the contact parity is advanced when the run is launched
+    const bool contact = in_contact(s);
+    const int par0 = contact ? dense_contact_parity(s) : 0;
     CK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = PARADMM_OK;
     for (int i = 0; i < n && !rc; ++i) rc = fused_iteration(s, 0, mphase_of(s, i, n));
     hipGraph_t gr = nullptr;
     hipError_t e = hipStreamEndCapture(s->stream, &gr);
+    if (contact) dense_contact_set_parity(s, par0);
     if (rc) {
         if (gr) (void)hipGraphDestroy(gr);
         return rc;
@@ -1159,16 +1172,19 @@ static int run_fused(DevGraph* s, int iters)
     if (!s->host && iters >= 2) {
         const int GI = s->graph_iters;
         hipGraphExec_t ex = nullptr;
+        const bool contact = in_contact(s);
         if (iters >= GI) {
             RC(captured_run(s, GI, &ex));
             while (iters >= GI) {
                 CK(hipGraphLaunch(ex, s->stream));
+                if (contact) dense_contact_set_parity(s, dense_contact_parity(s) ^ (GI & 1));
                 iters -= GI;
             }
         }
         if (iters >= 2) {
             RC(captured_run(s, iters, &ex));
             CK(hipGraphLaunch(ex, s->stream));
+            if (contact) dense_contact_set_parity(s, dense_contact_parity(s) ^ (iters & 1));
             iters = 0;
         }
     }

@@ -198,5 +198,10 @@ bool dense_in_contact(DevGraph* s);
 int dense_contact_enter(DevGraph* s);
 // back to the steady X, U state (no-op outside the contact state)
 int dense_contact_leave(DevGraph* s);
+// which contact list the last iteration wrote: contact kernels take it as an
+// argument, so a captured run is bound to its start parity and leaves the
+// parity flipped by its length
+int dense_contact_parity(DevGraph* s);
+void dense_contact_set_parity(DevGraph* s, int p);
 
 }  // namespace pdm

@@ -25,6 +25,7 @@ __device__ __forceinline__ float pow_exp(float r, float e)
 // -r^(a-1) + (rho/a)(r - n) = 0 from |n/2|; a negative iterate returns 0.
 __device__ __forceinline__ float max_alpha_radius(float n, float rho, float a)
 {
+#pragma clang fp contract(off)
     float r = fabsf(n * 0.5f);
 #pragma unroll 1
     for (int it = 0; it < 10; ++it) {
@@ -38,11 +39,16 @@ __device__ __forceinline__ float max_alpha_radius(float n, float rho, float a)
     return r;
 }
 
+// The operators are evaluated without contracting a * b + c into an FMA:
+// every inlined copy (the contact kernel tests a pair in one place and
+// projects it in another) then rounds the same way.
+//
 // the collision test of pair_project: |c1 - c2|^2 < (r1 + r2)^2
 __device__ __forceinline__ bool pair_hits(float c1x, float c1y, float r1, float c2x, float c2y, float r2)
 {
+#pragma clang fp contract(off)
     const float dx = c1x - c2x, dy = c1y - c2y;
-    const float d2 = dx * dx + dy * dy;
+    const float d2 = fmaf(dx, dx, dy * dy);
     const float s = r1 + r2;
     return d2 < s * s;
 }
@@ -59,6 +65,7 @@ __device__ __forceinline__ bool pair_project(float c1x, float c1y, float r1, flo
                                              float& o1x, float& o1y, float& or1,
                                              float& o2x, float& o2y, float& or2)
 {
+#pragma clang fp contract(off)
     if (!pair_hits(c1x, c1y, r1, c2x, c2y, r2)) return false;
     const float dx = c1x - c2x, dy = c1y - c2y;
     const float d2 = dx * dx + dy * dy;

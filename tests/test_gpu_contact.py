@@ -4,7 +4,7 @@ oracle, the reference's golden trajectories and the per-edge (M state)
 path, through the C ABI.
 
 Sizes cover one diagonal tile (N=20), exact tiles (N=64, 128) and ragged
-edge tiles (N=70, 150).
+edge tiles (N=70, 150), on contact tiles of side 32 and 64.
 """
 import os
 
@@ -19,11 +19,14 @@ pytestmark = pytest.mark.gpu
 STATE = ("X", "M", "N", "U", "Z")
 
 
-@pytest.fixture(autouse=True)
-def contact_on(monkeypatch):
+@pytest.fixture(autouse=True, params=["32", "64"])
+def contact_on(monkeypatch, request):
+    """Every test on both contact tile sides (the default picks 32 below
+    N = 4000)."""
     monkeypatch.delenv("PARADMM_LAYOUT", raising=False)
     monkeypatch.delenv("PARADMM_CONTACT", raising=False)
     monkeypatch.delenv("PARADMM_DENSE_TILE", raising=False)
+    monkeypatch.setenv("PARADMM_CONTACT_TILE", request.param)
 
 
 def _kernels(g, iters):
