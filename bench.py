@@ -7,7 +7,10 @@ factors, 5000 box and 5000 radius factors, E = 25,005,000 edges of dim 3,
 fp32 messages, fixed rho = 1, alpha = 1, seeded synthetic init.  Inputs are
 resident in HBM before the timed region; the timed region is K iterations
 launched through the engine's captured hipGraph and bracketed by HIP events
-on the engine's stream (the library synchronises before and after).
+on the engine's stream (the library synchronises before and after).  With
+one rho and alpha = 1 the engine runs the dense layout's contact state (only
+pairs whose prox is active keep per-edge values; every edge is still updated
+every iteration).
 
 Prints one JSON line: value = edge-updates/s, plus the roofline of the
 dominant kernel (algorithmic bytes / mean launch time from HIP events) and a
@@ -39,7 +42,9 @@ EDGE_UPDATE_BYTES = 72
 #   the same pass inside a captured run with alpha = 1, the X array
 #   holding M = X + U (U not stored): read M, rho; write M             = 28
 # and with one rho and one alpha on every edge (the variant without those
-# streams) 48 and 24.
+# streams) 48 and 24.  The contact-state tile kernel (one rho, alpha = 1)
+# keeps per-edge values only for the pairs whose prox is active: its bytes
+# are not a per-edge constant (the measured traffic is reported instead).
 BYTES_PER_EDGE = {
     "disk_collision_3d": 28, "box_3d": 28, "max_radius_3d": 28,
     "zpass_fused": 44,
@@ -48,8 +53,12 @@ BYTES_PER_EDGE = {
 UNIFORM_BYTES_PER_EDGE = {"pairtile_fused": 48, "pairtile_m": 24}
 # kernels that carry a whole iteration's edge work (their launches count
 # E edge-updates each)
-FULL_ITERATION_KERNELS = ("pairtile_fused", "pairtile_m")
+FULL_ITERATION_KERNELS = ("pairtile_fused", "pairtile_m", "contact_tile")
 TRAFFIC_PROFILES = {k: os.path.join(ROOT, "profiles", f"{k}_n5000_hbm_traffic.md") for k in FULL_ITERATION_KERNELS}
+VALU_PROFILES = {"contact_tile": os.path.join(ROOT, "profiles", "contact_tile_n5000_valu.md")}
+# MI355X vector issue: 256 CUs x 4 SIMDs, one wave64 VALU instruction per
+# SIMD every 4 cycles at 2.4 GHz
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 
 
 def edges_per_launch(name, n):
@@ -60,17 +69,58 @@ def edges_per_launch(name, n):
     }.get(name, 0)
 
 
-def measured_traffic(kernel, n, seed):
-    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary (copy-kernel calibrated), when it covers this kernel and workload."""
-    path = TRAFFIC_PROFILES.get(kernel)
+def _profile_value(path, key, n, seed):
     if path is None or n != 5000 or seed != 0 or not os.path.exists(path):
         return None
     with open(path) as fh:
         for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return int(line.split(":")[1])
     return None
+
+
+def measured_traffic(kernel, n, seed):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (copy-kernel calibrated), when it covers this kernel and workload."""
+    return _profile_value(TRAFFIC_PROFILES.get(kernel), "traffic_bytes_per_launch", n, seed)
+
+
+def measured_valu(kernel, n, seed):
+    """VALU wave-instructions per launch from the committed rocprofv3
+    SQ_INSTS_VALU summary, when it covers this kernel and workload."""
+    return _profile_value(VALU_PROFILES.get(kernel), "valu_insts_per_launch", n, seed)
+
+
+def roofline(dom, dom_ms, dom_bytes, kernel_bytes, achieved, traffic, valu, copy_gbs, contact):
+    r = {
+        "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "bytes_per_launch": dom_bytes, "bytes_per_edge_update": EDGE_UPDATE_BYTES,
+        "traffic_gbs": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None,
+        "traffic_frac_of_copy": (traffic / (dom_ms * 1e-3) / 1e9 / copy_gbs) if traffic else None,
+        "mean_launch_ms": dom_ms, "copy_kernel_gbs": copy_gbs,
+    }
+    if contact:
+        # the contact state stores per-edge values only for active pairs:
+        # the kernel moves traffic << 72 B per edge-update and is bound by
+        # its VALU issue (the pair tests) and memory latency
+        r["note"] = ("frac counts 72 B per edge-update, the per-edge state an X pass and a Z pass move; the "
+                     "contact state keeps per-edge values only for pairs whose prox is active, so the kernel "
+                     "moves `traffic` bytes per launch and is bound by VALU issue (valu.frac) and latency")
+        if valu:
+            rate = valu / (dom_ms * 1e-3)
+            r["valu"] = {"insts_per_launch": valu, "achieved": rate, "peak": VALU_PEAK_WAVE_INSTS,
+                         "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK_WAVE_INSTS}
+    else:
+        r["kernel_min_bytes_per_launch"] = kernel_bytes
+        r["kernel_min_gbs"] = kernel_bytes / (dom_ms * 1e-3) / 1e9
+        # frac above uses the 72 B/edge-update unit (an X pass and a Z pass);
+        # the fused kernel needs only kernel_min bytes per edge
+        r["kernel_min_frac"] = kernel_bytes / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+        r["note"] = ("frac counts 72 B per edge-update (X and U read and written by an X pass and a Z pass); "
+                     "the dominant kernel moves kernel_min bytes per edge (M alone inside captured runs), so "
+                     "kernel_min_frac and traffic_frac_of_copy are the fractions that bound it")
+    return r
 
 
 def cpu_baseline(n, iters, threads):
@@ -244,6 +294,8 @@ def main():
     kernel_bytes = UNIFORM_BYTES_PER_EDGE.get(dom, BYTES_PER_EDGE.get(dom, 0)) * dom_edges
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(dom, args.n, args.seed)
+    valu = measured_valu(dom, args.n, args.seed)
+    contact = dom == "contact_tile"
     print(f"# kernels (mean ms per launch): " + ", ".join(f"{k} {v[0]:.4f}x{v[1]}" for k, v in prof.items()),
           file=sys.stderr, flush=True)
 
@@ -287,24 +339,9 @@ def main():
             "workload": f"plain ADMM unit-square circle packing, N={args.n} circles, E={E} edges of dim 3, "
                         f"fixed rho=1, alpha=1, radius exponent 1, seed {args.seed}",
             "circles": args.n, "edges": E, "layout": layout,
+            "state": "contact" if contact else "per-edge",
         },
-        "roofline": {
-            "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "bytes_per_launch": dom_bytes, "bytes_per_edge_update": EDGE_UPDATE_BYTES,
-            "kernel_min_bytes_per_launch": kernel_bytes,
-            "kernel_min_gbs": kernel_bytes / (dom_ms * 1e-3) / 1e9,
-            # frac above uses the 72 B/edge-update unit (an X pass and a Z
-            # pass); the fused kernel needs only kernel_min bytes per edge,
-            # so these are the fractions that bound it
-            "kernel_min_frac": kernel_bytes / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "note": ("frac counts 72 B per edge-update (X and U read and written by an X pass and a Z pass); "
-                     "the dominant kernel moves kernel_min bytes per edge (M alone inside captured runs), so "
-                     "kernel_min_frac and traffic_frac_of_copy are the fractions that bound it"),
-            "traffic_gbs": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None,
-            "traffic_frac_of_copy": (traffic / (dom_ms * 1e-3) / 1e9 / copy_gbs) if traffic else None,
-            "mean_launch_ms": dom_ms, "copy_kernel_gbs": copy_gbs,
-        },
+        "roofline": roofline(dom, dom_ms, dom_bytes, kernel_bytes, achieved, traffic, valu, copy_gbs, contact),
         "kernels_ms": {k: v[0] for k, v in prof.items()},
         "residuals_last": {"primal2": p2, "dual2": d2},
     }
