@@ -97,8 +97,10 @@ def test_batch_equals_single_instance_solver(monkeypatch, iters, sizes):
     and not at all with two blocks (N <= 128): then 40 iterations agree bit
     for bit.  (With three blocks the 4e-7 of the first iterations grows to
     about 6e-4 by iteration 40, as any rounding difference does early in a
-    packing.)"""
+    packing.)  The single-instance solver runs its per-edge path: the
+    contact state, its default for one rho, rounds differently."""
     monkeypatch.setenv("PARADMM_DENSE_TILE", "64")
+    monkeypatch.setenv("PARADMM_CONTACT", "0")
     gs = _instances(sizes, 10.0)
     b = P.Batch(gs)
     b.configure(64, 2)
@@ -113,7 +115,7 @@ def test_batch_equals_single_instance_solver(monkeypatch, iters, sizes):
         assert err < 1e-6 if iters == 3 else err == 0.0
 
 
-def test_batch_retires_instances_independently():
+def test_batch_retires_instances_independently(monkeypatch):
     """Instances converge at different iterations; each retires at its own
     check and keeps its solution; the rest run to the cap."""
     specs = [(20, 0, 100.0), (20, 1, 100.0), (24, 2, 100.0), (40, 3, 1.0), (20, 4, 100.0)]
@@ -149,7 +151,9 @@ def test_batch_retires_instances_independently():
     assert res[3]["elapsed_ms"] >= max(el)
     # the single-instance solver converges too (fp32 rounding of the two
     # paths differs, and on this nonconvex problem may lead to another
-    # local optimum, so only the verdicts are compared)
+    # local optimum, so only the verdicts are compared); its per-edge path,
+    # the one the batch shares
+    monkeypatch.setenv("PARADMM_CONTACT", "0")
     for k in conv[:2]:
         n, s, r = specs[k]
         g1 = P.packing_square(n, s, rho=r)
@@ -231,9 +235,11 @@ def test_batch_rho_policy_matches_single_instance(monkeypatch, mode, average):
     from the network's mean estimate, in mode 3 plus the edges the network
     raises) follows the single-instance solver with the same policy and mode
     (both on tile side 64, so the two differ only in the order of the
-    per-circle block sums)."""
+    per-circle block sums; the single-instance solver on its per-edge path,
+    the one the batch shares, not the contact state)."""
     from engine_amd import rho_train as rt
     monkeypatch.setenv("PARADMM_DENSE_TILE", "64")
+    monkeypatch.setenv("PARADMM_CONTACT", "0")
     m = rt.load_policy()
     specs = [(64, 3), (120, 5)]
     gs = [P.packing_square(n, s, rho=rt.LEARNED_START_RHO) for n, s in specs]
