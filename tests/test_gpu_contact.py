@@ -182,3 +182,25 @@ def test_contact_solve_matches_per_edge_solve(monkeypatch):
     assert res["1"]["converged"] and res["0"]["converged"]
     assert res["1"]["max_overlap"] <= 1e-4 and res["1"]["max_box"] <= 1e-4
     assert res["1"]["objective"] == pytest.approx(res["0"]["objective"], rel=2e-2)
+
+
+def test_contact_on_dense_tile_64(monkeypatch):
+    """The contact state over the dense layout of side 64 (entry, exit and
+    the box/radius slots map to that layout): the reference's golden
+    trajectory, then leaving and re-entering against a run that stays."""
+    monkeypatch.setenv("PARADMM_DENSE_TILE", "64")
+    f = np.load(os.path.join(GOLDEN, "square20_seed0.npz"), allow_pickle=False)
+    g, _ = square_pair(int(f["n"]), int(f["seed"]), float(f["rho"]), float(f["alpha"]), float(f["a"]))
+    hist = np.zeros_like(f["z"])
+    for it in range(hist.shape[0]):
+        g.iterate(1)
+        hist[it] = g.read("Z").ravel()
+    assert "contact_tile" in _kernels(g, 2)
+    assert trajectory_error(hist, f["z"]).max() < 1e-5
+    ga, _ = square_pair(150, 5, rho=2.0)
+    gb, _ = square_pair(150, 5, rho=2.0)
+    ga.iterate(60)
+    gb.iterate(30)
+    gb.sync_to_host()
+    gb.iterate(30)
+    assert max_rel(gb.read("Z"), ga.read("Z")) < 1e-5
