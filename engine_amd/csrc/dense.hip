@@ -68,6 +68,8 @@ struct DenseState {
     int cs = 64;               // contact tile side (32 or 64)
     int nbs = 0, nts = 0;      // contact tiling: blocks of cs circles, tiles
     int c_par = 0;             // the list the last iteration wrote (kept on the host, passed to the kernels)
+    bool c_rhoe = false;       // rho differs between edges (fixed while in the state)
+    float* d_W = nullptr;      // [2][N] sum rho, sum rho^2 per circle (per-edge rho)
     unsigned* d_Lmask = nullptr; // [2][nts][cs][cs/32] the pairs of each list, per tile row
     float* d_Pc = nullptr;     // [CQ][nbs][N] circle partials of the contact iteration
     double* d_Rpc = nullptr;   // [N] primal residual per circle
@@ -213,6 +215,8 @@ struct CArgs {
     float* Zpp;
     const float* aexp;
     const float* uni;      // {rho, alpha}
+    const float* rho;      // per-edge rho (dense slots), read when rho differs between edges
+    float* W;              // [2][N] sum rho, sum rho^2 over each circle's edges (per-edge rho)
     int par;               // the list the last iteration wrote
     float4* L;             // [2][nts][CS*CS][2]
     unsigned* Lmask;       // [2][nts][CS][CS/32] pairs of each list, per tile row
@@ -307,9 +311,9 @@ struct ContactWave {
 // One pair of the contact tile through the prox: N from its list entry (ex)
 // or the circles' common N, the circle sums; returns whether the prox was
 // active, with the new M of both edges.
-template <int CS, int Q>
+template <int CS, int Q, bool RE>
 __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bool ex, float4 pa, float4 pb,
-                                             bool diag, float rho, float (&mA)[3], float (&mB)[3])
+                                             bool diag, float rA, float rB, float (&mA)[3], float (&mB)[3])
 {
     const int rl = idx / CS, cl = idx % CS;
     const int cs = diag ? cl : CS + cl;
@@ -340,7 +344,7 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
         }
     }
     float xA[3] = {nA[0], nA[1], nA[2]}, xB[3] = {nB[0], nB[1], nB[2]};
-    const bool act = pair_project(nA[0], nA[1], nA[2], nB[0], nB[1], nB[2], rho, rho, rho, rho, xA[0], xA[1], xA[2],
+    const bool act = pair_project(nA[0], nA[1], nA[2], nB[0], nB[1], nB[2], rA, rA, rB, rB, xA[0], xA[1], xA[2],
                                   xB[0], xB[1], xB[2]);
     if constexpr (Q == CQ) if (act || ex) {  // g = 0 on the other edges
         float gA2 = 0.0f, gB2 = 0.0f;
@@ -361,8 +365,9 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
         for (int k = 0; k < 3; ++k) {
             mA[k] = xA[k] + uA[k];
             mB[k] = xB[k] + uB[k];
-            cadd(&sh.acc[rl][k], mA[k] - zA[k]);
-            cadd(&sh.acc[cs][k], mB[k] - zB[k]);
+            // per-edge rho: the consensus weighs each deviation with its edge's rho
+            cadd(&sh.acc[rl][k], RE ? rA * (mA[k] - zA[k]) : mA[k] - zA[k]);
+            cadd(&sh.acc[cs][k], RE ? rB * (mB[k] - zB[k]) : mB[k] - zB[k]);
         }
     }
     return act;
@@ -371,9 +376,10 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
 // One round of the tile's pair stream: items d .. d + 63 (item < cp: entry
 // of the previous list, else ring slot item - cp); the active pairs are
 // appended to the new list in lane order.
-template <int CS, int Q>
-__device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, int d, int avail, int cp, const float4* Lp,
-                                              float4 pre0, float4 pre1, float4* Ln, int& base, bool diag, float rho)
+template <int CS, int Q, bool RE>
+__device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArgs& a, int P, int Qb, int d, int avail,
+                                              int cp, const float4* Lp, float4 pre0, float4 pre1, float4* Ln,
+                                              int& base, bool diag, float rho)
 {
     constexpr int CSS = CS * CS;
     const int lane = threadIdx.x;
@@ -393,7 +399,13 @@ __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, int d, int
         } else {
             idx = sh.ring[(item - cp) & (CRING - 1)];
         }
-        act = contact_pair(sh, idx, ex, pa, pb, diag, rho, mA, mB);
+        float rA = rho, rB = rho;
+        if constexpr (RE) {  // the pair's own rho from its dense slots
+            const long q = dense_pair_slot(P * CS + idx / CS, Qb * CS + idx % CS, a.nb, a.T);
+            rA = a.rho[q];
+            rB = a.rho[q + (long)a.T * a.T];
+        }
+        act = contact_pair<CS, Q, RE>(sh, idx, ex, pa, pb, diag, rA, rB, mA, mB);
     }
     const unsigned long long bal = __ballot(act);
     if (act) {
@@ -413,7 +425,9 @@ __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, int d, int
 // order, then pair order); whenever 64 items are pending they go through
 // the prox as one round.  The new list is written in stream order.  RES:
 // also the residual sums (the last iteration before residuals are read).
-template <int CS, bool RES>
+// RE: rho differs between edges; the pairs taken through the prox read
+// theirs from the dense slots (inactive pairs need none).
+template <int CS, bool RES, bool RE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_contact_tile(CArgs a)
 {
     constexpr int Q = RES ? CQ : 3;
@@ -465,7 +479,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     int done = 0;           // stream items taken through the prox
     int avail = cp;         // stream items so far (previous list + ring)
     while (avail - done >= 64) {
-        contact_round(sh, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
+        contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
         done += 64;
     }
 #pragma unroll 1
@@ -515,11 +529,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         wave_lds_fence();
         while (avail - done >= 64) {
-            contact_round(sh, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
+            contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
             done += 64;
         }
     }
-    if (avail > done) contact_round(sh, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
+    if (avail > done) contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
     if (lane == 0) a.cnt[tn] = base;
 
     // the diagonal tile's box and radius edges: the plain U update, prox
@@ -540,6 +554,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const long q = p0 + side * TT;
+            const float r = RE ? a.rho[q] : rho;
             float x[3], u[3], nn[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -552,13 +567,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             } else {
                 x[0] = nn[0];
                 x[1] = nn[1];
-                x[2] = max_alpha_radius(nn[2], rho, a.aexp[cc]);
+                x[2] = max_alpha_radius(nn[2], r, a.aexp[cc]);
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 a.X[k * S + q] = x[k];
                 a.U[k * S + q] = u[k];
-                cadd(&sh.acc[e][k], (x[k] + u[k]) - z[k]);
+                cadd(&sh.acc[e][k], RE ? r * ((x[k] + u[k]) - z[k]) : (x[k] + u[k]) - z[k]);
                 if constexpr (RES) {
                     const float g = (x[k] - z[k]) - cz[k];
                     cadd(&sh.acc[e][3 + k], g);
@@ -582,12 +597,46 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 }
 
 // Z_{k+1} = Z_k + sum a / (N + 1) over the circle's nbs block partials in
-// fixed order, the Z history shift and, RES, the per-circle primal and dual
-// residuals.  Workgroup = CZ_CIRC circles (128-byte rows of the partials) x
-// CZ_SLICES slices of the blocks, added by a fixed-shape tree.
+// fixed order (RE, rho per edge: Z_k + sum rho a / sum rho), the Z history
+// shift and, RES, the per-circle primal and dual residuals.  Workgroup =
+// CZ_CIRC circles (128-byte rows of the partials) x CZ_SLICES slices of the
+// blocks, added by a fixed-shape tree.
 constexpr int CZ_CIRC = 32, CZ_SLICES = 32;
 
-template <bool RES>
+// Per-edge rho: W[c] = sum rho and W[N + c] = sum rho^2 over circle c's N + 1
+// edges, FP64 in a fixed order (lane j's pairs j, j + 64, ..., then the
+// wavefront's butterfly).  One wavefront per circle, when the state is
+// entered (rho does not change inside it).
+__global__ __launch_bounds__(64) void k_contact_weights(CArgs a)
+{
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const int N = a.N;
+    const long TT = (long)a.T * a.T;
+    double s = 0.0, s2 = 0.0;
+    for (int j = lane; j < N; j += 64) {
+        if (j == c) continue;
+        const long q = j > c ? dense_pair_slot(c, j, a.nb, a.T) : dense_pair_slot(j, c, a.nb, a.T) + TT;
+        const double r = a.rho[q];
+        s += r;
+        s2 += r * r;
+    }
+    if (lane < 2) {  // the box and radius edges
+        const double r = a.rho[dense_unary_slot(c, a.nb, a.T) + lane * TT];
+        s += r;
+        s2 += r * r;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    if (lane == 0) {
+        a.W[c] = (float)s;
+        a.W[N + c] = (float)s2;
+    }
+}
+
+template <bool RES, bool RE>
 __global__ __launch_bounds__(CZ_CIRC * CZ_SLICES) void k_contact_zreduce(CArgs a)
 {
     constexpr int Q = RES ? CQ : 3;
@@ -619,7 +668,7 @@ __global__ __launch_bounds__(CZ_CIRC * CZ_SLICES) void k_contact_zreduce(CArgs a
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float zk = a.Z[3 * c + k], zp = a.Zp[3 * c + k];
-        const float zn = zk + part[0][k][cl] / deg;
+        const float zn = zk + part[0][k][cl] / (RE ? a.W[c] : deg);
         if constexpr (RES) {
             const float dz = zn - zk;
             const double e = (double)(dz - (zk - zp));
@@ -633,7 +682,7 @@ __global__ __launch_bounds__(CZ_CIRC * CZ_SLICES) void k_contact_zreduce(CArgs a
     if constexpr (RES) {
         const float rho = a.uni[0];
         a.Rpc[c] = p;
-        a.Rd[c] = (double)rho * rho * (double)deg * dz2;
+        a.Rd[c] = RE ? (double)a.W[a.N + c] * dz2 : (double)rho * rho * (double)deg * dz2;
     }
 }
 
@@ -641,11 +690,12 @@ __global__ __launch_bounds__(CZ_CIRC * CZ_SLICES) void k_contact_zreduce(CArgs a
 // Z = Z_k, Z_prev = Z_{k-1}): list 0 of contact tile blockIdx.x receives
 // every pair with M = X + U != Z_{k-1} on either edge, exactly, so the first
 // contact iteration computes the same U = M - Z_k as the fused one.  256
-// threads; thread: row tid / (256 / CS), CS^2 / 256 consecutive columns.
+// threads; thread: column tid % CS of rows tid / CS, + 256 / CS, ... (a
+// wavefront reads consecutive slots of one row of the dense tiles).
 template <int CS>
 __global__ __launch_bounds__(256) void k_contact_enter(CArgs a)
 {
-    constexpr int CSS = CS * CS, W = CS / 32, TPR = 256 / CS, CPT = CSS / 256;
+    constexpr int CSS = CS * CS, W = CS / 32, RS = 256 / CS, CPT = CSS / 256;
     __shared__ int wsum[4];
     __shared__ unsigned emask[CS][W];
     const int tid = threadIdx.x;
@@ -657,16 +707,16 @@ __global__ __launch_bounds__(256) void k_contact_enter(CArgs a)
     if (tid < CS * W) (&emask[0][0])[tid] = 0u;
     __syncthreads();
     float4* L0 = a.L + (long)t * (2 * CSS);
-    const int il = tid / TPR, c0 = (tid % TPR) * CPT;
-    const int i = P * CS + il;
-    const int ic = min(i, N - 1);
-    float zpi[3];
+    const int jl = tid % CS, r0 = tid / CS;
+    const int j = Qb * CS + jl;
+    const int jc = min(j, N - 1);
+    float zpj[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) zpi[k] = a.Zp[3 * ic + k];
+    for (int k = 0; k < 3; ++k) zpj[k] = a.Zp[3 * jc + k];
     unsigned exm = 0u;
 #pragma unroll 4
     for (int c = 0; c < CPT; ++c) {
-        const int jl = c0 + c, j = Qb * CS + jl;
+        const int il = r0 + c * RS, i = P * CS + il;
         const bool vm = (i < N) && (j < N) && (P != Qb || il < jl);
         if (!vm) continue;
         const long pa = dense_pair_slot(i, j, a.nb, a.T), pb = pa + TT;
@@ -675,7 +725,7 @@ __global__ __launch_bounds__(256) void k_contact_enter(CArgs a)
         for (int k = 0; k < 3; ++k) {
             const float mA = a.X[k * S + pa] + a.U[k * S + pa];
             const float mB = a.X[k * S + pb] + a.U[k * S + pb];
-            ex = ex || mA != zpi[k] || mB != a.Zp[3 * j + k];
+            ex = ex || mA != a.Zp[3 * i + k] || mB != zpj[k];
         }
         if (ex) {
             exm |= 1u << c;
@@ -687,7 +737,7 @@ __global__ __launch_bounds__(256) void k_contact_enter(CArgs a)
     while (exm) {
         const int c = __ffs(exm) - 1;
         exm &= exm - 1u;
-        const int jl = c0 + c, j = Qb * CS + jl;
+        const int il = r0 + c * RS, i = P * CS + il;
         const long pa = dense_pair_slot(i, j, a.nb, a.T), pb = pa + TT;
         float mA[3], mB[3];
 #pragma unroll
@@ -711,7 +761,7 @@ __global__ __launch_bounds__(256) void k_contact_enter(CArgs a)
 template <int CS>
 __global__ __launch_bounds__(256) void k_contact_leave(CArgs a)
 {
-    constexpr int CSS = CS * CS, W = CS / 32, TPR = 256 / CS, CPT = CSS / 256;
+    constexpr int CSS = CS * CS, W = CS / 32, RS = 256 / CS, CPT = CSS / 256;
     __shared__ unsigned short map[2][CSS];
     __shared__ unsigned mask[2][CS][W];
     const int tid = threadIdx.x;
@@ -733,14 +783,14 @@ __global__ __launch_bounds__(256) void k_contact_leave(CArgs a)
             atomicOr(&mask[b][idx / CS][(idx % CS) >> 5], 1u << (idx & 31));
         }
     __syncthreads();
-    const int il = tid / TPR, c0 = (tid % TPR) * CPT;
-    const int i = P * CS + il;
-    const int ic = min(i, N - 1);
+    const int jl = tid % CS, r0 = tid / CS;
+    const int j = Qb * CS + jl;
 #pragma unroll 2
     for (int c = 0; c < CPT; ++c) {
-        const int jl = c0 + c, j = Qb * CS + jl;
+        const int il = r0 + c * RS, i = P * CS + il;
         const bool vm = (i < N) && (j < N) && (P != Qb || il < jl);
         if (!vm) continue;
+        const int ic = i;
         const int idx = il * CS + jl;
         const bool in_k = (mask[0][il][jl >> 5] >> (jl & 31)) & 1u;
         const bool in_k1 = (mask[1][il][jl >> 5] >> (jl & 31)) & 1u;
@@ -1039,6 +1089,8 @@ static CArgs cargs(DevGraph* s)
     a.Zpp = d->d_Zpp;
     a.aexp = d->d_aexp;
     a.uni = d->d_uni;
+    a.rho = s->d_rho;
+    a.W = d->d_W;
     a.par = d->c_par;
     a.L = d->d_L;
     a.Lmask = d->d_Lmask;
@@ -1053,12 +1105,14 @@ bool dense_contact_ok(DevGraph* s)
 {
     const DenseState* d = s->dense;
     if (!d) return false;
-    if (!(s->rho_uni && s->alpha_uni && s->alpha_u == 1.0f && s->policy_raised == 0)) return false;
+    if (!(s->alpha_uni && s->alpha_u == 1.0f)) return false;
     const char* e = getenv("PARADMM_CONTACT");
     return !(e && e[0] == '0');
 }
 
 bool dense_in_contact(DevGraph* s) { return s->dense && s->dense->c_state; }
+
+bool dense_contact_rho_per_edge(DevGraph* s) { return s->dense && s->dense->c_state && s->dense->c_rhoe; }
 
 // Contact tile side: 64 (a quarter of the per-pair overhead of 32) once
 // the tiles are enough to fill the GPU, 32 below N = 4000 (the smaller
@@ -1089,6 +1143,7 @@ static int contact_alloc(DevGraph* s)
     RC(dmalloc(s, (void**)&d->d_Pc, sizeof(float) * CQ * (size_t)d->nbs * d->N));
     RC(dmalloc(s, (void**)&d->d_Rpc, sizeof(double) * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_Zpp, sizeof(float) * 3 * (size_t)d->N));
+    RC(dmalloc(s, (void**)&d->d_W, sizeof(float) * 2 * (size_t)d->N));
     return PARADMM_OK;
 }
 
@@ -1099,19 +1154,26 @@ static int contact_iteration(DevGraph* s, bool res)
     const CArgs a = cargs(s);
     const unsigned g = (unsigned)d->nts;
     prof_begin(s, res ? "contact_tile_res" : "contact_tile");
+    const bool re = d->c_rhoe;
     if (d->cs == 64) {
-        if (res) k_contact_tile<64, true><<<g, 64, 0, s->stream>>>(a);
-        else k_contact_tile<64, false><<<g, 64, 0, s->stream>>>(a);
+        if (res) re ? k_contact_tile<64, true, true><<<g, 64, 0, s->stream>>>(a)
+                    : k_contact_tile<64, true, false><<<g, 64, 0, s->stream>>>(a);
+        else re ? k_contact_tile<64, false, true><<<g, 64, 0, s->stream>>>(a)
+                : k_contact_tile<64, false, false><<<g, 64, 0, s->stream>>>(a);
     } else {
-        if (res) k_contact_tile<32, true><<<g, 64, 0, s->stream>>>(a);
-        else k_contact_tile<32, false><<<g, 64, 0, s->stream>>>(a);
+        if (res) re ? k_contact_tile<32, true, true><<<g, 64, 0, s->stream>>>(a)
+                    : k_contact_tile<32, true, false><<<g, 64, 0, s->stream>>>(a);
+        else re ? k_contact_tile<32, false, true><<<g, 64, 0, s->stream>>>(a)
+                : k_contact_tile<32, false, false><<<g, 64, 0, s->stream>>>(a);
     }
     prof_end(s);
     CK(hipGetLastError());
     prof_begin(s, res ? "contact_zreduce_res" : "contact_zreduce");
     const unsigned zgrid = (unsigned)((d->N + CZ_CIRC - 1) / CZ_CIRC);
-    if (res) k_contact_zreduce<true><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a);
-    else k_contact_zreduce<false><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a);
+    if (res) re ? k_contact_zreduce<true, true><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a)
+                : k_contact_zreduce<true, false><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a);
+    else re ? k_contact_zreduce<false, true><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a)
+            : k_contact_zreduce<false, false><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a);
     prof_end(s);
     CK(hipGetLastError());
     d->c_par ^= 1;
@@ -1136,7 +1198,15 @@ int dense_contact_enter(DevGraph* s)
     }
     RC(contact_alloc(s));
     d->c_par = 0;  // the entry writes list 0
+    // one rho everywhere (a policy's raised edges make it per edge)
+    d->c_rhoe = !(s->rho_uni && s->policy_raised == 0);
     const CArgs a = cargs(s);
+    if (d->c_rhoe) {
+        prof_begin(s, "contact_weights");
+        k_contact_weights<<<(unsigned)d->N, 64, 0, s->stream>>>(a);
+        prof_end(s);
+        CK(hipGetLastError());
+    }
     prof_begin(s, "contact_enter");
     if (d->cs == 64) k_contact_enter<64><<<(unsigned)d->nts, 256, 0, s->stream>>>(a);
     else k_contact_enter<32><<<(unsigned)d->nts, 256, 0, s->stream>>>(a);

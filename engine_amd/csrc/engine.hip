@@ -1066,9 +1066,11 @@ static int fused_iteration(DevGraph* s, int from_n, int mphase = DENSE_PLAIN)
 }
 
 // kernel variant a captured run depends on: the uniform rho/alpha kernels,
-// the M state (dense, alpha = 1) and the contact state with its start parity
-// (RUN_PARITY: runs that differ only there are both kept)
-constexpr int RUN_PARITY = 8;
+// the M state (dense, alpha = 1), the contact state with its start parity
+// (RUN_PARITY: runs that differ only there are both kept) and whether its
+// kernels read per-edge rho (a policy's raised edges leave the uniform-rho
+// flag set, so that bit alone does not tell)
+constexpr int RUN_PARITY = 8, RUN_CONTACT_RHO_EDGE = 16;
 
 static bool in_contact(const DevGraph* s)
 {
@@ -1079,7 +1081,8 @@ static int run_variant(const DevGraph* s)
 {
     const bool contact = in_contact(s);
     const int par = contact && dense_contact_parity(const_cast<DevGraph*>(s)) ? RUN_PARITY : 0;
-    return ((s->rho_uni && s->alpha_uni) ? 1 : 0) | (mrun_ok(s) ? 2 : 0) | (contact ? 4 : 0) | par;
+    const int re = contact && dense_contact_rho_per_edge(const_cast<DevGraph*>(s)) ? RUN_CONTACT_RHO_EDGE : 0;
+    return ((s->rho_uni && s->alpha_uni) ? 1 : 0) | (mrun_ok(s) ? 2 : 0) | (contact ? 4 : 0) | par | re;
 }
 
 // The captured run of n iterations for the current variant (recorded on

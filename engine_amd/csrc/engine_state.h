@@ -190,10 +190,12 @@ void dense_reset(DevGraph* s);
 // applies the pending U update (and its primal residual) so U is current
 // (leaves the contact state first)
 int dense_finish(DevGraph* s);
-// contact state (one rho, alpha = 1, tile side 32): only the pairs whose
-// prox is active keep per-edge values; PARADMM_CONTACT=0 disables it
+// contact state (alpha = 1): only the pairs whose prox is active keep
+// per-edge values; PARADMM_CONTACT=0 disables it
 bool dense_contact_ok(DevGraph* s);
 bool dense_in_contact(DevGraph* s);
+// the contact state's kernels read per-edge rho (chosen when it is entered)
+bool dense_contact_rho_per_edge(DevGraph* s);
 // from the steady state: enters the contact state and runs one iteration in it
 int dense_contact_enter(DevGraph* s);
 // back to the steady X, U state (no-op outside the contact state)

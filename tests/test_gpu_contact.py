@@ -159,14 +159,65 @@ def test_contact_leave_and_reenter():
     assert max_rel(gb.read("Z"), ga.read("Z")) < 1e-5
 
 
-def test_contact_off_for_per_edge_rho_and_alpha():
+def test_contact_off_for_alpha_other_than_1():
     g, _ = square_pair(64, 3, rho=2.0, alpha=0.8)
     assert "contact_tile" not in _kernels(g, 4)
     g, _ = square_pair(64, 3, rho=2.0)
     g.sync_to_host()
-    g.array("RHOS")[5] = 3.0
+    g.array("ALPHAS")[5] = 0.5
     g.sync_from_host()
     assert "contact_tile" not in _kernels(g, 4)
+
+
+def _per_edge_rho(nc, seed):
+    """alpha = 1 and a different rho on every edge, in the library and the oracle"""
+    g, o = square_pair(nc, seed, rho=10.0)
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(2.0, 30.0, size=o.nedge)
+    o.array("RHO")[:] = r
+    g.array("RHOS")[:] = r
+    g.sync_from_host()
+    return g, o
+
+
+def test_contact_per_edge_rho_runs_vs_oracle():
+    """Per-edge rho in the contact state (pairs through the prox read their
+    own rho; the consensus divides by each circle's sum of rho): runs of 16
+    iterations from the oracle's state after 400 iterations follow the
+    oracle, and the per-circle residuals equal the per-edge ones."""
+    g, o = _per_edge_rho(80, 4)
+    o.iterate(400)
+    errs = []
+    for _ in range(3):
+        for n in STATE:
+            g.array(n)[:] = o.array(OR_NAME[n])
+        g.sync_from_host()
+        g.iterate(16)
+        assert "contact_tile" in _kernels(g, 2)
+        g.sync_to_host()
+        o.iterate(18)
+        errs.append(max_rel(g.array("Z"), o.array("Z")))
+    print("contact runs with per-edge rho, Z error after 18 iterations:", " ".join(f"{e:.1e}" for e in errs))
+    assert max(errs) < 1e-5
+    g.iterate(40)
+    p2, d2 = g.residuals()
+    g.sync_to_host()  # leaves the contact state
+    p2b, d2b = g.residuals()
+    assert p2b == pytest.approx(p2, rel=1e-4)
+    assert d2b == pytest.approx(d2, rel=1e-4)
+
+
+def test_contact_per_edge_rho_runs_match_eager():
+    g1, _ = _per_edge_rho(70, 9)
+    g2, _ = _per_edge_rho(70, 9)
+    g1.iterate(66)
+    for _ in range(66):
+        g2.iterate(1)
+    assert np.array_equal(g1.read("Z"), g2.read("Z"))
+    g1.sync_to_host()
+    g2.sync_to_host()
+    for n in ("Z", "U", "X"):
+        assert np.array_equal(g1.array(n), g2.array(n)), n
 
 
 def test_contact_solve_matches_per_edge_solve(monkeypatch):
@@ -204,3 +255,28 @@ def test_contact_on_dense_tile_64(monkeypatch):
     gb.sync_to_host()
     gb.iterate(30)
     assert max_rel(gb.read("Z"), ga.read("Z")) < 1e-5
+
+
+def test_contact_runs_after_a_policy_raises_edges():
+    """Captured contact runs recorded before a mode-3 policy application
+    (one rho) are not replayed after it (raised edges: the kernels read
+    per-edge rho; the uniform-rho flag stays set): 64 captured iterations
+    after the application equal 64 eager ones bit for bit."""
+    from engine_amd import paradmm as P
+    from oracle import mlp_oracle as mo
+    m = P.RhoMLP(params=(mo.init_params(7) * 1.5).astype(np.float32))
+    zs = []
+    for eager in (False, True):
+        g, _ = square_pair(70, 5, rho=30.0)
+        g.iterate(64)  # records the uniform-rho runs
+        g.set_rho_policy_mode(3)
+        g.apply_rho_policy(m, 0.3)
+        g.sync_to_host()
+        assert np.unique(g.array("RHOS")).size > 1  # some edges raised
+        if eager:
+            for _ in range(64):
+                g.iterate(1)
+        else:
+            g.iterate(64)
+        zs.append(g.read("Z"))
+    assert np.array_equal(zs[0], zs[1])

@@ -333,14 +333,16 @@ def _per_edge_rho_alpha1(nc, seed):
     return g, o
 
 
-def test_m_state_runs_with_per_edge_rho_vs_oracle():
+def test_m_state_runs_with_per_edge_rho_vs_oracle(monkeypatch):
     """alpha = 1 with a different rho on every edge: a captured run of 16
     iterations keeps M = X + U in the X array (the per-edge rho variant)
     and must follow the FP64 oracle; every step starts from the oracle's
     state (uploaded), so only the run itself is compared.  The states are
     taken after 400 iterations: earlier, a pair whose overlap test flips
     between fp32 and FP64 makes any fp32 run part from the oracle (the
-    plain kernel as much as this one)."""
+    plain kernel as much as this one).  The per-edge path (the contact state
+    takes per-edge rho too by default)."""
+    monkeypatch.setenv("PARADMM_CONTACT", "0")
     g, o = _per_edge_rho_alpha1(80, 4)
     o.iterate(400)
     errs = []
@@ -357,9 +359,11 @@ def test_m_state_runs_with_per_edge_rho_vs_oracle():
     assert max(errs) < 1e-5
 
 
-def test_m_state_runs_match_eager_with_per_edge_rho():
+def test_m_state_runs_match_eager_with_per_edge_rho(monkeypatch):
     """Bit-for-bit: 32 iterations as two captured runs (the M state, per-edge
-    rho) against 32 single iterations (the plain fused kernel)."""
+    rho) against 32 single iterations (the plain fused kernel), on the
+    per-edge path."""
+    monkeypatch.setenv("PARADMM_CONTACT", "0")
     g1, _ = _per_edge_rho_alpha1(70, 9)
     g2, _ = _per_edge_rho_alpha1(70, 9)
     g1.iterate(32)

@@ -191,11 +191,14 @@ def test_policy_raise_mode_keeps_edges_the_network_raises(monkeypatch, layout):
     assert np.all(np.isfinite(g.array("Z")))
 
 
-def test_raised_tiles_match_the_per_edge_variant():
+def test_raised_tiles_match_the_per_edge_variant(monkeypatch):
     """After a mode-3 application with raised edges the dense layout keeps
     its uniform-rho kernel and sends only the tiles holding raised edges
     through the per-edge streams; iterations match the per-edge kernel run
-    on the same state (re-uploaded from the host, which selects it)."""
+    on the same state (re-uploaded from the host, which selects it).  On
+    the per-edge path (the contact state reads every processed pair's own
+    rho instead; test_gpu_contact.py)."""
+    monkeypatch.setenv("PARADMM_CONTACT", "0")
     n = 70
     m = P.RhoMLP(params=(mo.init_params(7) * 1.5).astype(np.float32))
     gs = []
