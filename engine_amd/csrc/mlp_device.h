@@ -11,6 +11,7 @@
 // the D layout of one layer to the A layout of the next through LDS.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
 
 #include <stdint.h>
 
@@ -21,15 +22,9 @@ namespace pdm {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// round-to-nearest-even fp32 -> bf16 (NaN stays NaN)
-__device__ __forceinline__ short f2bf(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (short)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
-
+// round-to-nearest-even fp32 -> bf16 (NaN stays NaN): the gfx950
+// conversion instruction (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ short f2bf(float f) { return __builtin_bit_cast(short, __float2bfloat16(f)); }
 __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
 
 __device__ __forceinline__ v4f mfma16(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
