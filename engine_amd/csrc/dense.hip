@@ -22,6 +22,9 @@
 //   k_dense_zreduce: Z_c = sum rho M / sum rho over the circle's T-block
 //     partials in fixed order (atomic-free, deterministic), and the FP64
 //     dual residual.
+// With alpha = 1 the layout runs the contact state instead (below): only
+// the pairs whose prox is active keep per-edge values, in per-tile lists,
+// and an iteration is k_contact_tile + k_contact_zreduce.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
