@@ -74,10 +74,12 @@ __device__ __forceinline__ bool pair_project(float c1x, float c1y, float r1, flo
     const float ix1 = 1.0f / wx1, ir1 = 1.0f / wr1, ix2 = 1.0f / wx2, ir2 = 1.0f / wr2;
     const float lam = (dist - s) / (ix1 + ir1 + ix2 + ir2);
     // coincident centres have no centre line (the reference divides 0/0,
-    // packing/host.c:123): they are separated along +x
+    // packing/host.c:123): they are separated along x, the circle of the
+    // smaller radius towards -x (circle 1 on equal radii), whichever of the
+    // two is listed first
     const bool same = !(dist > 0.0f);
     const float idist = same ? 0.0f : 1.0f / dist;
-    const float ux = same ? 1.0f : -dx * idist, uy = same ? 0.0f : -dy * idist;
+    const float ux = same ? (r2 < r1 ? -1.0f : 1.0f) : -dx * idist, uy = same ? 0.0f : -dy * idist;
     const float m1 = lam * ix1, m2 = lam * ix2;
     o1x = c1x + ux * m1;
     o1y = c1y + uy * m1;

@@ -381,8 +381,10 @@ static int pair_project(const double *c1, const double *c2, int nc, double r1, d
     *r2_out = r2 + lam / w_r2;
     for (int k = 0; k < nc; k++) {
         /* coincident centres: the reference divides 0/0 (packing/host.c:123);
-         * here they separate along the first axis */
-        double u = dist > 0 ? (c2[k] - c1[k]) / dist : (k == 0 ? 1.0 : 0.0);
+         * here they separate along the first axis, the circle of the smaller
+         * radius towards -x (circle 1 on equal radii), so the outcome does
+         * not depend on which circle is listed first */
+        double u = dist > 0 ? (c2[k] - c1[k]) / dist : (k == 0 ? (r2 < r1 ? -1.0 : 1.0) : 0.0);
         c1_out[k] = c1[k] + u * x1;
         c2_out[k] = c2[k] + u * (x2 - dist);
     }

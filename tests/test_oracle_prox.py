@@ -238,6 +238,11 @@ def test_pair_coincident_centres_separate_along_x(oracle):
     assert x[0] < 0.3 < x[3]                          # circle 1 to -x, circle 2 to +x
     assert x[3] - x[0] == pytest.approx(x[2] + x[5], abs=1e-12)
     assert (0.3 - x[0]) * 2.0 == pytest.approx((x[3] - 0.3) * 6.0, rel=1e-12)
+    # listed the other way round, the smaller circle still goes to -x: the
+    # same result with the circles swapped
+    n2 = np.concatenate([n[3:], n[:3]])
+    x2 = prox(oracle, "or_prox_pair3", n2, [6.0, 2.0], 2, 3)
+    assert x2 == pytest.approx(np.concatenate([x[3:], x[:3]]), rel=1e-12, abs=1e-15)
 
 
 def test_radius_a1_at_zero_is_finite(oracle):
