@@ -74,7 +74,7 @@ struct DenseState {
     bool c_rhoe = false;       // rho differs between edges (fixed while in the state)
     float* d_W = nullptr;      // [2][N] sum rho, sum rho^2 per circle (per-edge rho)
     unsigned* d_Lmask = nullptr; // [2][nts][cs][cs/32] the pairs of each list, per tile row
-    float* d_Pc = nullptr;     // [CQ][nbs][N] circle partials of the contact iteration
+    unsigned long long* d_Ac = nullptr;  // [CQ][N] circle sums of the contact iteration
     double* d_Rpc = nullptr;   // [N] primal residual per circle
     float* d_Zpp = nullptr;    // [N][3] Z two iterations back
 };
@@ -197,8 +197,10 @@ __global__ __launch_bounds__(256) void k_dense_res_final(int nt, const double* _
 // where a and g are 0 on every other edge.  The primal residual
 // sum |X - Z_{k+1}|^2 is then, per circle, |g - e|^2 summed over all N + 1
 // edges with e = Z_{k+1} - 2 Z_k + Z_{k-1}: S2 - 2 e.S1 + (N + 1)|e|^2.
-// The sums run in 2^-31 fixed point (64-bit LDS atomics), so they do not
-// depend on the order the lanes add them: runs are bitwise reproducible.
+// The sums run in 2^-31 fixed point (64-bit LDS atomics inside a tile, then
+// 64-bit global atomics of the tile's nonzero sums into the circle's
+// accumulator), so they do not depend on the order the lanes or the tiles
+// add them: runs are bitwise reproducible.
 //
 // The per-pair test replaces 24 bytes of M per edge read and written by
 // ~10 VALU operations; the state is Z, Z_prev, Z_prev2 and the lists.
@@ -224,20 +226,21 @@ struct CArgs {
     float4* L;             // [2][nts][CS*CS][2]
     unsigned* Lmask;       // [2][nts][CS][CS/32] pairs of each list, per tile row
     int* cnt;              // [2][nts]
-    float* Pc;             // [CQ][nbs][N]
+    unsigned long long* Ac; // [CQ][N] circle sums, 2^-31 fixed point (zero between iterations)
     double* Rpc;
     double* Rd;
 };
 
-// acc += v in 2^-31 fixed point: v = k + f with k = rint(v) and f = v - k
-// exact (|f| <= 1/2), so v 2^31 = k 2^31 + f 2^31 needs no FP64 (|v| < 2^31)
-__device__ __forceinline__ void cadd(unsigned long long* acc, float v)
+// v in 2^-31 fixed point: v = k + f with k = rint(v) and f = v - k exact
+// (|f| <= 1/2), so v 2^31 = k 2^31 + f 2^31 needs no FP64 (|v| < 2^31)
+__device__ __forceinline__ long long cfix(float v)
 {
     const float k = rintf(v);
     const float f = v - k;
-    const long long q = (long long)(int)k * 2147483648LL + (long long)(int)(f * 2147483648.0f);
-    atomicAdd(acc, (unsigned long long)q);
+    return (long long)(int)k * 2147483648LL + (long long)(int)(f * 2147483648.0f);
 }
+
+__device__ __forceinline__ void cadd(unsigned long long* acc, float v) { atomicAdd(acc, (unsigned long long)cfix(v)); }
 
 // (I, J) of tile t of a triangular tiling with nb blocks: row I starts at
 // I nb - I (I - 1) / 2 (the float estimate is corrected by the loops)
@@ -430,6 +433,69 @@ __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArg
 // also the residual sums (the last iteration before residuals are read).
 // RE: rho differs between edges; the pairs taken through the prox read
 // theirs from the dense slots (inactive pairs need none).
+// The box and radius edges of the CS circles of block P, one circle per
+// lane: the plain U update, prox and X, U stores of the fused iteration,
+// their sums added to the circles' accumulators.  These are wavefronts of
+// their own in the tile launch, so the diagonal tiles are no longer than
+// the rest.
+template <int CS, bool RES, bool RE>
+__device__ __forceinline__ void contact_unary(const CArgs& a, int P)
+{
+    constexpr int Q = RES ? CQ : 3;
+    const long S = a.stride, TT = (long)a.T * a.T;
+    for (int e = threadIdx.x; e < CS; e += 64) {
+        const int cc = P * CS + e;
+        if (cc >= a.N) break;
+        float z[3], cz[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            z[k] = a.Z[3 * cc + k];
+            cz[k] = z[k] - a.Zp[3 * cc + k];
+        }
+        const long p0 = dense_unary_slot(cc, a.nb, a.T);
+        long long s[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) s[q] = 0;
+        float g2 = 0.0f;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const long q = p0 + side * TT;
+            const float r = RE ? a.rho[q] : a.uni[0];
+            float x[3], u[3], nn[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                x[k] = a.X[k * S + q];
+                u[k] = (x[k] + a.U[k * S + q]) - z[k];
+                nn[k] = z[k] - u[k];
+            }
+            if (side == 0) {
+                box3(x[0], x[1], x[2], nn[0], nn[1], nn[2]);
+            } else {
+                x[0] = nn[0];
+                x[1] = nn[1];
+                x[2] = max_alpha_radius(nn[2], r, a.aexp[cc]);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a.X[k * S + q] = x[k];
+                a.U[k * S + q] = u[k];
+                s[k] += cfix(RE ? r * ((x[k] + u[k]) - z[k]) : (x[k] + u[k]) - z[k]);
+                if constexpr (RES) {
+                    const float g = (x[k] - z[k]) - cz[k];
+                    s[3 + k] += cfix(g);
+                    g2 += g * g;
+                }
+            }
+        }
+        if constexpr (RES) s[6] += cfix(g2);
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (s[q]) atomicAdd(&a.Ac[(long)q * a.N + cc], (unsigned long long)s[q]);
+    }
+}
+
+// Launch: nbs + nts wavefronts, the first nbs the blocks' box and radius
+// edges (contact_unary), then the contact tiles.
 template <int CS, bool RES, bool RE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_contact_tile(CArgs a)
 {
@@ -437,8 +503,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     constexpr int CSS = CS * CS, RL = CS / 8, W = CS / 32;
     __shared__ ContactWave<CS, Q> sh;
     const int lane = threadIdx.x;
-    const int t = blockIdx.x;
     const int N = a.N, nbs = a.nbs;
+    if ((int)blockIdx.x < nbs) {
+        contact_unary<CS, RES, RE>(a, blockIdx.x);
+        return;
+    }
+    const int t = blockIdx.x - nbs;
     int P, Qb;
     tri_tile(t, nbs, P, Qb);
     const bool diag = P == Qb;
@@ -539,72 +609,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (avail > done) contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
     if (lane == 0) a.cnt[tn] = base;
 
-    // the diagonal tile's box and radius edges: the plain U update, prox
-    // and X, U stores of the fused iteration
-    for (int e = lane; diag && e < CS; e += 64) {
-        const int cc = P * CS + e;
-        if (cc >= N) break;
-        const long S = a.stride;
-        float z[3], cz[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            z[k] = sh.zc[e][k];
-            cz[k] = sh.zc[e][3 + k];
-        }
-        const long p0 = dense_unary_slot(cc, a.nb, a.T);
-        const long TT = (long)a.T * a.T;
-        float g2 = 0.0f;
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const long q = p0 + side * TT;
-            const float r = RE ? a.rho[q] : rho;
-            float x[3], u[3], nn[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                x[k] = a.X[k * S + q];
-                u[k] = (x[k] + a.U[k * S + q]) - z[k];
-                nn[k] = z[k] - u[k];
-            }
-            if (side == 0) {
-                box3(x[0], x[1], x[2], nn[0], nn[1], nn[2]);
-            } else {
-                x[0] = nn[0];
-                x[1] = nn[1];
-                x[2] = max_alpha_radius(nn[2], r, a.aexp[cc]);
-            }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                a.X[k * S + q] = x[k];
-                a.U[k * S + q] = u[k];
-                cadd(&sh.acc[e][k], RE ? r * ((x[k] + u[k]) - z[k]) : (x[k] + u[k]) - z[k]);
-                if constexpr (RES) {
-                    const float g = (x[k] - z[k]) - cz[k];
-                    cadd(&sh.acc[e][3 + k], g);
-                    g2 += g * g;
-                }
-            }
-        }
-        if constexpr (RES) cadd(&sh.acc[e][6], g2);
-    }
     wave_lds_fence();
     for (int e = lane; e < CS * W; e += 64) a.Lmask[tn * CS * W + e] = (&sh.nmask[0][0])[e];
-    // circle partials: Pc[q][K][c], rows into K = Q, columns into K = P
+    // the tile's nonzero circle sums into the circles' accumulators (most
+    // circles of a tile have no active pair in it)
     for (int e = lane; e < (diag ? CS : 2 * CS); e += 64) {
         const int circ = e < CS ? P * CS + e : Qb * CS + (e - CS);
-        const int K = e < CS ? Qb : P;
         if (circ < N)
 #pragma unroll
-            for (int q = 0; q < Q; ++q)
-                a.Pc[((long)q * nbs + K) * N + circ] = (float)((double)(long long)sh.acc[e][q] * CFIX_INV);
+            for (int q = 0; q < Q; ++q) {
+                const unsigned long long v = sh.acc[e][q];
+                if (v) atomicAdd(&a.Ac[(long)q * N + circ], v);
+            }
     }
 }
 
-// Z_{k+1} = Z_k + sum a / (N + 1) over the circle's nbs block partials in
-// fixed order (RE, rho per edge: Z_k + sum rho a / sum rho), the Z history
-// shift and, RES, the per-circle primal and dual residuals.  Workgroup =
-// CZ_CIRC circles (128-byte rows of the partials) x CZ_SLICES slices of the
-// blocks, added by a fixed-shape tree.
-constexpr int CZ_CIRC = 32, CZ_SLICES = 32;
+// Z_{k+1} = Z_k + sum a / (N + 1) from the circle's accumulators (RE, rho
+// per edge: Z_k + sum rho a / sum rho), which it clears for the next
+// iteration, the Z history shift and, RES, the per-circle primal and dual
+// residuals.  One thread per circle.
+constexpr int CZ_THREADS = 256;
 
 // Per-edge rho: W[c] = sum rho and W[N + c] = sum rho^2 over circle c's N + 1
 // edges, FP64 in a fixed order (lane j's pairs j, j + 64, ..., then the
@@ -640,42 +664,30 @@ __global__ __launch_bounds__(64) void k_contact_weights(CArgs a)
 }
 
 template <bool RES, bool RE>
-__global__ __launch_bounds__(CZ_CIRC * CZ_SLICES) void k_contact_zreduce(CArgs a)
+__global__ __launch_bounds__(CZ_THREADS) void k_contact_zreduce(CArgs a)
 {
     constexpr int Q = RES ? CQ : 3;
-    __shared__ float part[CZ_SLICES][Q][CZ_CIRC];
-    const int ks = threadIdx.x / CZ_CIRC, cl = threadIdx.x % CZ_CIRC;
-    const int c = blockIdx.x * CZ_CIRC + cl;
-    const int N = a.N, nbs = a.nbs;
-    float s[Q];
+    const int c = blockIdx.x * CZ_THREADS + threadIdx.x;
+    const int N = a.N;
+    if (c >= N) return;
+    double s[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) s[q] = 0.0f;
-    if (c < N)
-#pragma unroll 4
-        for (int K = ks; K < nbs; K += CZ_SLICES)
-#pragma unroll
-            for (int q = 0; q < Q; ++q) s[q] += a.Pc[((long)q * nbs + K) * N + c];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) part[ks][q][cl] = s[q];
-    __syncthreads();
-    for (int h = CZ_SLICES / 2; h > 0; h >>= 1) {
-        if (ks < h)
-#pragma unroll
-            for (int q = 0; q < Q; ++q) part[ks][q][cl] += part[ks + h][q][cl];
-        __syncthreads();
+    for (int q = 0; q < Q; ++q) {
+        unsigned long long* acc = a.Ac + (long)q * N + c;
+        s[q] = (double)(long long)*acc * CFIX_INV;
+        *acc = 0ull;
     }
-    if (threadIdx.x >= CZ_CIRC || c >= N) return;
     const float deg = (float)(N + 1);
     double p = 0.0, dz2 = 0.0;
-    if constexpr (RES) p = (double)part[0][6][cl];
+    if constexpr (RES) p = s[6];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float zk = a.Z[3 * c + k], zp = a.Zp[3 * c + k];
-        const float zn = zk + part[0][k][cl] / (RE ? a.W[c] : deg);
+        const float zn = zk + (float)s[k] / (RE ? a.W[c] : deg);
         if constexpr (RES) {
             const float dz = zn - zk;
             const double e = (double)(dz - (zk - zp));
-            p += -2.0 * e * (double)part[0][3 + k][cl] + (double)deg * e * e;
+            p += -2.0 * e * s[3 + k] + (double)deg * e * e;
             dz2 += (double)dz * dz;
         }
         a.Zpp[3 * c + k] = zp;
@@ -1098,7 +1110,7 @@ static CArgs cargs(DevGraph* s)
     a.L = d->d_L;
     a.Lmask = d->d_Lmask;
     a.cnt = d->d_cnt;
-    a.Pc = d->d_Pc;
+    a.Ac = d->d_Ac;
     a.Rpc = d->d_Rpc;
     a.Rd = d->d_Rd;
     return a;
@@ -1143,7 +1155,8 @@ static int contact_alloc(DevGraph* s)
     RC(dmalloc(s, (void**)&d->d_Lmask, sizeof(unsigned) * 2 * nts * words));
     CK(hipMemsetAsync(d->d_cnt, 0, sizeof(int) * 2 * nts, s->stream));
     CK(hipMemsetAsync(d->d_Lmask, 0, sizeof(unsigned) * 2 * nts * words, s->stream));
-    RC(dmalloc(s, (void**)&d->d_Pc, sizeof(float) * CQ * (size_t)d->nbs * d->N));
+    RC(dmalloc(s, (void**)&d->d_Ac, sizeof(unsigned long long) * CQ * (size_t)d->N));
+    CK(hipMemsetAsync(d->d_Ac, 0, sizeof(unsigned long long) * CQ * (size_t)d->N, s->stream));
     RC(dmalloc(s, (void**)&d->d_Rpc, sizeof(double) * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_Zpp, sizeof(float) * 3 * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_W, sizeof(float) * 2 * (size_t)d->N));
@@ -1155,7 +1168,7 @@ static int contact_iteration(DevGraph* s, bool res)
 {
     DenseState* d = s->dense;
     const CArgs a = cargs(s);
-    const unsigned g = (unsigned)d->nts;
+    const unsigned g = (unsigned)(d->nbs + d->nts);  // unary wavefronts, then the tiles
     prof_begin(s, res ? "contact_tile_res" : "contact_tile");
     const bool re = d->c_rhoe;
     if (d->cs == 64) {
@@ -1172,11 +1185,11 @@ static int contact_iteration(DevGraph* s, bool res)
     prof_end(s);
     CK(hipGetLastError());
     prof_begin(s, res ? "contact_zreduce_res" : "contact_zreduce");
-    const unsigned zgrid = (unsigned)((d->N + CZ_CIRC - 1) / CZ_CIRC);
-    if (res) re ? k_contact_zreduce<true, true><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a)
-                : k_contact_zreduce<true, false><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a);
-    else re ? k_contact_zreduce<false, true><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a)
-            : k_contact_zreduce<false, false><<<zgrid, CZ_CIRC * CZ_SLICES, 0, s->stream>>>(a);
+    const unsigned zgrid = (unsigned)((d->N + CZ_THREADS - 1) / CZ_THREADS);
+    if (res) re ? k_contact_zreduce<true, true><<<zgrid, CZ_THREADS, 0, s->stream>>>(a)
+                : k_contact_zreduce<true, false><<<zgrid, CZ_THREADS, 0, s->stream>>>(a);
+    else re ? k_contact_zreduce<false, true><<<zgrid, CZ_THREADS, 0, s->stream>>>(a)
+            : k_contact_zreduce<false, false><<<zgrid, CZ_THREADS, 0, s->stream>>>(a);
     prof_end(s);
     CK(hipGetLastError());
     d->c_par ^= 1;
