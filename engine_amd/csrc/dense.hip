@@ -25,6 +25,7 @@
 // With alpha = 1 the layout runs the contact state instead (below): only
 // the pairs whose prox is active keep per-edge values, in per-tile lists,
 // and an iteration is k_contact_tile + k_contact_zreduce.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -1164,33 +1165,42 @@ static int contact_alloc(DevGraph* s)
 }
 
 // res: also the residual sums (the last iteration of a run)
+template <typename K>
+static void contact_launch(DevGraph* s, const char* name, K kernel, unsigned grid, unsigned block, const CArgs& a)
+{
+    hipEvent_t e0, e1;
+    prof_dispatch(s, name, e0, e1);
+    if (e0)  // timed, eager
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, e0, e1, 0, a);
+    else
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, a);
+}
+
 static int contact_iteration(DevGraph* s, bool res)
 {
     DenseState* d = s->dense;
     const CArgs a = cargs(s);
     const unsigned g = (unsigned)(d->nbs + d->nts);  // unary wavefronts, then the tiles
-    prof_begin(s, res ? "contact_tile_res" : "contact_tile");
+    const char* tn = res ? "contact_tile_res" : "contact_tile";
     const bool re = d->c_rhoe;
     if (d->cs == 64) {
-        if (res) re ? k_contact_tile<64, true, true><<<g, 64, 0, s->stream>>>(a)
-                    : k_contact_tile<64, true, false><<<g, 64, 0, s->stream>>>(a);
-        else re ? k_contact_tile<64, false, true><<<g, 64, 0, s->stream>>>(a)
-                : k_contact_tile<64, false, false><<<g, 64, 0, s->stream>>>(a);
+        if (res) re ? contact_launch(s, tn, k_contact_tile<64, true, true>, g, 64, a)
+                    : contact_launch(s, tn, k_contact_tile<64, true, false>, g, 64, a);
+        else re ? contact_launch(s, tn, k_contact_tile<64, false, true>, g, 64, a)
+                : contact_launch(s, tn, k_contact_tile<64, false, false>, g, 64, a);
     } else {
-        if (res) re ? k_contact_tile<32, true, true><<<g, 64, 0, s->stream>>>(a)
-                    : k_contact_tile<32, true, false><<<g, 64, 0, s->stream>>>(a);
-        else re ? k_contact_tile<32, false, true><<<g, 64, 0, s->stream>>>(a)
-                : k_contact_tile<32, false, false><<<g, 64, 0, s->stream>>>(a);
+        if (res) re ? contact_launch(s, tn, k_contact_tile<32, true, true>, g, 64, a)
+                    : contact_launch(s, tn, k_contact_tile<32, true, false>, g, 64, a);
+        else re ? contact_launch(s, tn, k_contact_tile<32, false, true>, g, 64, a)
+                : contact_launch(s, tn, k_contact_tile<32, false, false>, g, 64, a);
     }
-    prof_end(s);
     CK(hipGetLastError());
-    prof_begin(s, res ? "contact_zreduce_res" : "contact_zreduce");
+    const char* zn = res ? "contact_zreduce_res" : "contact_zreduce";
     const unsigned zgrid = (unsigned)((d->N + CZ_THREADS - 1) / CZ_THREADS);
-    if (res) re ? k_contact_zreduce<true, true><<<zgrid, CZ_THREADS, 0, s->stream>>>(a)
-                : k_contact_zreduce<true, false><<<zgrid, CZ_THREADS, 0, s->stream>>>(a);
-    else re ? k_contact_zreduce<false, true><<<zgrid, CZ_THREADS, 0, s->stream>>>(a)
-            : k_contact_zreduce<false, false><<<zgrid, CZ_THREADS, 0, s->stream>>>(a);
-    prof_end(s);
+    if (res) re ? contact_launch(s, zn, k_contact_zreduce<true, true>, zgrid, CZ_THREADS, a)
+                : contact_launch(s, zn, k_contact_zreduce<true, false>, zgrid, CZ_THREADS, a);
+    else re ? contact_launch(s, zn, k_contact_zreduce<false, true>, zgrid, CZ_THREADS, a)
+            : contact_launch(s, zn, k_contact_zreduce<false, false>, zgrid, CZ_THREADS, a);
     CK(hipGetLastError());
     d->c_par ^= 1;
     return PARADMM_OK;

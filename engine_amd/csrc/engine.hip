@@ -1555,13 +1555,13 @@ extern "C" int paradmm_gpu_time_iterations(graph* G, int iters, double* ms)
 
 namespace pdm {
 
-void prof_begin(DevGraph* s, const char* name)
+// a new record with two events of the pool (false: no event left)
+static bool prof_record(DevGraph* s, const char* name)
 {
-    if (!s->prof) return;
     if (s->prof_used + 2 > s->prof_pool.size()) {
         for (int i = 0; i < 64; ++i) {
             hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return;
+            if (hipEventCreate(&e) != hipSuccess) return false;
             s->prof_pool.push_back(e);
         }
     }
@@ -1569,14 +1569,28 @@ void prof_begin(DevGraph* s, const char* name)
     r.name = name;
     r.a = s->prof_pool[s->prof_used++];
     r.b = s->prof_pool[s->prof_used++];
-    (void)hipEventRecord(r.a, s->stream);
     s->prof_recs.push_back(r);
+    return true;
+}
+
+void prof_begin(DevGraph* s, const char* name)
+{
+    if (!s->prof || !prof_record(s, name)) return;
+    (void)hipEventRecord(s->prof_recs.back().a, s->stream);
 }
 
 void prof_end(DevGraph* s)
 {
     if (!s->prof || s->prof_recs.empty()) return;
     (void)hipEventRecord(s->prof_recs.back().b, s->stream);
+}
+
+void prof_dispatch(DevGraph* s, const char* name, hipEvent_t& start, hipEvent_t& stop)
+{
+    start = stop = nullptr;
+    if (!s->prof || !prof_record(s, name)) return;
+    start = s->prof_recs.back().a;
+    stop = s->prof_recs.back().b;
 }
 
 }  // namespace pdm

@@ -135,6 +135,10 @@ struct DevGraph {
 // Brackets one kernel launch with events when profiling (no-op otherwise).
 void prof_begin(DevGraph* s, const char* name);
 void prof_end(DevGraph* s);
+// For a launch through hipExtLaunchKernelGGL, whose dispatch records the
+// events at the kernel's own start and end (no event-record overhead in
+// the figure): both null when not profiling.
+void prof_dispatch(DevGraph* s, const char* name, hipEvent_t& start, hipEvent_t& stop);
 
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 // device state of G, finalised; with upload, host writes since are uploaded
