@@ -180,6 +180,25 @@ def _per_edge_rho(nc, seed):
     return g, o
 
 
+@pytest.mark.parametrize("per_edge_rho", [False, True])
+def test_contact_tile_side_does_not_change_the_bits(monkeypatch, per_edge_rho):
+    """Each pair's and each box/radius edge's contribution to its circle's
+    sum is converted to fixed point once and summed as an integer, so which
+    contact tile (of 32 or of 64) holds a pair, and in what order the lanes
+    and tiles add, leaves Z, X and U bitwise the same."""
+    out = []
+    for cs in ("32", "64"):
+        monkeypatch.setenv("PARADMM_CONTACT_TILE", cs)
+        g, _ = _per_edge_rho(150, 5) if per_edge_rho else square_pair(150, 21, rho=3.0)
+        g.iterate(70)
+        assert "contact_tile" in _kernels(g, 2)
+        z = g.read("Z")
+        g.sync_to_host()
+        out.append([z] + [g.array(n).copy() for n in ("X", "U")])
+    for a, b, n in zip(out[0], out[1], ("Z", "X", "U")):
+        assert np.array_equal(a, b), n
+
+
 def test_contact_per_edge_rho_runs_vs_oracle():
     """Per-edge rho in the contact state (pairs through the prox read their
     own rho; the consensus divides by each circle's sum of rho): runs of 16
