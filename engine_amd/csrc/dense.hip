@@ -24,7 +24,8 @@
 //     dual residual.
 // With alpha = 1 the layout runs the contact state instead (below): only
 // the pairs whose prox is active keep per-edge values, in per-tile lists,
-// and an iteration is k_contact_tile + k_contact_zreduce.
+// and an iteration is one k_contact_tile launch (k_contact_zreduce ends a
+// run of them).
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -75,7 +76,9 @@ struct DenseState {
     bool c_rhoe = false;       // rho differs between edges (fixed while in the state)
     float* d_W = nullptr;      // [2][N] sum rho, sum rho^2 per circle (per-edge rho)
     unsigned* d_Lmask = nullptr; // [2][nts][cs][cs/32] the pairs of each list, per tile row
-    unsigned long long* d_Ac = nullptr;  // [CQ][N] circle sums of the contact iteration
+    unsigned long long* d_Ac = nullptr;  // [3][CQ][N] circle sums of the last three launches of a run
+    float* d_Zb = nullptr;               // [2][N][3] Z inside a run (its canonical arrays hold the run's start)
+    int c_run = 0;                       // contact launches since the last run end
     double* d_Rpc = nullptr;   // [N] primal residual per circle
     float* d_Zpp = nullptr;    // [N][3] Z two iterations back
 };
@@ -227,7 +230,12 @@ struct CArgs {
     float4* L;             // [2][nts][CS*CS][2]
     unsigned* Lmask;       // [2][nts][CS][CS/32] pairs of each list, per tile row
     int* cnt;              // [2][nts]
-    unsigned long long* Ac; // [CQ][N] circle sums, 2^-31 fixed point (zero between iterations)
+    unsigned long long* Ac; // [CQ][N] this launch's circle sums, 2^-31 fixed point (starts at zero)
+    const unsigned long long* Ap; // [CQ][N] the previous launch's sums, not yet applied to Z (or null)
+    unsigned long long* Acl; // [CQ][N] the sums two launches back, cleared here (or null)
+    const float* Zin;      // the Z the pending sums Ap apply to
+    const float* Zpin;     // (run end) the Z before Zin
+    float* Zo;             // Zin + Ap / (N + 1): this launch's Z, stored by the box-and-radius wavefronts
     double* Rpc;
     double* Rd;
 };
@@ -434,6 +442,31 @@ __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArg
 // also the residual sums (the last iteration before residuals are read).
 // RE: rho differs between edges; the pairs taken through the prox read
 // theirs from the dense slots (inactive pairs need none).
+// Circle cc's consensus of this launch, z = Z_k, and c = Z_k - Z_{k-1}.
+// Inside a run the previous launch's sums are applied here, by every
+// wavefront that needs the circle (the same operations as the run's last
+// update in k_contact_zreduce, so the same bits): the Z update needs no
+// launch of its own between iterations.
+template <bool RE>
+__device__ __forceinline__ void contact_zc(const CArgs& a, int cc, float (&z)[3], float (&c)[3])
+{
+    if (a.Ap) {
+        const float den = RE ? a.W[cc] : (float)(a.N + 1);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float zk = a.Zin[3 * cc + k];
+            z[k] = zk + (float)((double)(long long)a.Ap[(long)k * a.N + cc] * CFIX_INV) / den;
+            c[k] = z[k] - zk;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            z[k] = a.Z[3 * cc + k];
+            c[k] = z[k] - a.Zp[3 * cc + k];
+        }
+    }
+}
+
 // The box and radius edges of the CS circles of block P, one circle per
 // lane: the plain U update, prox and X, U stores of the fused iteration,
 // their sums added to the circles' accumulators.  These are wavefronts of
@@ -448,11 +481,13 @@ __device__ __forceinline__ void contact_unary(const CArgs& a, int P)
         const int cc = P * CS + e;
         if (cc >= a.N) break;
         float z[3], cz[3];
+        contact_zc<RE>(a, cc, z, cz);
+        if (a.Zo)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            z[k] = a.Z[3 * cc + k];
-            cz[k] = z[k] - a.Zp[3 * cc + k];
-        }
+            for (int k = 0; k < 3; ++k) a.Zo[3 * cc + k] = z[k];
+        if (a.Acl)
+#pragma unroll
+            for (int q = 0; q < CQ; ++q) a.Acl[(long)q * a.N + cc] = 0ull;
         const long p0 = dense_unary_slot(cc, a.nb, a.T);
         long long s[Q];
 #pragma unroll
@@ -522,11 +557,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // circle table: entries 0..CS-1 the rows' circles, CS..2CS-1 the columns'
     for (int e = lane; e < 2 * CS; e += 64) {
         const int cc = min((e < CS ? P * CS + e : Qb * CS + (e - CS)), N - 1);
+        float z[3], c[3];
+        contact_zc<RE>(a, cc, z, c);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float z = a.Z[3 * cc + k];
-            sh.zc[e][k] = z;
-            sh.zc[e][3 + k] = z - a.Zp[3 * cc + k];
+            sh.zc[e][k] = z[k];
+            sh.zc[e][3 + k] = c[k];
         }
     }
     for (int e = lane; e < CS * W; e += 64) {
@@ -625,10 +661,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
 }
 
-// Z_{k+1} = Z_k + sum a / (N + 1) from the circle's accumulators (RE, rho
-// per edge: Z_k + sum rho a / sum rho), which it clears for the next
-// iteration, the Z history shift and, RES, the per-circle primal and dual
-// residuals.  One thread per circle.
+// The end of a run: Z_{k+1} = Z_k + sum a / (N + 1) from the last launch's
+// sums (RE, rho per edge: Z_k + sum rho a / sum rho) into the canonical Z,
+// Z_prev, Z_prev2 (Zin = Z_k, Zpin = Z_{k-1}: canonical or inside the run's
+// ring), the clearing of the sum buffers and, RES, the per-circle primal
+// and dual residuals.  One thread per circle.
 constexpr int CZ_THREADS = 256;
 
 // Per-edge rho: W[c] = sum rho and W[N + c] = sum rho^2 over circle c's N + 1
@@ -673,26 +710,33 @@ __global__ __launch_bounds__(CZ_THREADS) void k_contact_zreduce(CArgs a)
     if (c >= N) return;
     double s[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        unsigned long long* acc = a.Ac + (long)q * N + c;
-        s[q] = (double)(long long)*acc * CFIX_INV;
-        *acc = 0ull;
+    for (int q = 0; q < Q; ++q) s[q] = (double)(long long)a.Ac[(long)q * N + c] * CFIX_INV;
+    float zk[3], zp[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        zk[k] = a.Zin[3 * c + k];
+        zp[k] = a.Zpin[3 * c + k];
+    }
+    // every sum buffer of the run is zero for the next one
+#pragma unroll
+    for (int q = 0; q < CQ; ++q) {
+        a.Ac[(long)q * N + c] = 0ull;
+        a.Acl[(long)q * N + c] = 0ull;
     }
     const float deg = (float)(N + 1);
     double p = 0.0, dz2 = 0.0;
     if constexpr (RES) p = s[6];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float zk = a.Z[3 * c + k], zp = a.Zp[3 * c + k];
-        const float zn = zk + (float)s[k] / (RE ? a.W[c] : deg);
+        const float zn = zk[k] + (float)s[k] / (RE ? a.W[c] : deg);
         if constexpr (RES) {
-            const float dz = zn - zk;
-            const double e = (double)(dz - (zk - zp));
+            const float dz = zn - zk[k];
+            const double e = (double)(dz - (zk[k] - zp[k]));
             p += -2.0 * e * s[3 + k] + (double)deg * e * e;
             dz2 += (double)dz * dz;
         }
-        a.Zpp[3 * c + k] = zp;
-        a.Zp[3 * c + k] = zk;
+        a.Zpp[3 * c + k] = zp[k];
+        a.Zp[3 * c + k] = zk[k];
         a.Z[3 * c + k] = zn;
     }
     if constexpr (RES) {
@@ -1112,6 +1156,11 @@ static CArgs cargs(DevGraph* s)
     a.Lmask = d->d_Lmask;
     a.cnt = d->d_cnt;
     a.Ac = d->d_Ac;
+    a.Ap = nullptr;
+    a.Acl = nullptr;
+    a.Zin = nullptr;
+    a.Zpin = nullptr;
+    a.Zo = nullptr;
     a.Rpc = d->d_Rpc;
     a.Rd = d->d_Rd;
     return a;
@@ -1156,8 +1205,9 @@ static int contact_alloc(DevGraph* s)
     RC(dmalloc(s, (void**)&d->d_Lmask, sizeof(unsigned) * 2 * nts * words));
     CK(hipMemsetAsync(d->d_cnt, 0, sizeof(int) * 2 * nts, s->stream));
     CK(hipMemsetAsync(d->d_Lmask, 0, sizeof(unsigned) * 2 * nts * words, s->stream));
-    RC(dmalloc(s, (void**)&d->d_Ac, sizeof(unsigned long long) * CQ * (size_t)d->N));
-    CK(hipMemsetAsync(d->d_Ac, 0, sizeof(unsigned long long) * CQ * (size_t)d->N, s->stream));
+    RC(dmalloc(s, (void**)&d->d_Ac, sizeof(unsigned long long) * 3 * CQ * (size_t)d->N));
+    CK(hipMemsetAsync(d->d_Ac, 0, sizeof(unsigned long long) * 3 * CQ * (size_t)d->N, s->stream));
+    RC(dmalloc(s, (void**)&d->d_Zb, sizeof(float) * 2 * 3 * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_Rpc, sizeof(double) * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_Zpp, sizeof(float) * 3 * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_W, sizeof(float) * 2 * (size_t)d->N));
@@ -1176,10 +1226,25 @@ static void contact_launch(DevGraph* s, const char* name, K kernel, unsigned gri
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, a);
 }
 
+// Launch j of a run (j = c_run launches since the last run end) applies the
+// sums of launch j - 1 to Z as it reads it and adds its own to buffer j % 3
+// of three; its box-and-radius wavefronts store its Z in ring slot j % 2 and
+// clear buffer (j + 1) % 3.  res (the last iteration of every run, and every
+// single iteration): also the residual sums, then k_contact_zreduce ends the
+// run into the canonical Z arrays, so outside a run the state is as it
+// always was.
 static int contact_iteration(DevGraph* s, bool res)
 {
     DenseState* d = s->dense;
-    const CArgs a = cargs(s);
+    const int j = d->c_run;
+    const size_t AQ = (size_t)CQ * d->N, ZN = 3 * (size_t)d->N;
+    unsigned long long* A = d->d_Ac;
+    CArgs a = cargs(s);
+    a.Ac = A + (j % 3) * AQ;
+    a.Ap = j ? A + ((j - 1) % 3) * AQ : nullptr;
+    a.Acl = A + ((j + 1) % 3) * AQ;
+    a.Zin = j == 0 ? nullptr : j == 1 ? s->d_Z : d->d_Zb + ((j - 1) % 2) * ZN;
+    a.Zo = j ? d->d_Zb + (j % 2) * ZN : nullptr;
     const unsigned g = (unsigned)(d->nbs + d->nts);  // unary wavefronts, then the tiles
     const char* tn = res ? "contact_tile_res" : "contact_tile";
     const bool re = d->c_rhoe;
@@ -1195,14 +1260,21 @@ static int contact_iteration(DevGraph* s, bool res)
                 : contact_launch(s, tn, k_contact_tile<32, false, false>, g, 64, a);
     }
     CK(hipGetLastError());
-    const char* zn = res ? "contact_zreduce_res" : "contact_zreduce";
-    const unsigned zgrid = (unsigned)((d->N + CZ_THREADS - 1) / CZ_THREADS);
-    if (res) re ? contact_launch(s, zn, k_contact_zreduce<true, true>, zgrid, CZ_THREADS, a)
-                : contact_launch(s, zn, k_contact_zreduce<true, false>, zgrid, CZ_THREADS, a);
-    else re ? contact_launch(s, zn, k_contact_zreduce<false, true>, zgrid, CZ_THREADS, a)
-            : contact_launch(s, zn, k_contact_zreduce<false, false>, zgrid, CZ_THREADS, a);
-    CK(hipGetLastError());
     d->c_par ^= 1;
+    if (!res) {
+        d->c_run = j + 1;
+        return PARADMM_OK;
+    }
+    CArgs f = cargs(s);
+    f.Ac = A + (j % 3) * AQ;
+    f.Acl = A + ((j + 2) % 3) * AQ;  // launch j - 1's, read by launch j
+    f.Zin = j == 0 ? s->d_Z : d->d_Zb + (j % 2) * ZN;
+    f.Zpin = j == 0 ? s->d_Zprev : j == 1 ? s->d_Z : d->d_Zb + ((j - 1) % 2) * ZN;
+    const unsigned zgrid = (unsigned)((d->N + CZ_THREADS - 1) / CZ_THREADS);
+    re ? contact_launch(s, "contact_zreduce_res", k_contact_zreduce<true, true>, zgrid, CZ_THREADS, f)
+       : contact_launch(s, "contact_zreduce_res", k_contact_zreduce<true, false>, zgrid, CZ_THREADS, f);
+    CK(hipGetLastError());
+    d->c_run = 0;
     return PARADMM_OK;
 }
 
@@ -1224,6 +1296,7 @@ int dense_contact_enter(DevGraph* s)
     }
     RC(contact_alloc(s));
     d->c_par = 0;  // the entry writes list 0
+    d->c_run = 0;
     // one rho everywhere (a policy's raised edges make it per edge)
     d->c_rhoe = !(s->rho_uni && s->policy_raised == 0);
     const CArgs a = cargs(s);
