@@ -1180,16 +1180,17 @@ bool dense_in_contact(DevGraph* s) { return s->dense && s->dense->c_state; }
 bool dense_contact_rho_per_edge(DevGraph* s) { return s->dense && s->dense->c_state && s->dense->c_rhoe; }
 
 // Contact tile side: 64 (a quarter of the per-pair overhead of 32) once
-// the tiles are enough to fill the GPU, 32 below N = 4000 (the smaller
-// tiles spread a smaller packing over more wavefronts); PARADMM_CONTACT_TILE
-// = 32 or 64 overrides.
+// the tiles are enough to fill the GPU, 32 below N = 2500 (the smaller
+// tiles spread a smaller packing over more wavefronts; measured crossover
+// between N = 2000 and 3000).  The two give the same bits (the circle sums
+// are integers).  PARADMM_CONTACT_TILE = 32 or 64 overrides.
 static int contact_tile_side(int N)
 {
     if (const char* e = getenv("PARADMM_CONTACT_TILE")) {
         const int v = atoi(e);
         if (v == 32 || v == 64) return v;
     }
-    return N >= 4000 ? 64 : 32;
+    return N >= 2500 ? 64 : 32;
 }
 
 static int contact_alloc(DevGraph* s)

@@ -22,7 +22,7 @@ STATE = ("X", "M", "N", "U", "Z")
 @pytest.fixture(autouse=True, params=["32", "64"])
 def contact_on(monkeypatch, request):
     """Every test on both contact tile sides (the default picks 32 below
-    N = 4000)."""
+    N = 2500)."""
     monkeypatch.delenv("PARADMM_LAYOUT", raising=False)
     monkeypatch.delenv("PARADMM_CONTACT", raising=False)
     monkeypatch.delenv("PARADMM_DENSE_TILE", raising=False)
