@@ -83,6 +83,14 @@ def load_ref():
         lib.ref_square_run.argtypes = [_I, _U64, _D, _D, _D, _I, _DP]
         lib.ref_triangle_run.restype = _I
         lib.ref_triangle_run.argtypes = [_I, _U64, _D, _D, _I, _DP]
+        lib.ref_square_run_omp.restype = _I
+        lib.ref_square_run_omp.argtypes = [_I, _U64, _D, _D, _D, _I, _I, _I, _DP]
+        lib.ref_triangle_run_omp.restype = _I
+        lib.ref_triangle_run_omp.argtypes = [_I, _U64, _D, _D, _I, _I, _I, _DP]
+        lib.ref_square_warm_start.restype = _I
+        lib.ref_square_warm_start.argtypes = [_I, _D, _D, _DP, _DP, _I, _DP, _DP, _DP, _DP, _DP]
+        lib.ref_triangle_warm_start.restype = _I
+        lib.ref_triangle_warm_start.argtypes = [_I, _DP, _DP, _I, _DP, _DP, _DP, _DP, _DP]
         for name in ("ref_prox_pair2", "ref_prox_wall2", "ref_prox_radius2"):
             f = getattr(lib, name)
             f.restype = None

@@ -16,6 +16,16 @@ Fixtures (numpy .npz, float64):
                        Z after iterations 1..200.
   prox2_kat.npz        the reference's own dim-2 operators (packing/host.c) on
                        seeded random inputs: inputs and outputs.
+  warm_square9.npz     initialize_X_N_Z_M_U_optimal (core/paradmm.c:266-375) of
+  warm_triangle7.npz   the reference on the square (oracle dim-3 operators) and
+                       triangle (its own operators) graphs: random Zinit, Uinit,
+                       maxiter 3; X, M, N, U, Z out.
+  omp_square40.npz     the reference's OpenMP passes (the five update*OpenMP
+  omp_triangle12.npz   calls, packing/host.c:237-244, on 4 threads) and its
+                       one-region updateGOpenMP (core/paradmm.c:949-1092): Z
+                       after 20 iterations of each.
+
+`python make_golden.py warm_omp` writes only the last four.
 """
 import ctypes as C
 import os
@@ -89,8 +99,55 @@ def main():
     np.savez_compressed(os.path.join(HERE, "prox2_kat.npz"), pair_n=pair_n, pair_rho=pair_rho, pair_x=pair_x,
                         walls=walls, wall_n=wall_n, wall_rho=wall_rho, wall_k=wall_k, wall_x=wall_x,
                         rad_n=rad_n, rad_rho=rad_rho, rad_a=rad_a, rad_x=rad_x)
+    write_warm_omp(ref)
     print("golden fixtures written to", HERE)
 
 
+def write_warm_omp(ref):
+    rng = np.random.default_rng(2024)
+    nc, rho, maxiter = 9, 3.0, 3
+    ne = nc + nc + nc * (nc - 1) // 2 * 2
+    zi = np.ascontiguousarray(np.column_stack([rng.uniform(0, 1, nc), rng.uniform(0, 1, nc),
+                                               rng.uniform(0, 0.3, nc)]).ravel())
+    ui = rng.uniform(-0.1, 0.1, 3 * ne)
+    out = {k: np.zeros(3 * ne) for k in "XMNU"}
+    out["Z"] = np.zeros(3 * nc)
+    ref.ref_square_warm_start(nc, rho, 1.0, po.dptr(zi), po.dptr(ui), maxiter, *[po.dptr(out[k]) for k in "XMNUZ"])
+    np.savez_compressed(os.path.join(HERE, "warm_square9.npz"), n=nc, rho=rho, a=1.0, maxiter=maxiter,
+                        zinit=zi, uinit=ui, **out)
+
+    nc = 7
+    nv, ne = 2 * nc, nc + 3 * nc * 2 + nc * (nc - 1) // 2 * 4
+    zi = rng.uniform(-1, 2, 2 * nv)
+    ui = rng.uniform(-1, 1, 2 * ne)
+    out = {k: np.zeros(2 * ne) for k in "XMNU"}
+    out["Z"] = np.zeros(2 * nv)
+    ref.ref_triangle_warm_start(nc, po.dptr(zi), po.dptr(ui), maxiter, *[po.dptr(out[k]) for k in "XMNUZ"])
+    np.savez_compressed(os.path.join(HERE, "warm_triangle7.npz"), n=nc, maxiter=maxiter, zinit=zi, uinit=ui,
+                        **out)
+
+    nc, seed, rho, alpha, iters = 40, 5, 10.0, 0.5, 20
+    z5 = np.zeros((iters, 3 * nc))
+    ref.ref_square_run_omp(nc, seed, rho, alpha, 1.0, iters, 1, 4, po.dptr(z5))
+    zg = np.zeros((1, 3 * nc))
+    ref.ref_square_run_omp(nc, seed, rho, alpha, 1.0, iters, 2, 4, po.dptr(zg))
+    np.savez_compressed(os.path.join(HERE, "omp_square40.npz"), n=nc, seed=seed, rho=rho, alpha=alpha, a=1.0,
+                        iters=iters, z_passes=z5, z_region=zg[0])
+
+    nc, seed = 12, 7
+    z5 = np.zeros((iters, 4 * nc))
+    ref.ref_triangle_run_omp(nc, seed, -1.0, 2.0, iters, 1, 4, po.dptr(z5))
+    zg = np.zeros((1, 4 * nc))
+    ref.ref_triangle_run_omp(nc, seed, -1.0, 2.0, iters, 2, 4, po.dptr(zg))
+    np.savez_compressed(os.path.join(HERE, "omp_triangle12.npz"), n=nc, seed=seed, lo=-1.0, hi=2.0, iters=iters,
+                        z_passes=z5, z_region=zg[0])
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "warm_omp":
+        r = po.load_ref()
+        if r is None:
+            raise SystemExit("oracle/_ref is not built (needs the reference sources): make -C oracle")
+        write_warm_omp(r)
+    else:
+        main()

@@ -249,3 +249,57 @@ def test_radius_a1_at_zero_is_finite(oracle):
     """n = 0 with a = 1: the curvature term is identically zero; the result is 1/rho."""
     lib = oracle.load()
     assert lib.or_max_alpha_radius(0.0, 4.0, 1.0) == pytest.approx(0.25)
+
+
+def _quality_of(oracle, z):
+    """or_square_quality on a square graph whose Z is set to z (rows cx, cy, r)."""
+    z = np.asarray(z, dtype=np.float64)
+    g = oracle.Graph.square(z.shape[0], 0)
+    g.array("Z")[:] = z.ravel()
+    return g.square_quality(z.shape[0])
+
+
+def test_square_quality_two_circles_closed_form(oracle):
+    """Two circles 0.3 apart with radii 0.2 and 0.15 overlap by exactly
+    0.05; both lie inside the square, so the container violation is 0 and
+    the objective is 0.35.  Moving the second circle to distance 0.5 leaves
+    no overlap (the measure is clamped at 0, not negative)."""
+    ov, bx, sr = _quality_of(oracle, [[0.3, 0.5, 0.2], [0.6, 0.5, 0.15]])
+    assert ov == pytest.approx(0.05, abs=1e-15)
+    assert bx == 0.0
+    assert sr == pytest.approx(0.35, abs=1e-15)
+    ov, bx, sr = _quality_of(oracle, [[0.25, 0.5, 0.2], [0.75, 0.5, 0.15]])
+    assert ov == 0.0
+    # a diagonal pair: distance 5 * 0.06 (a 3-4-5 triangle), radii sum 0.31
+    ov, _, _ = _quality_of(oracle, [[0.2, 0.2, 0.16], [0.38, 0.44, 0.15]])
+    assert ov == pytest.approx(0.01, abs=1e-15)
+
+
+@pytest.mark.parametrize("circle,viol", [
+    ((0.05, 0.5, 0.1), 0.05),    # left wall: r - cx
+    ((0.97, 0.5, 0.1), 0.07),    # right wall: cx + r - 1
+    ((0.5, 0.02, 0.1), 0.08),    # bottom wall: r - cy
+    ((0.5, 0.95, 0.1), 0.05),    # top wall: cy + r - 1
+    ((0.5, 0.5, -0.01), 0.01),   # negative radius: -r
+    ((0.5, 0.5, 0.5), 0.0),      # touching all four walls
+])
+def test_square_quality_container_closed_form(oracle, circle, viol):
+    """One circle crossing one wall by a known amount (the other circle well
+    inside and apart): the container violation is that amount."""
+    ov, bx, sr = _quality_of(oracle, [circle, (0.9, 0.9, 0.0)] if circle[2] != 0.5 else [circle])
+    assert bx == pytest.approx(viol, abs=1e-15)
+
+
+def test_square_quality_maxima_over_many_pairs(oracle):
+    """Among many circles the reported overlap is the largest one: a row of
+    touching circles (overlap 0) with one pair pushed together by 0.013,
+    and the container violation is the largest of several."""
+    r = 0.04
+    z = [[0.1 + 2 * r * k, 0.5, r] for k in range(10)]
+    z[6][0] -= 0.013
+    z[0][1] = 0.03   # bottom violation r - cy = 0.01
+    z[9][1] = 0.975  # top violation cy + r - 1 = 0.015
+    ov, bx, sr = _quality_of(oracle, z)
+    assert ov == pytest.approx(0.013, abs=1e-14)
+    assert bx == pytest.approx(0.015, abs=1e-15)
+    assert sr == pytest.approx(10 * r, abs=1e-15)

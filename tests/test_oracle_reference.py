@@ -92,3 +92,96 @@ def test_triangle_live_reference(oracle, nc, seed):
     g = oracle.Graph.triangle(nc, seed)
     h = g.iterate(iters, record=True)
     assert np.max(np.abs(h - hr)) <= 1e-12 * max(1.0, np.max(np.abs(hr)))
+
+
+def _warm_oracle(oracle, f, kind):
+    if kind == "square":
+        g = oracle.Graph.square(int(f["n"]), 0, float(f["rho"]), 1.0, float(f["a"]))
+    else:
+        g = oracle.Graph.triangle(int(f["n"]), 0)
+    zi = np.ascontiguousarray(f["zinit"])
+    ui = np.ascontiguousarray(f["uinit"])
+    g.lib.or_warm_start(g.h, oracle.dptr(zi), oracle.dptr(ui), int(f["maxiter"]))
+    return g
+
+
+@pytest.mark.parametrize("fixture,kind", [("warm_square9.npz", "square"), ("warm_triangle7.npz", "triangle")])
+def test_warm_start_matches_reference(oracle, fixture, kind):
+    """or_warm_start against initialize_X_N_Z_M_U_optimal (core/paradmm.c:266-375)
+    of the reference compiled from source: every array it writes."""
+    f = load(fixture)
+    g = _warm_oracle(oracle, f, kind)
+    for k in "XMNUZ":
+        scale = max(1.0, float(np.max(np.abs(f[k]))))
+        assert np.max(np.abs(g.array(k) - f[k])) <= TOL * scale, k
+    # the sweeps really moved U away from Uinit and X stayed the broadcast Z
+    assert np.mean(f["U"] != f["uinit"]) > 0.5
+
+
+@pytest.mark.parametrize("nc,rho,maxiter", [(6, 3.0, 1), (11, 50.0, 5)])
+def test_warm_start_live_reference(oracle, nc, rho, maxiter):
+    ref = oracle.load_ref()
+    if ref is None:
+        pytest.skip("reference sources not present (oracle/_ref not built)")
+    rng = np.random.default_rng(nc)
+    g = oracle.Graph.square(nc, 1, rho, 1.0, 1.0)
+    zi = np.ascontiguousarray(np.column_stack([rng.uniform(0, 1, nc), rng.uniform(0, 1, nc),
+                                               rng.uniform(0, 0.3, nc)]).ravel())
+    ui = rng.uniform(-0.1, 0.1, 3 * g.nedge)
+    out = {k: np.zeros(3 * g.nedge) for k in "XMNU"}
+    out["Z"] = np.zeros(3 * nc)
+    ref.ref_square_warm_start(nc, rho, 1.0, oracle.dptr(zi), oracle.dptr(ui), maxiter,
+                              *[oracle.dptr(out[k]) for k in "XMNUZ"])
+    g.lib.or_warm_start(g.h, oracle.dptr(zi), oracle.dptr(ui), maxiter)
+    for k in "XMNUZ":
+        assert np.max(np.abs(g.array(k) - out[k])) <= TOL, k
+
+
+def test_omp_passes_match_reference_square(oracle):
+    """or_iterate_omp (the timed CPU baseline's loop) against the reference's
+    five update*OpenMP passes and its one-region updateGOpenMP: the -O2
+    checker build to 1e-12 after 20 iterations, the -Ofast baseline build
+    (reassociated sums, so its rounding differs and grows with the
+    iterations) to 1e-12 after 5."""
+    f = load("omp_square40.npz")
+    n, seed, rho, alpha = int(f["n"]), int(f["seed"]), float(f["rho"]), float(f["alpha"])
+    assert np.max(np.abs(f["z_passes"][-1] - f["z_region"])) <= TOL
+    g = oracle.Graph.square(n, seed, rho, alpha, 1.0)
+    g.iterate_omp(int(f["iters"]), 4)
+    assert np.max(np.abs(g.array("Z") - f["z_passes"][-1])) <= TOL
+    # the serial oracle gives the same Z: the schedule does not change results
+    s = oracle.Graph.square(n, seed, rho, alpha, 1.0)
+    h = s.iterate(int(f["iters"]), record=True)
+    assert np.max(np.abs(h - f["z_passes"])) <= TOL
+    fast = oracle.Graph.square(n, seed, rho, alpha, 1.0, fast=True)
+    fast.iterate_omp(5, 4)
+    assert np.max(np.abs(fast.array("Z") - f["z_passes"][4])) <= TOL
+
+
+def test_omp_passes_match_reference_triangle(oracle):
+    f = load("omp_triangle12.npz")
+    g = oracle.Graph.triangle(int(f["n"]), int(f["seed"]), float(f["lo"]), float(f["hi"]))
+    g.iterate_omp(int(f["iters"]), 4)
+    scale = max(1.0, float(np.max(np.abs(f["z_region"]))))
+    assert np.max(np.abs(g.array("Z") - f["z_passes"][-1])) <= TOL * scale
+    assert np.max(np.abs(g.array("Z") - f["z_region"])) <= TOL * scale
+
+
+@pytest.mark.parametrize("nc,seed,rho,alpha,threads", [(7, 3, 1.0, 1.0, 3), (60, 9, 1.0, 1.0, 8)])
+def test_omp_live_reference(oracle, nc, seed, rho, alpha, threads):
+    ref = oracle.load_ref()
+    if ref is None:
+        pytest.skip("reference sources not present (oracle/_ref not built)")
+    iters = 30
+    for mode in (1, 2):
+        hr = np.zeros((iters, nc * 3))
+        ref.ref_square_run_omp(nc, seed, rho, alpha, 1.0, iters, mode, threads, oracle.dptr(hr))
+        zr = hr[-1] if mode == 1 else hr[0]
+        g = oracle.Graph.square(nc, seed, rho, alpha, 1.0)
+        g.iterate_omp(iters, threads)
+        assert np.max(np.abs(g.array("Z") - zr)) <= TOL, mode
+        fast = oracle.Graph.square(nc, seed, rho, alpha, 1.0, fast=True)
+        hr5 = np.zeros((5, nc * 3))
+        ref.ref_square_run_omp(nc, seed, rho, alpha, 1.0, 5, 1, threads, oracle.dptr(hr5))
+        fast.iterate_omp(5, threads)
+        assert np.max(np.abs(fast.array("Z") - hr5[-1])) <= TOL
