@@ -456,6 +456,8 @@ namespace pdm {
 void free_dev(DevGraph* s)
 {
     if (!s) return;
+    // nothing may still run on the stream when its graphs and buffers go
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto& r : s->runs) {
         if (r.exec) (void)hipGraphExecDestroy(r.exec);
         if (r.graph) (void)hipGraphDestroy(r.graph);
@@ -1092,8 +1094,19 @@ static int run_variant(const DevGraph* s)
 static int captured_run(DevGraph* s, int n, hipGraphExec_t* out)
 {
     const int variant = run_variant(s);
+    // A run launched earlier may still execute on the stream: wait for it
+    // before destroying any executable graph (launches are asynchronous).
+    bool synced = false;
+    auto drain = [&]() -> int {
+        if (!synced) {
+            CK(hipStreamSynchronize(s->stream));
+            synced = true;
+        }
+        return PARADMM_OK;
+    };
     for (auto it = s->runs.begin(); it != s->runs.end();) {
         if ((it->variant | RUN_PARITY) != (variant | RUN_PARITY)) {
+            RC(drain());
             CK(hipGraphExecDestroy(it->exec));
             CK(hipGraphDestroy(it->graph));
             it = s->runs.erase(it);
@@ -1107,6 +1120,7 @@ static int captured_run(DevGraph* s, int n, hipGraphExec_t* out)
     if (s->runs.size() >= 4) {  // the oldest run other than the main length
         auto victim = s->runs.begin();
         if (victim->iters == s->graph_iters && s->runs.size() > 1) ++victim;
+        RC(drain());
         CK(hipGraphExecDestroy(victim->exec));
         CK(hipGraphDestroy(victim->graph));
         s->runs.erase(victim);

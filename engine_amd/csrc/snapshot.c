@@ -107,16 +107,28 @@ int paradmm_load_state(graph *G, const char *path)
         return PARADMM_ERR_SHAPE;
     }
     const size_t ed = (size_t)G->num_edges * G->num_dims, vd = (size_t)G->num_vars * G->num_dims;
+    const size_t ne = (size_t)G->num_edges;
+    /* the payload must be complete before any array of the graph is touched:
+     * a short file leaves the graph as it was */
+    const long payload_at = ftell(f);
+    const long long need = (long long)sizeof(double) * (long long)(4 * ed + vd + 2 * ne);
+    long end = -1;
+    if (payload_at >= 0 && fseek(f, 0, SEEK_END) == 0) end = ftell(f);
+    if (end < 0 || (long long)(end - payload_at) != need || fseek(f, payload_at, SEEK_SET) != 0) {
+        fclose(f);
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_load_state: %s is truncated or has trailing bytes", path);
+        return PARADMM_ERR_ARG;
+    }
     double *arrs[4] = {G->X, G->M, G->N, G->U};
     int ok = 1;
     for (int a = 0; ok && a < 4; a++) ok = fread(arrs[a], sizeof(double), ed, f) == ed;
     ok = ok && fread(G->Z, sizeof(double), vd, f) == vd;
-    ok = ok && fread(G->RHOS, sizeof(double), (size_t)G->num_edges, f) == (size_t)G->num_edges;
-    ok = ok && fread(G->ALPHAS, sizeof(double), (size_t)G->num_edges, f) == (size_t)G->num_edges;
+    ok = ok && fread(G->RHOS, sizeof(double), ne, f) == ne;
+    ok = ok && fread(G->ALPHAS, sizeof(double), ne, f) == ne;
     fclose(f);
     pdm_host_written(G);
     if (!ok) {
-        pdm_set_error(PARADMM_ERR_ARG, "paradmm_load_state: %s is truncated", path);
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_load_state: reading %s failed", path);
         return PARADMM_ERR_ARG;
     }
     return PARADMM_OK;

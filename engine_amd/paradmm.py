@@ -6,8 +6,11 @@ points (startG, addNode, initialize_*, update*) and the MI355X extension
 (paradmm_gpu_*).  It raises at import time if the library is not built; there
 is no fallback.
 """
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -73,6 +76,25 @@ class ParadmmError(RuntimeError):
 
 
 _lib = None
+
+# Every live handle, so that they are released while the HIP runtime (and a
+# profiler attached to it) is still up: at interpreter exit an atexit hook
+# closes batches, then policies, then graphs, and __del__ makes no library
+# call once the interpreter is finalising.
+_live = {"batch": weakref.WeakSet(), "mlp": weakref.WeakSet(), "graph": weakref.WeakSet()}
+
+
+def release_all():
+    """Closes every live Batch, RhoMLP and Graph (batches first: they use graphs)."""
+    for kind in ("batch", "mlp", "graph"):
+        for obj in list(_live[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+atexit.register(release_all)
 
 
 def load():
@@ -202,6 +224,7 @@ class RhoMLP:
             p = np.ascontiguousarray(params, dtype=np.float32)
             check(self.lib.paradmm_rho_mlp_create(C.byref(h), _fptr(p), p.size, seed), "paradmm_rho_mlp_create")
         self.h = h
+        _live["mlp"].add(self)
 
     def params(self):
         p = np.zeros(self.NPARAMS, dtype=np.float32)
@@ -237,6 +260,8 @@ class RhoMLP:
             self.h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -265,6 +290,7 @@ class Graph:
         self.ptr = C.pointer(self.g)
         self.lib.startG(self.ptr, ndims)
         self._keep = []  # parameter blocks referenced by factors
+        _live["graph"].add(self)
 
     # ---- construction
     def add_factor(self, fn_addr, variables, params=None):
@@ -447,6 +473,8 @@ class Graph:
             self.ptr = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -473,6 +501,7 @@ class Batch:
         h = C.c_void_p()
         check(self.lib.paradmm_batch_create(C.byref(h), arr, len(self.graphs)), "paradmm_batch_create")
         self.h = h
+        _live["batch"].add(self)
 
     def configure(self, tile, blocks_per_cu=1, barrier=1):
         check(self.lib.paradmm_batch_configure(self.h, tile, blocks_per_cu, barrier), "paradmm_batch_configure")
@@ -519,6 +548,8 @@ class Batch:
             self.h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

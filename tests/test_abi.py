@@ -182,9 +182,17 @@ def test_snapshot_round_trip(pdm, tmp_path):
     with pytest.raises(pdm.ParadmmError, match="another graph structure"):
         other.load_state(path)
     raw = open(path, "rb").read()
-    open(path, "wb").write(raw[:-100])
-    with pytest.raises(pdm.ParadmmError, match="truncated"):
-        g.load_state(path)
+    # a truncated file (cut anywhere in the payload) or one with trailing
+    # bytes is refused and leaves every array of the graph as it was
+    marker = {n: rng.uniform(5, 6, size=g.array(n).shape) for n in saved}
+    for n in saved:
+        g.array(n)[:] = marker[n]
+    for bad in (raw[:-100], raw[:len(raw) // 2], raw[:100], raw + b"\0" * 8):
+        open(path, "wb").write(bad)
+        with pytest.raises(pdm.ParadmmError, match="truncated"):
+            g.load_state(path)
+        for n in saved:
+            assert np.array_equal(g.array(n), marker[n]), n
     open(path, "wb").write(b"not a snapshot at all" * 4)
     with pytest.raises(pdm.ParadmmError, match="not a snapshot"):
         g.load_state(path)
