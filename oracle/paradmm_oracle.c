@@ -305,6 +305,29 @@ double or_iterate_omp(or_graph *g, int iters, int nthreads)
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-6 * (double)(t1.tv_usec - t0.tv_usec);
 }
 
+void or_iterate_par(or_graph *g, int iters, int nthreads, double *zhist)
+{
+    /* or_iterate through the OpenMP passes: each variable's sums run in one
+     * thread in list order, so the results equal the serial ones bit for
+     * bit (tests/test_oracle_reference.py); Zprev kept, Z recorded. */
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    size_t lv = (size_t)g->ndims * g->nvar;
+    for (int it = 0; it < iters; it++) {
+        memcpy(g->Zprev, g->Z, sizeof(double) * lv);
+        or_update_X_omp(g);
+        or_update_M_omp(g);
+        or_update_Z_omp(g);
+        or_update_U_omp(g);
+        or_update_N_omp(g);
+        g->iter++;
+        if (zhist) memcpy(zhist + (size_t)it * lv, g->Z, sizeof(double) * lv);
+    }
+}
+
 /* ------------------------------------------------------------ prox operators */
 
 double or_max_alpha_radius(double n, double rho, double a)

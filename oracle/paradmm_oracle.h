@@ -73,6 +73,9 @@ void or_update_N_omp(or_graph *g);
 /* iters x the five OpenMP calls, as the reference's OpenMP driver loop (packing/host.c:237-244).
  * Returns wall seconds measured with gettimeofday around the loop only. */
 double or_iterate_omp(or_graph *g, int iters, int nthreads);
+/* or_iterate (Zprev kept, Z recorded) through the OpenMP passes on nthreads:
+ * the same results, faster, for the checks at large sizes. */
+void or_iterate_par(or_graph *g, int iters, int nthreads, double *zhist);
 
 /* ---- packing prox operators ---- */
 /* dim-2 operators of the reference demo (packing/host.c) */

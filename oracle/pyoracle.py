@@ -36,6 +36,7 @@ def _bind(lib):
         "or_update_N": (None, [_P]),
         "or_iterate": (None, [_P, _I, _DP]),
         "or_iterate_omp": (_D, [_P, _I, _I]),
+        "or_iterate_par": (None, [_P, _I, _I, _DP]),
         "or_max_alpha_radius": (_D, [_D, _D, _D]),
         "or_splitmix64": (_U64, [C.POINTER(_U64)]),
         "or_uniform01": (_D, [C.POINTER(_U64)]),
@@ -183,6 +184,16 @@ class Graph:
 
     def iterate_omp(self, iters, nthreads):
         return self.lib.or_iterate_omp(self.h, iters, nthreads)
+
+    def iterate_par(self, iters, nthreads=0, record=False):
+        """iterate() through the OpenMP passes (same results; nthreads 0:
+        the OpenMP default)."""
+        if record:
+            hist = np.zeros((iters, self.nvar * self.ndims))
+            self.lib.or_iterate_par(self.h, iters, nthreads, dptr(hist))
+            return hist
+        self.lib.or_iterate_par(self.h, iters, nthreads, None)
+        return None
 
     def residuals(self):
         p, d = C.c_double(), C.c_double()

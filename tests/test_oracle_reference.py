@@ -185,3 +185,15 @@ def test_omp_live_reference(oracle, nc, seed, rho, alpha, threads):
         ref.ref_square_run_omp(nc, seed, rho, alpha, 1.0, 5, 1, threads, oracle.dptr(hr5))
         fast.iterate_omp(5, threads)
         assert np.max(np.abs(fast.array("Z") - hr5[-1])) <= TOL
+
+
+def test_parallel_iterate_equals_serial(oracle):
+    """or_iterate_par (the OpenMP passes, used by the checks at full size)
+    gives the serial oracle's bits: Z history, U and Zprev."""
+    a = oracle.Graph.square(300, 4, 1.0, 1.0, 1.0)
+    b = oracle.Graph.square(300, 4, 1.0, 1.0, 1.0)
+    ha = a.iterate(30, record=True)
+    hb = b.iterate_par(30, 4, record=True)
+    assert np.array_equal(ha, hb)
+    for k in ("X", "U", "N", "M", "Zprev"):
+        assert np.array_equal(a.array(k), b.array(k)), k
