@@ -11,18 +11,30 @@ N = 5000).  These tests check it where it runs:
     (PARADMM_CONTACT=0);
   * teacher-forced steps: the oracle's FP64 state uploaded, four device
     iterations (after-upload pass, contact entry, two contact iterations),
-    every array compared with four oracle iterations;
+    every array compared with four oracle iterations, and with the per-edge
+    path (PARADMM_CONTACT=0) from the same upload;
   * for each size, the first iteration whose Z leaves the oracle by more
     than 1e-5 and the number of pairs whose no-overlap test on the next
     prox input differs from the oracle's (overlap-test flips), printed.
 
 A pair the tile kernel failed to list would leave the contact state with
-M = Z and X = N on that pair, which the teacher-forced X comparison catches.
+M = Z and X = N on that pair: its overlap test would differ from the
+oracle's (a flip) and its X would be off by the whole projection step.
+
+At rho = 1 the early iterations amplify any perturbation about eightfold
+per iteration on the edges of deeply overlapping pairs, in FP64 as well:
+the oracle started from the same state rounded to fp32 (the least change
+any fp32 engine starts from) ends four iterations later up to 1e-3 away on
+a few edges.  So per-edge arrays are judged against that measured
+sensitivity (max error <= 30 x it + 1e-5), on their 99.9th percentile
+(<= 1e-5), and against the per-edge path (<= 3 x its error); Z, an average
+over N + 1 edges, must stay within 1e-5 outright.
 """
 import numpy as np
 import pytest
 
 from gpu_helpers import OR_NAME, max_rel, square_pair, trajectory_error
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
@@ -90,45 +102,72 @@ def test_contact_full_size_untethered(monkeypatch, nc, iters, starts):
     assert ep[:5].max() <= 1e-5
     assert cp[:5].max() <= 1e-5
     # the tile kernel ran (not the per-edge path)
+    monkeypatch.setenv("PARADMM_CONTACT", "1")
     gc.sync_from_host()
     assert "contact_tile" in _kernels(gc)
     gc.close()
     o.close()
 
 
+def _abs_err(a, b):
+    d = np.abs(np.asarray(a, dtype=np.float64) - np.asarray(b, dtype=np.float64))
+    return float(d.max()), float(np.quantile(d, 0.999))
+
+
 @pytest.mark.parametrize("nc,iters,starts", CASES, ids=[str(c[0]) for c in CASES])
 def test_contact_full_size_teacher_forced(monkeypatch, nc, iters, starts):
     monkeypatch.delenv("PARADMM_LAYOUT", raising=False)
-    monkeypatch.delenv("PARADMM_CONTACT", raising=False)
     monkeypatch.delenv("PARADMM_CONTACT_TILE", raising=False)
-    g, o = square_pair(nc, 0)
+    steps = 4
+    paths = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("PARADMM_CONTACT", mode)
+        paths[mode], _ = square_pair(nc, 0)
+    _, o = square_pair(nc, 0)
     done = 0
     rows = []
     for k in starts:
         o.iterate_par(k - done, THREADS)
         done = k
         snap = {n: o.array(OR_NAME[n]).copy() for n in STATE}
+        zprev = o.array("Zprev").copy()
+        err = {}
+        for mode, g in paths.items():
+            monkeypatch.setenv("PARADMM_CONTACT", mode)
+            for n in STATE:
+                g.array(n)[:] = snap[n]
+            g.sync_from_host()
+            g.iterate(steps)
+            g.sync_to_host()
+        o.iterate_par(steps, THREADS)
+        # the oracle from the same state rounded to fp32: the sensitivity
+        b = po.Graph.square(nc, 0)
         for n in STATE:
-            g.array(n)[:] = snap[n]
-        g.sync_from_host()
-        g.iterate(4)
-        g.sync_to_host()
-        o.iterate_par(4, THREADS)
-        errs = {n: max_rel(g.array(n), o.array(OR_NAME[n])) for n in STATE}
-        fl, act = _flips(g, o, nc)
-        rows.append((k, errs, fl, act))
-        print(f"\nN={nc} from oracle iteration {k}: four-step errors "
-              + " ".join(f"{n} {errs[n]:.1e}" for n in STATE) + f"; overlap-test flips {fl} of {act} active pairs")
+            b.array(OR_NAME[n])[:] = snap[n].astype(np.float32).astype(np.float64)
+        b.array("Zprev")[:] = zprev
+        b.iterate_par(steps, THREADS)
+        sens = {n: _abs_err(b.array(OR_NAME[n]), o.array(OR_NAME[n]))[0] for n in STATE}
+        b.close()
+        for mode, g in paths.items():
+            err[mode] = {n: _abs_err(g.array(n), o.array(OR_NAME[n])) for n in STATE}
+        zrel = max_rel(paths["1"].array("Z"), o.array("Z"))
+        fl, act = _flips(paths["1"], o, nc)
+        rows.append((k, err, sens, zrel, fl, act))
+        print(f"\nN={nc} from oracle iteration {k}, {steps} steps: max / 99.9% abs error contact | per-edge | "
+              "FP64 sensitivity: " + "; ".join(
+                  f"{n} {err['1'][n][0]:.1e}/{err['1'][n][1]:.1e} | {err['0'][n][0]:.1e} | {sens[n]:.1e}"
+                  for n in STATE) + f"; Z rel {zrel:.1e}; overlap-test flips {fl} of {act} active pairs")
         for n in STATE:
             o.array(OR_NAME[n])[:] = snap[n]
-        if k == starts[-1]:
-            break
-    for k, errs, fl, act in rows:
-        assert errs["Z"] <= 1e-5, k
-        for n in ("X", "M", "N"):
-            assert errs[n] <= 2e-5, (k, n)
-        # U = M - Z cancels: its error is that of M and Z relative to the small U
-        assert errs["U"] <= 1e-4, k
+        o.array("Zprev")[:] = zprev
+    for k, err, sens, zrel, fl, act in rows:
+        assert zrel <= 1e-5, k
+        for n in STATE:
+            mx, q = err["1"][n]
+            assert q <= 1e-5, (k, n, q)
+            assert mx <= 30.0 * sens[n] + 1e-5, (k, n, mx, sens[n])
+            assert mx <= 3.0 * err["0"][n][0] + 1e-6, (k, n, mx, err["0"][n][0])
         assert fl <= max(2, act // 1000), k
-    g.close()
+    for g in paths.values():
+        g.close()
     o.close()
