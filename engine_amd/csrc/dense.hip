@@ -81,6 +81,7 @@ struct DenseState {
     int c_run = 0;                       // contact launches since the last run end
     double* d_Rpc = nullptr;   // [N] primal residual per circle
     float* d_Zpp = nullptr;    // [N][3] Z two iterations back
+    unsigned* d_flag = nullptr; // a circle sum lost a value (non-finite or out of range) since the state was entered
 };
 
 // one workgroup per tile of side T; UNI: one rho and one alpha for every edge
@@ -157,8 +158,11 @@ __global__ __launch_bounds__(32 * ZR_SLICES) void k_dense_zreduce(int N, int nb,
 }
 
 // primal = sum Rp[0..nt), dual = sum Rd[0..N): one workgroup, fixed order.
+// flag (or null): set when the contact state's sums lost a value; the
+// residuals are then NaN
 __global__ __launch_bounds__(256) void k_dense_res_final(int nt, const double* __restrict__ Rp, int N,
-                                                         const double* __restrict__ Rd, double* __restrict__ out)
+                                                         const double* __restrict__ Rd, double* __restrict__ out,
+                                                         const unsigned* __restrict__ flag)
 {
     __shared__ double sp[256], sq[256];
     double p = 0.0, q = 0.0;
@@ -175,9 +179,16 @@ __global__ __launch_bounds__(256) void k_dense_res_final(int nt, const double* _
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[0] = sp[0];
-        out[1] = sq[0];
+        const bool lost = flag && *flag;
+        out[0] = lost ? __builtin_nan("") : sp[0];
+        out[1] = lost ? __builtin_nan("") : sq[0];
     }
+}
+
+__global__ void k_fill_u32(unsigned* p, int n, unsigned v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ------------------------------------------------------------------ contact state
@@ -201,16 +212,22 @@ __global__ __launch_bounds__(256) void k_dense_res_final(int nt, const double* _
 // where a and g are 0 on every other edge.  The primal residual
 // sum |X - Z_{k+1}|^2 is then, per circle, |g - e|^2 summed over all N + 1
 // edges with e = Z_{k+1} - 2 Z_k + Z_{k-1}: S2 - 2 e.S1 + (N + 1)|e|^2.
-// The sums run in 2^-31 fixed point (64-bit LDS atomics inside a tile, then
-// 64-bit global atomics of the tile's nonzero sums into the circle's
-// accumulator), so they do not depend on the order the lanes or the tiles
-// add them: runs are bitwise reproducible.
+// The sums run in fixed point, each value rounded to the nearest multiple
+// of 2^-31 (64-bit LDS atomics inside a tile, then 64-bit global atomics of
+// the tile's nonzero sums into the circle's accumulator), so they do not
+// depend on the order the lanes or the tiles add them: runs are bitwise
+// reproducible.  |g|^2 is held exactly to 2^-62 as two sums, its multiple
+// of 2^-31 and the remainder in units of 2^-62, so the primal residual does
+// not lose the small terms of a nearly converged iterate.  A value the
+// fixed point cannot hold (non-finite, or |v| >= 2^30) is not added: it
+// sets the state's flag, and the residuals read NaN from then on.
 //
 // The per-pair test replaces 24 bytes of M per edge read and written by
 // ~10 VALU operations; the state is Z, Z_prev, Z_prev2 and the lists.
 
-constexpr int CQ = 7;         // per-circle sums: a (3), g (3), |g|^2
+constexpr int CQ = 8;         // per-circle sums: a (3), g (3), |g|^2 (multiple of 2^-31, remainder in 2^-62)
 constexpr double CFIX_INV = 1.0 / 2147483648.0;  // 2^-31
+constexpr double CFIX2_INV = CFIX_INV * CFIX_INV; // 2^-62
 
 struct CArgs {
     int N;
@@ -238,18 +255,56 @@ struct CArgs {
     float* Zo;             // Zin + Ap / (N + 1): this launch's Z, stored by the box-and-radius wavefronts
     double* Rpc;
     double* Rd;
+    unsigned* flag;        // set when a circle sum could not take a value
 };
 
-// v in 2^-31 fixed point: v = k + f with k = rint(v) and f = v - k exact
-// (|f| <= 1/2), so v 2^31 = k 2^31 + f 2^31 needs no FP64 (|v| < 2^31)
-__device__ __forceinline__ long long cfix(float v)
+// v rounded to the nearest multiple of 2^-31, in units of 2^-31:
+// v = k + f with k = rint(v) and f = v - k exact (|f| <= 1/2), so
+// v 2^31 = k 2^31 + rint(f 2^31) needs no FP64.  A value outside
+// |v| < 2^30 (NaN included) gives 0 and sets bad.
+__device__ __forceinline__ long long cfix(float v, unsigned& bad)
 {
+    if (!(fabsf(v) < 1073741824.0f)) {
+        bad = 1u;
+        return 0;
+    }
     const float k = rintf(v);
     const float f = v - k;
-    return (long long)(int)k * 2147483648LL + (long long)(int)(f * 2147483648.0f);
+    return (long long)(int)k * 2147483648LL + (long long)(int)rintf(f * 2147483648.0f);
 }
 
-__device__ __forceinline__ void cadd(unsigned long long* acc, float v) { atomicAdd(acc, (unsigned long long)cfix(v)); }
+// v >= 0 exactly to 2^-62 as hi 2^-31 + lo 2^-62: hi = the multiple of
+// 2^-31 below v, lo = rint((v - hi 2^-31) 2^62) with the remainder exact
+__device__ __forceinline__ void cfix2(float v, long long& hi, long long& lo, unsigned& bad)
+{
+    if (!(fabsf(v) < 1073741824.0f)) {
+        bad = 1u;
+        hi = lo = 0;
+        return;
+    }
+    const float t = v * 2147483648.0f;  // exact
+    const float k = floorf(t);
+    hi = (long long)k;
+    lo = (long long)rintf((t - k) * 2147483648.0f);  // t - k exact, in [0, 1)
+}
+
+// a circle sum (units of 2^-31) as fp32: one rounding, no FP64
+__device__ __forceinline__ float csum_f32(unsigned long long v) { return (float)(long long)v * 4.656612873077393e-10f; }
+
+template <typename SH>
+__device__ __forceinline__ void cadd(SH& sh, unsigned long long* acc, float v)
+{
+    atomicAdd(acc, (unsigned long long)cfix(v, sh.bad));
+}
+
+template <typename SH>
+__device__ __forceinline__ void cadd2(SH& sh, unsigned long long* acc, float v)
+{
+    long long hi, lo;
+    cfix2(v, hi, lo, sh.bad);
+    atomicAdd(acc, (unsigned long long)hi);
+    atomicAdd(acc + 1, (unsigned long long)lo);
+}
 
 // (I, J) of tile t of a triangular tiling with nb blocks: row I starts at
 // I nb - I (I - 1) / 2 (the float estimate is corrected by the loops)
@@ -321,6 +376,7 @@ struct ContactWave {
     unsigned short ring[CRING];         // colliding pairs not in the previous list (slot)
     unsigned mask[CS][CS / 32];         // pairs in the previous list, per row
     unsigned nmask[CS][CS / 32];        // pairs in the new list
+    unsigned bad;                       // a circle sum could not take a value
 };
 
 // One pair of the contact tile through the prox: N from its list entry (ex)
@@ -367,13 +423,13 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
         for (int k = 0; k < 3; ++k) {
             const float gA = (xA[k] - zA[k]) - cA[k];
             const float gB = (xB[k] - zB[k]) - cB[k];
-            cadd(&sh.acc[rl][3 + k], gA);
-            cadd(&sh.acc[cs][3 + k], gB);
+            cadd(sh, &sh.acc[rl][3 + k], gA);
+            cadd(sh, &sh.acc[cs][3 + k], gB);
             gA2 += gA * gA;
             gB2 += gB * gB;
         }
-        cadd(&sh.acc[rl][6], gA2);
-        cadd(&sh.acc[cs][6], gB2);
+        cadd2(sh, &sh.acc[rl][6], gA2);
+        cadd2(sh, &sh.acc[cs][6], gB2);
     }
     if (act) {
 #pragma unroll
@@ -381,8 +437,8 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
             mA[k] = xA[k] + uA[k];
             mB[k] = xB[k] + uB[k];
             // per-edge rho: the consensus weighs each deviation with its edge's rho
-            cadd(&sh.acc[rl][k], RE ? rA * (mA[k] - zA[k]) : mA[k] - zA[k]);
-            cadd(&sh.acc[cs][k], RE ? rB * (mB[k] - zB[k]) : mB[k] - zB[k]);
+            cadd(sh, &sh.acc[rl][k], RE ? rA * (mA[k] - zA[k]) : mA[k] - zA[k]);
+            cadd(sh, &sh.acc[cs][k], RE ? rB * (mB[k] - zB[k]) : mB[k] - zB[k]);
         }
     }
     return act;
@@ -455,7 +511,7 @@ __device__ __forceinline__ void contact_zc(const CArgs& a, int cc, float (&z)[3]
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const float zk = a.Zin[3 * cc + k];
-            z[k] = zk + (float)((double)(long long)a.Ap[(long)k * a.N + cc] * CFIX_INV) / den;
+            z[k] = zk + csum_f32(a.Ap[(long)k * a.N + cc]) / den;
             c[k] = z[k] - zk;
         }
     } else {
@@ -493,6 +549,7 @@ __device__ __forceinline__ void contact_unary(const CArgs& a, int P)
 #pragma unroll
         for (int q = 0; q < Q; ++q) s[q] = 0;
         float g2 = 0.0f;
+        unsigned bad = 0u;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const long q = p0 + side * TT;
@@ -515,18 +572,25 @@ __device__ __forceinline__ void contact_unary(const CArgs& a, int P)
             for (int k = 0; k < 3; ++k) {
                 a.X[k * S + q] = x[k];
                 a.U[k * S + q] = u[k];
-                s[k] += cfix(RE ? r * ((x[k] + u[k]) - z[k]) : (x[k] + u[k]) - z[k]);
+                s[k] += cfix(RE ? r * ((x[k] + u[k]) - z[k]) : (x[k] + u[k]) - z[k], bad);
                 if constexpr (RES) {
                     const float g = (x[k] - z[k]) - cz[k];
-                    s[3 + k] += cfix(g);
+                    s[3 + k] += cfix(g, bad);
                     g2 += g * g;
                 }
             }
+            if constexpr (RES) {
+                long long hi, lo;
+                cfix2(g2, hi, lo, bad);
+                s[6] += hi;
+                s[7] += lo;
+                g2 = 0.0f;
+            }
         }
-        if constexpr (RES) s[6] += cfix(g2);
 #pragma unroll
         for (int q = 0; q < Q; ++q)
             if (s[q]) atomicAdd(&a.Ac[(long)q * a.N + cc], (unsigned long long)s[q]);
+        if (bad) atomicOr(a.flag, 1u);
     }
 }
 
@@ -570,6 +634,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         (&sh.nmask[0][0])[e] = 0u;
     }
     for (int k = lane; k < 2 * CS * Q; k += 64) (&sh.acc[0][0])[k] = 0ull;
+    if (lane == 0) sh.bad = 0u;
     wave_lds_fence();
     // the first 64 entries of the previous list, in flight during the tests
     float4 pre0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pre1 = pre0;
@@ -659,6 +724,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 if (v) atomicAdd(&a.Ac[(long)q * N + circ], v);
             }
     }
+    if (lane == 0 && sh.bad) atomicOr(a.flag, 1u);
 }
 
 // The end of a run: Z_{k+1} = Z_k + sum a / (N + 1) from the last launch's
@@ -708,9 +774,12 @@ __global__ __launch_bounds__(CZ_THREADS) void k_contact_zreduce(CArgs a)
     const int c = blockIdx.x * CZ_THREADS + threadIdx.x;
     const int N = a.N;
     if (c >= N) return;
+    unsigned long long sa[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sa[k] = a.Ac[(long)k * N + c];
     double s[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) s[q] = (double)(long long)a.Ac[(long)q * N + c] * CFIX_INV;
+    for (int q = 3; q < Q; ++q) s[q] = (double)(long long)a.Ac[(long)q * N + c] * (q == 7 ? CFIX2_INV : CFIX_INV);
     float zk[3], zp[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -725,10 +794,11 @@ __global__ __launch_bounds__(CZ_THREADS) void k_contact_zreduce(CArgs a)
     }
     const float deg = (float)(N + 1);
     double p = 0.0, dz2 = 0.0;
-    if constexpr (RES) p = s[6];
+    if constexpr (RES) p = s[6] + s[7];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float zn = zk[k] + (float)s[k] / (RE ? a.W[c] : deg);
+        // the same operations as contact_zc's, so the same bits
+        const float zn = zk[k] + csum_f32(sa[k]) / (RE ? a.W[c] : deg);
         if constexpr (RES) {
             const float dz = zn - zk[k];
             const double e = (double)(dz - (zk[k] - zp[k]));
@@ -741,7 +811,9 @@ __global__ __launch_bounds__(CZ_THREADS) void k_contact_zreduce(CArgs a)
     }
     if constexpr (RES) {
         const float rho = a.uni[0];
-        a.Rpc[c] = p;
+        // S2 - 2 e.S1 + (N + 1)|e|^2 >= 0; the roundings of the two sides can
+        // leave a tiny negative
+        a.Rpc[c] = p > 0.0 ? p : 0.0;
         a.Rd[c] = RE ? (double)a.W[a.N + c] * dz2 : (double)rho * rho * (double)deg * dz2;
     }
 }
@@ -1163,6 +1235,7 @@ static CArgs cargs(DevGraph* s)
     a.Zo = nullptr;
     a.Rpc = d->d_Rpc;
     a.Rd = d->d_Rd;
+    a.flag = d->d_flag;
     return a;
 }
 
@@ -1212,6 +1285,7 @@ static int contact_alloc(DevGraph* s)
     RC(dmalloc(s, (void**)&d->d_Rpc, sizeof(double) * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_Zpp, sizeof(float) * 3 * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_W, sizeof(float) * 2 * (size_t)d->N));
+    RC(dmalloc(s, (void**)&d->d_flag, sizeof(unsigned)));
     return PARADMM_OK;
 }
 
@@ -1298,6 +1372,10 @@ int dense_contact_enter(DevGraph* s)
     RC(contact_alloc(s));
     d->c_par = 0;  // the entry writes list 0
     d->c_run = 0;
+    // a fresh state: no sum has lost a value (a fill kernel, not a memset,
+    // as the entry may run inside a captured stream)
+    k_fill_u32<<<1, 64, 0, s->stream>>>(d->d_flag, 1, 0u);
+    CK(hipGetLastError());
     // one rho everywhere (a policy's raised edges make it per edge)
     d->c_rhoe = !(s->rho_uni && s->policy_raised == 0);
     const CArgs a = cargs(s);
@@ -1422,12 +1500,12 @@ int dense_residual_reduce(DevGraph* s)
 {
     DenseState* d = s->dense;
     if (d->c_state) {  // per-circle residuals of the last contact iteration
-        k_dense_res_final<<<1, 256, 0, s->stream>>>(d->N, d->d_Rpc, d->N, d->d_Rd, s->d_res);
+        k_dense_res_final<<<1, 256, 0, s->stream>>>(d->N, d->d_Rpc, d->N, d->d_Rd, s->d_res, d->d_flag);
         CK(hipGetLastError());
         return PARADMM_OK;
     }
     RC(dense_finish(s));
-    k_dense_res_final<<<1, 256, 0, s->stream>>>(d->nt, d->d_Rp, d->N, d->d_Rd, s->d_res);
+    k_dense_res_final<<<1, 256, 0, s->stream>>>(d->nt, d->d_Rp, d->N, d->d_Rd, s->d_res, nullptr);
     CK(hipGetLastError());
     return PARADMM_OK;
 }

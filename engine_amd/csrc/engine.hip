@@ -1512,6 +1512,9 @@ extern "C" int paradmm_gpu_solve(graph* G, const paradmm_solve_opts* o, paradmm_
         CK(hipStreamSynchronize(s->stream));
         r->primal_rms = sqrt(h[0] * scale);
         r->dual_rms = sqrt(h[1] * scale);
+        // a non-finite residual: the iterate has lost a value (diverged),
+        // more iterations cannot bring it back
+        if (!std::isfinite(r->primal_rms) || !std::isfinite(r->dual_rms)) break;
         if (r->primal_rms <= o->tol_primal && r->dual_rms <= o->tol_dual) {
             bool ok = true;
             if (o->tol_feas >= 0) {

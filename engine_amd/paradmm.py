@@ -412,9 +412,11 @@ class Graph:
         check(self.lib.paradmm_gpu_time_iterations(self.ptr, iters, C.byref(ms)), "paradmm_gpu_time_iterations")
         return ms.value
 
-    def solve(self, max_iters, tol=1e-4, tol_feas=1e-4, check_every=50, policy_every=0):
-        """Iterates to tolerance on the GPU; returns the paradmm_solve_result as a dict."""
-        o = SolveOpts(max_iters, check_every, tol, tol, tol_feas, policy_every)
+    def solve(self, max_iters, tol=1e-4, tol_feas=1e-4, check_every=50, policy_every=0, tol_dual=None):
+        """Iterates to tolerance on the GPU; returns the paradmm_solve_result as
+        a dict.  tol bounds the primal RMS residual, and the dual one unless
+        tol_dual is given."""
+        o = SolveOpts(max_iters, check_every, tol, tol if tol_dual is None else tol_dual, tol_feas, policy_every)
         r = SolveResult()
         check(self.lib.paradmm_gpu_solve(self.ptr, C.byref(o), C.byref(r)), "paradmm_gpu_solve")
         return r.as_dict()

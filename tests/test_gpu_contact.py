@@ -299,3 +299,27 @@ def test_contact_runs_after_a_policy_raises_edges():
             g.iterate(64)
         zs.append(g.read("Z"))
     assert np.array_equal(zs[0], zs[1])
+
+
+@pytest.mark.parametrize("nc,seed,rho", [(20, 0, 100.0), (40, 3, 60.0)])
+def test_contact_primal_residual_near_convergence(nc, seed, rho):
+    """Near convergence the listed edges' |g|^2 are small (|g| ~ 1e-5 and
+    below): their sum must keep them (held exactly to 2^-62), so the contact
+    state's primal residual equals the per-edge recomputation from the
+    materialised X and Z (fp32 rounding of X - Z apart) and never goes
+    negative."""
+    g, _ = square_pair(nc, seed, rho=rho)
+    res = g.solve(40000, tol=3e-6, tol_dual=1.0, tol_feas=-1.0, check_every=100)
+    p2, _ = g.residuals()
+    g.sync_to_host()  # leaves the contact state
+    ev = g.edge_var()
+    X = g.array("X").reshape(-1, 3)
+    Z = g.array("Z").reshape(-1, 3)
+    host = float(np.sum((X - Z[ev]) ** 2))
+    rms = np.sqrt(p2 / (3 * g.nedge))
+    print(f"N={nc} rho={rho}: after {res['iterations']} iterations primal rms {rms:.2e}, "
+          f"contact primal2 {p2:.4e} host recomputation {host:.4e}")
+    assert res["primal_rms"] < 1e-5
+    assert p2 > 0.0
+    assert p2 == pytest.approx(host, rel=0.05)
+    assert "contact_tile" in _kernels(g, 2)
