@@ -7,11 +7,13 @@
 // These judge every solve (fixed or learned rho); the reference has none.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "engine_internal.h"
 #include "engine_state.h"
 #include "paradmm_gpu.h"
 
-#define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hip_fail(e_, #x, __FILE__, __LINE__); } while (0)
+#define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return pdm::hip_fail(e_, #x, __FILE__, __LINE__); } while (0)
 #define RC(x) do { int r_ = (x); if (r_) return r_; } while (0)
 
 namespace pdm {
@@ -94,3 +96,65 @@ int square_quality(DevGraph* s, double out[3])
 }
 
 }  // namespace pdm
+
+// ---------------------------------------------------------------- no allocation
+// paradmm_square_quality_noalloc keeps the per-circle partials of
+// k_square_quality in this file-scope array instead of memory the library
+// allocates: one copy per device, held until the process ends.  Host code
+// never uses the name `scratch` as an address (on the host it is only a
+// stand-in for the symbol); it uses the device address that
+// hipGetSymbolAddress returns, looked up once per device with that device
+// current and kept in g_scratch.
+static __device__ __align__(256) double scratch[3 * PARADMM_QUALITY_NOALLOC_MAX];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch[kMaxDevices];
+
+static int scratch_of_current_device(double** out)
+{
+    int dev = -1;
+    CK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_square_quality_noalloc: device %d out of range", dev);
+        return PARADMM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        CK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            pdm_set_error(PARADMM_ERR_HIP, "paradmm_square_quality_noalloc: no device address for the scratch array");
+            return PARADMM_ERR_HIP;
+        }
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch[dev];
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_square_quality_noalloc(const float* Z, int ncircles, double* out3, void* stream)
+{
+    if (!Z || !out3 || ncircles < 0) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_square_quality_noalloc: Z and out3 must be device pointers, "
+                                       "ncircles >= 0");
+        return PARADMM_ERR_ARG;
+    }
+    if (ncircles > PARADMM_QUALITY_NOALLOC_MAX) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_square_quality_noalloc: %d circles exceed the scratch limit of %d",
+                      ncircles, PARADMM_QUALITY_NOALLOC_MAX);
+        return PARADMM_ERR_ARG;
+    }
+    double* part = nullptr;
+    RC(scratch_of_current_device(&part));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // every partial the final pass reads ([0, 3 ncircles)) is written first:
+    // nothing of an earlier call's contents is used
+    if (ncircles > 0) {
+        hipLaunchKernelGGL(pdm::k_square_quality, dim3((unsigned)ncircles), dim3(256), 0, st, ncircles, Z, part);
+        CK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(pdm::k_square_quality_final, dim3(1), dim3(256), 0, st, ncircles, part, out3);
+    CK(hipGetLastError());
+    return PARADMM_OK;
+}

@@ -171,6 +171,19 @@ int paradmm_gpu_solve(graph *G, const paradmm_solve_opts *opts, paradmm_solve_re
  * circles): max pair overlap, max container violation, sum of radii. */
 int paradmm_gpu_square_quality(graph *G, double *max_overlap, double *max_box, double *sum_r);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the computation of paradmm_gpu_square_quality on a
+ * caller-owned device array Z (ncircles x 3 fp32, rows cx, cy, r), writing
+ * out3 (device memory, 3 doubles: max pair overlap, max container
+ * violation, sum of radii) asynchronously on `stream` (a hipStream_t; NULL
+ * is the null stream).  Its per-circle partials live in a static device
+ * array of PARADMM_QUALITY_NOALLOC_MAX circles (1.5 MB per device, held
+ * until the process ends); more circles is PARADMM_ERR_ARG.  It allocates
+ * nothing; calls that overlap on one device race on that array. */
+#define PARADMM_QUALITY_NOALLOC_MAX 65536
+int paradmm_square_quality_noalloc(const float *Z, int ncircles, double *out3, void *stream);
+
 /* ---- learned rho (NEW: the reference's hook update_outgoing_rhos,
  * core/paradmm.c:564-566, is empty) ---------------------------------------
  * A 6 -> 32 -> 32 -> 1 ReLU MLP maps per-edge features to log10(rho_e);
