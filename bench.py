@@ -13,9 +13,14 @@ pairs whose prox is active keep per-edge values; every edge is still updated
 every iteration).
 
 Prints one JSON line: value = edge-updates/s, plus the roofline of the
-dominant kernel (algorithmic bytes / mean launch time from HIP events) and a
-bounded OpenMP CPU baseline (the FP64 restatement of the reference's OpenMP
-path, timed on this host).
+dominant kernel (its algorithmic bytes per launch / mean launch time from HIP
+events recorded by the dispatch itself, the HBM traffic of the committed
+rocprofv3 counter profile, and for the contact state its VALU issue rate).
+Further legs: the per-edge path on the same workload (PARADMM_CONTACT=0, HBM
+bound), N = 1000 circles, the 256-instance batch, learned vs fixed rho to
+tolerance, and the CPU baseline: the FP64 restatement of the reference's
+OpenMP path compiled on this host and timed on all its usable cores and on
+one, with the GPU's first iterations checked against the FP64 oracle.
 """
 import argparse
 import json
@@ -91,48 +96,134 @@ def measured_valu(kernel, n, seed):
     return _profile_value(VALU_PROFILES.get(kernel), "valu_insts_per_launch", n, seed)
 
 
-def roofline(dom, dom_ms, dom_bytes, kernel_bytes, achieved, traffic, valu, copy_gbs, contact):
+def contact_alg_bytes(n, cs, listed):
+    """Bytes one contact-state iteration must move at the least: the contact
+    lists (both edges' M, 32 B per listed pair, read, and as many written:
+    the list is about the same length from one iteration to the next), the
+    list bit masks (cs*cs bits per contact tile, read and written), per
+    circle Z and the previous launch's three sums read (36 B), its Z stored
+    (12 B), eight sum words cleared (64 B) and this launch's three sums
+    added (24 B), and the box and radius edges' X and U read and written
+    (2 x 48 B per circle)."""
+    nbs = (n + cs - 1) // cs
+    nts = nbs * (nbs + 1) // 2
+    return 64 * listed + nts * cs * cs // 4 + n * (36 + 12 + 64 + 24) + 96 * n
+
+
+def roofline_contact(dom_ms, alg_bytes, traffic, valu, copy_gbs, n, edges, listed):
+    """Roofline of the contact tile kernel: its own algorithmic bytes per
+    launch (contact_alg_bytes) against HBM peak, the measured traffic of the
+    committed counter profile, and its VALU issue rate."""
+    t = dom_ms * 1e-3
+    achieved = alg_bytes / t / 1e9
     r = {
-        "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "bound": "hbm", "kernel": "contact_tile", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-        "bytes_per_launch": dom_bytes, "bytes_per_edge_update": EDGE_UPDATE_BYTES,
-        "traffic_gbs": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None,
-        "traffic_frac_of_copy": (traffic / (dom_ms * 1e-3) / 1e9 / copy_gbs) if traffic else None,
+        "algorithmic_bytes_per_launch": alg_bytes, "listed_pairs": listed,
+        "traffic_gbs": (traffic / t / 1e9) if traffic else None,
+        "traffic_frac": (traffic / t / 1e9 / HBM_PEAK_GBS) if traffic else None,
+        "traffic_frac_of_copy": (traffic / t / 1e9 / copy_gbs) if traffic else None,
         "mean_launch_ms": dom_ms, "copy_kernel_gbs": copy_gbs,
+        "pair_tests_per_s": (n * (n - 1) // 2) / t,
+        "edge_update_unit": {"bytes_per_edge_update": EDGE_UPDATE_BYTES,
+                             "bytes_per_launch": EDGE_UPDATE_BYTES * edges,
+                             "equivalent_gbs": EDGE_UPDATE_BYTES * edges / t / 1e9,
+                             "note": "the per-edge state an X pass and a Z pass would move for the same "
+                                     "edge-updates; the contact state stores per-edge values only for listed "
+                                     "pairs, so this is a throughput equivalent, not a bandwidth"},
+        "limiter": "latency: all tile wavefronts are resident at once and wait on their first loads "
+                   "(Z of the tile's circles, list entries); HBM and VALU are both well below peak",
     }
-    if contact:
-        # the contact state stores per-edge values only for active pairs:
-        # the kernel moves traffic << 72 B per edge-update and is bound by
-        # its VALU issue (the pair tests) and memory latency
-        r["note"] = ("frac counts 72 B per edge-update, the per-edge state an X pass and a Z pass move; the "
-                     "contact state keeps per-edge values only for pairs whose prox is active, so the kernel "
-                     "moves `traffic` bytes per launch and is bound by VALU issue (valu.frac) and latency")
-        if valu:
-            rate = valu / (dom_ms * 1e-3)
-            r["valu"] = {"insts_per_launch": valu, "achieved": rate, "peak": VALU_PEAK_WAVE_INSTS,
-                         "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK_WAVE_INSTS}
-    else:
-        r["kernel_min_bytes_per_launch"] = kernel_bytes
-        r["kernel_min_gbs"] = kernel_bytes / (dom_ms * 1e-3) / 1e9
-        # frac above uses the 72 B/edge-update unit (an X pass and a Z pass);
-        # the fused kernel needs only kernel_min bytes per edge
-        r["kernel_min_frac"] = kernel_bytes / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
-        r["note"] = ("frac counts 72 B per edge-update (X and U read and written by an X pass and a Z pass); "
-                     "the dominant kernel moves kernel_min bytes per edge (M alone inside captured runs), so "
-                     "kernel_min_frac and traffic_frac_of_copy are the fractions that bound it")
+    if valu:
+        rate = valu / t
+        r["valu"] = {"insts_per_launch": valu, "achieved": rate, "peak": VALU_PEAK_WAVE_INSTS,
+                     "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK_WAVE_INSTS}
     return r
 
 
-def cpu_baseline(n, iters, threads):
-    """The reference's OpenMP path (five parallel passes per iteration,
-    packing/host.c:237-244) as the oracle's -Ofast restatement."""
+def roofline_per_edge(dom, dom_ms, kernel_bytes, traffic, copy_gbs, edges):
+    """Roofline of the per-edge tile kernel: the bytes it must move per edge
+    (24 B inside captured runs: M read and written; 48 B for the plain fused
+    pass) against HBM peak."""
+    t = dom_ms * 1e-3
+    achieved = kernel_bytes / t / 1e9
+    return {
+        "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "algorithmic_bytes_per_launch": kernel_bytes,
+        "traffic_gbs": (traffic / t / 1e9) if traffic else None,
+        "traffic_frac_of_copy": (traffic / t / 1e9 / copy_gbs) if traffic else None,
+        "mean_launch_ms": dom_ms, "copy_kernel_gbs": copy_gbs,
+        "edge_update_unit_equivalent_gbs": EDGE_UPDATE_BYTES * edges / t / 1e9,
+    }
+
+
+def _physical_cores():
+    seen = set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("physical id"):
+                    phys = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    core = line.split(":")[1].strip()
+                elif not line.strip():
+                    if core is not None:
+                        seen.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    return len(seen) or (os.cpu_count() or 1)
+
+
+def cpu_baseline(n, gpu_zhist, threads=0):
+    """The reference's OpenMP path (the five update*OpenMP passes per
+    iteration, packing/host.c:237-244; core/paradmm.c:812-945) as the
+    oracle's FP64 restatement, pinned to the reference compiled from source,
+    compiled here with gcc -Ofast -march=native -fopenmp (the reference's
+    build flags plus this host's ISA) and timed with gettimeofday around the
+    iteration loop: N = 5000 (20 iterations) and N = 1000 (1000) on every
+    thread this process may use, and a shorter sample of each on 1 thread.
+    Also the checker's job of this leg: the GPU's Z over its first
+    iterations from the seeded start against the FP64 oracle (-O2 build)."""
+    import tempfile
+
     from oracle import pyoracle as po
-    g = po.Graph.square(n, 0, 1.0, 1.0, 1.0, fast=True)
-    dt = g.iterate_omp(iters, threads)
-    e = g.nedge
-    g.close()
-    return {"value": e * iters / dt, "unit": "edge-updates/s", "cores": threads, "kind": "port",
-            "sample": f"N={n} unit-square packing, {iters} OpenMP iterations (FP64, -Ofast), {dt:.2f} s"}
+    tmp = tempfile.mkdtemp(prefix="pdm_oracle_")
+    lib = po.build_native(tmp)
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env_t = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    phys = _physical_cores()
+    nt = threads or min(x for x in (avail, phys, env_t or avail) if x > 0)
+    runs = []
+    for size, iters, th in ((5000, 20, nt), (1000, 1000, nt), (5000, 3, 1), (1000, 100, 1)):
+        g = po.Graph(lib.or_build_square(size, 0, 1.0, 1.0, 1.0), lib)
+        dt = g.iterate_omp(iters, th)
+        runs.append({"n": size, "iterations": iters, "threads": th, "seconds": dt,
+                     "edge_updates_per_s": g.nedge * iters / dt})
+        print(f"# cpu baseline N={size} {iters} it on {th} threads: {dt:.2f} s, "
+              f"{runs[-1]['edge_updates_per_s']:.3e} edge-updates/s", file=sys.stderr, flush=True)
+        g.close()
+    main = runs[0]
+    out = {"value": main["edge_updates_per_s"], "unit": "edge-updates/s", "cores": main["threads"], "kind": "port",
+           "sample": f"N=5000 unit-square packing (E=25,005,000), 20 OpenMP iterations of the FP64 restatement "
+                     f"(-Ofast -march=native -fopenmp, compiled on this host), {main['seconds']:.2f} s",
+           "host_physical_cores": phys, "threads_available": avail, "runs": runs}
+    # the GPU run's error against the FP64 oracle over its first iterations
+    o = po.Graph.square(n, 0, 1.0, 1.0, 1.0)
+    hist = o.iterate_par(gpu_zhist.shape[0], nt, record=True)
+    o.close()
+    import numpy as np
+    num = np.max(np.abs(gpu_zhist - hist), axis=1)
+    den = np.maximum(np.max(np.abs(hist), axis=1), 1e-300)
+    err = (num / den).tolist()
+    oracle_error = {"n": n, "iterations": len(err), "max_rel_Z_error_per_iteration": err, "max": max(err),
+                    "note": "norm-wise max |Z_gpu - Z_oracle| / max |Z_oracle| after each iteration from the "
+                            "seeded start, the GPU in its default (contact) path, the oracle in FP64"}
+    print(f"# FP64 oracle error of the GPU's first {len(err)} iterations at N={n}: max {max(err):.2e}",
+          file=sys.stderr, flush=True)
+    return out, oracle_error
 
 
 def batch_rho(n):
@@ -187,6 +278,8 @@ def batch_bench(P, count, seed, iters, cap):
                                                          if c2 else None)}
         print(f"# batch {name}: {len(c2)}/{count} converged, {ms3:.1f} ms", file=sys.stderr, flush=True)
         b.close()
+        if pol is not None:
+            pol.close()
         for g in gs:
             g.close()
     return {
@@ -233,7 +326,39 @@ def learned_bench(P, cases, cap):
             f"{k} {'%d it %.0f ms' % (v['iterations'], v['ms']) if v['converged'] else 'not converged'}"
             for k, v in row.items() if isinstance(v, dict)), file=sys.stderr, flush=True)
         out.append(row)
+    m.close()
     return out
+
+
+def time_leg(P, n, seed, steps, warmup, profile_iters, contact):
+    """One single-instance leg: N circles, rho = 1, alpha = 1; the contact
+    state (the default for one rho and alpha = 1) or the per-edge path
+    (PARADMM_CONTACT=0).  Returns timing, kernel profile and what the
+    roofline needs."""
+    os.environ["PARADMM_CONTACT"] = "1" if contact else "0"
+    t0 = time.time()
+    g = P.packing_square(n, seed, rho=1.0, alpha=1.0, a_exp=1.0)
+    g.prepare()
+    t_setup = time.time() - t0
+    E = g.nedge
+    g.time_iterations(warmup)
+    ms = g.time_iterations(steps)
+    prof = g.profile_kernels(profile_iters)
+    in_c, cs, listed = g.contact_info()
+    p2, d2 = g.residuals()
+    layout = {1: "generic", 2: "dense"}.get(g.layout(), "?")
+    g.close()
+    per_iter = {k: v[0] * v[1] / profile_iters for k, v in prof.items()}
+    dom = max(per_iter, key=per_iter.get)
+    leg = {"n": n, "edges": E, "setup_s": t_setup, "steps": steps, "ms": ms, "ms_per_step": ms / steps,
+           "edge_updates_per_s": E * steps / (ms * 1e-3), "layout": layout,
+           "state": "contact" if in_c else "per-edge", "contact_tile_side": cs, "listed_pairs": listed,
+           "kernels_ms": {k: v[0] for k, v in prof.items()}, "dominant": dom, "dominant_ms": prof[dom][0],
+           "residuals_last": {"primal2": p2, "dual2": d2}}
+    print(f"# N={n} {leg['state']}: setup {t_setup:.1f} s, {steps} iterations {ms / steps:.4f} ms/iter, "
+          f"{leg['edge_updates_per_s']:.4e} edge-updates/s; kernels (mean ms x launches): "
+          + ", ".join(f"{k} {v[0]:.4f}x{v[1]}" for k, v in prof.items()), file=sys.stderr, flush=True)
+    return leg
 
 
 def main():
@@ -244,9 +369,10 @@ def main():
     ap.add_argument("--n", type=int, default=5000, help="circles")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--profile-iters", type=int, default=10)
-    ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-per-edge", action="store_true", help="skip the per-edge (PARADMM_CONTACT=0) leg")
+    ap.add_argument("--no-n1000", action="store_true", help="skip the N = 1000 leg")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batched workload (0: skip)")
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--batch-iters", type=int, default=200)
@@ -263,43 +389,31 @@ def main():
 
     from engine_amd import paradmm as P
 
-    t0 = time.time()
-    g = P.packing_square(args.n, args.seed, rho=1.0, alpha=1.0, a_exp=1.0)
-    g.prepare()
-    t_setup = time.time() - t0
-    E = g.nedge
-    layout = {1: "generic", 2: "dense"}.get(g.layout(), "?")
-    print(f"# setup {t_setup:.1f} s: N={args.n} E={E} layout={g.layout()}", file=sys.stderr, flush=True)
-
-    g.time_iterations(args.warmup)
-    ms = g.time_iterations(args.steps)
-    ms_per_step = ms / args.steps
-    value = E * args.steps / (ms * 1e-3)
-    print(f"# timed {args.steps} iterations: {ms_per_step:.4f} ms/iter, {value:.4e} edge-updates/s",
-          file=sys.stderr, flush=True)
-
-    prof = g.profile_kernels(args.profile_iters)
-    per_iter = {k: v[0] * v[1] / args.profile_iters for k, v in prof.items()}
-    dom = max(per_iter, key=per_iter.get)
-    dom_ms = prof[dom][0]
-    dom_edges = edges_per_launch(dom, args.n)
-    if dom in FULL_ITERATION_KERNELS:
-        dom_edges = E  # unary edges of the diagonal tiles included
-        dom_bytes = EDGE_UPDATE_BYTES * dom_edges
-    else:
-        dom_bytes = BYTES_PER_EDGE.get(dom, 0) * dom_edges
-    # rho = 1 and alpha = 1 on every edge: the tile kernel runs its variant
-    # without the per-edge rho/alpha streams (48 B per edge instead of 56, or
-    # 24 inside a captured run that keeps M in the X array)
-    kernel_bytes = UNIFORM_BYTES_PER_EDGE.get(dom, BYTES_PER_EDGE.get(dom, 0)) * dom_edges
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    # the headline: the default path (the contact state for one rho and alpha = 1)
+    head = time_leg(P, args.n, args.seed, args.steps, args.warmup, args.profile_iters, contact=True)
+    E = head["edges"]
+    value = head["edge_updates_per_s"]
+    copy_gbs = P.copy_bandwidth(1 << 30, 20)
+    print(f"# plain copy kernel (1 GiB): {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    dom, dom_ms = head["dominant"], head["dominant_ms"]
     traffic = measured_traffic(dom, args.n, args.seed)
     valu = measured_valu(dom, args.n, args.seed)
-    contact = dom == "contact_tile"
-    print(f"# kernels (mean ms per launch): " + ", ".join(f"{k} {v[0]:.4f}x{v[1]}" for k, v in prof.items()),
-          file=sys.stderr, flush=True)
+    if head["state"] == "contact":
+        alg = contact_alg_bytes(args.n, head["contact_tile_side"], head["listed_pairs"])
+        roof = roofline_contact(dom_ms, alg, traffic, valu, copy_gbs, args.n, E, head["listed_pairs"])
+    else:
+        kb = UNIFORM_BYTES_PER_EDGE.get(dom, BYTES_PER_EDGE.get(dom, 0)) * E
+        roof = roofline_per_edge(dom, dom_ms, kb, traffic, copy_gbs, E)
 
-    p2, d2 = g.residuals()
+    # the GPU's Z over its first iterations from the seeded start (checked
+    # against the FP64 oracle in the cpu_baseline leg)
+    import numpy as np
+    os.environ["PARADMM_CONTACT"] = "1"
+    g = P.packing_square(args.n, args.seed, rho=1.0, alpha=1.0, a_exp=1.0)
+    zhist = np.zeros((5, 3 * args.n))
+    for it in range(5):
+        g.iterate(1)
+        zhist[it] = g.read("Z").ravel()
     host_loop = None
     if args.host_loop_iters > 0:
         # the reference's own loop shape (packing/host.c:355-360): updateXM,
@@ -319,8 +433,6 @@ def main():
         print(f"# legacy host loop (PCIe-inclusive): {dt / args.host_loop_iters * 1e3:.1f} ms/iter, "
               f"{host_loop['edge_updates_per_s']:.3e} edge-updates/s", file=sys.stderr, flush=True)
     g.close()
-    copy_gbs = P.copy_bandwidth(1 << 30, 20)
-    print(f"# plain copy kernel (1 GiB): {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
     line = {
         "metric": "edge-updates/s (%HBM peak) + time to residual <=1e-4, learned-rho vs fixed-rho",
@@ -329,7 +441,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -338,13 +450,21 @@ def main():
         "config": {
             "workload": f"plain ADMM unit-square circle packing, N={args.n} circles, E={E} edges of dim 3, "
                         f"fixed rho=1, alpha=1, radius exponent 1, seed {args.seed}",
-            "circles": args.n, "edges": E, "layout": layout,
-            "state": "contact" if contact else "per-edge",
+            "circles": args.n, "edges": E, "layout": head["layout"], "state": head["state"],
         },
-        "roofline": roofline(dom, dom_ms, dom_bytes, kernel_bytes, achieved, traffic, valu, copy_gbs, contact),
-        "kernels_ms": {k: v[0] for k, v in prof.items()},
-        "residuals_last": {"primal2": p2, "dual2": d2},
+        "roofline": roof,
+        "kernels_ms": head["kernels_ms"],
+        "residuals_last": head["residuals_last"],
     }
+    if not args.no_per_edge:
+        pe = time_leg(P, args.n, args.seed, args.steps, args.warmup, args.profile_iters, contact=False)
+        kb = UNIFORM_BYTES_PER_EDGE.get(pe["dominant"], BYTES_PER_EDGE.get(pe["dominant"], 0)) * pe["edges"]
+        pe["roofline"] = roofline_per_edge(pe["dominant"], pe["dominant_ms"], kb,
+                                           measured_traffic(pe["dominant"], args.n, args.seed), copy_gbs, pe["edges"])
+        line["per_edge_path"] = pe
+    if not args.no_n1000 and args.n != 1000:
+        line["n1000"] = time_leg(P, 1000, args.seed, 10000, 200, args.profile_iters, contact=True)
+    os.environ.pop("PARADMM_CONTACT", None)
     if host_loop:
         line["host_loop_pcie_inclusive"] = host_loop
     if args.batch > 0:
@@ -353,8 +473,8 @@ def main():
         line["time_to_tol_learned_vs_fixed"] = learned_bench(P, ((5000, 3), (2000, 3), (2000, 4), (1000, 3), (1000, 4)),
                                                              30000)
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_iters, threads)
+        line["cpu_baseline"], line["oracle_error"] = cpu_baseline(args.n, zhist, args.cpu_threads)
+    P.release_all()
     print(json.dumps(line), flush=True)
 
 

@@ -1355,6 +1355,23 @@ static int contact_iteration(DevGraph* s, bool res)
 
 int dense_contact_parity(DevGraph* s) { return s->dense ? s->dense->c_par : 0; }
 
+int dense_contact_info(DevGraph* s, int* tile_side, long* listed_pairs)
+{
+    DenseState* d = s->dense;
+    *tile_side = 0;
+    *listed_pairs = 0;
+    if (!d || !d->c_state) return PARADMM_OK;
+    std::vector<int> cnt((size_t)d->nts);
+    CK(hipMemcpyAsync(cnt.data(), d->d_cnt + (size_t)d->c_par * d->nts, sizeof(int) * (size_t)d->nts,
+                      hipMemcpyDeviceToHost, s->stream));
+    CK(hipStreamSynchronize(s->stream));
+    long n = 0;
+    for (int c : cnt) n += c;
+    *tile_side = d->cs;
+    *listed_pairs = n;
+    return PARADMM_OK;
+}
+
 void dense_contact_set_parity(DevGraph* s, int p)
 {
     if (s->dense) s->dense->c_par = p & 1;

@@ -1447,6 +1447,18 @@ extern "C" int paradmm_gpu_layout(graph* G)
     return s ? s->layout : 0;
 }
 
+extern "C" int paradmm_gpu_contact_info(graph* G, int* in_contact, int* tile_side, long* listed_pairs)
+{
+    DevGraph* s = lookup(G);
+    int cs = 0;
+    long n = 0;
+    if (s && s->layout == 2) RC(dense_contact_info(s, &cs, &n));
+    if (in_contact) *in_contact = cs > 0;
+    if (tile_side) *tile_side = cs;
+    if (listed_pairs) *listed_pairs = n;
+    return PARADMM_OK;
+}
+
 extern "C" int paradmm_gpu_residuals(graph* G, double* primal2, double* dual2)
 {
     DevGraph* s = lookup(G);

@@ -208,6 +208,8 @@ int dense_contact_leave(DevGraph* s);
 // argument, so a captured run is bound to its start parity and leaves the
 // parity flipped by its length
 int dense_contact_parity(DevGraph* s);
+// contact tile side and pairs in the last list (0, 0 outside the contact state)
+int dense_contact_info(DevGraph* s, int* tile_side, long* listed_pairs);
 void dense_contact_set_parity(DevGraph* s, int p);
 
 }  // namespace pdm

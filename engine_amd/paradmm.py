@@ -140,6 +140,7 @@ def load():
         "paradmm_gpu_read": (C.c_int, [G, C.c_int, _DP]),
         "paradmm_gpu_release": (C.c_int, [G]),
         "paradmm_gpu_layout": (C.c_int, [G]),
+        "paradmm_gpu_contact_info": (C.c_int, [G, _IP, _IP, C.POINTER(C.c_long)]),
         "paradmm_gpu_residuals": (C.c_int, [G, _DP, _DP]),
         "paradmm_gpu_time_iterations": (C.c_int, [G, C.c_int, _DP]),
         "paradmm_gpu_profile_kernels": (C.c_int, [G, C.c_int, C.c_int, C.c_char_p, _DP, _IP, _IP]),
@@ -401,6 +402,12 @@ class Graph:
 
     def layout(self):
         return self.lib.paradmm_gpu_layout(self.ptr)
+
+    def contact_info(self):
+        """(in the contact state, contact tile side, pairs listed by the last iteration)"""
+        a, b, c = C.c_int(), C.c_int(), C.c_long()
+        check(self.lib.paradmm_gpu_contact_info(self.ptr, C.byref(a), C.byref(b), C.byref(c)), "contact_info")
+        return bool(a.value), b.value, c.value
 
     def residuals(self):
         p, d = C.c_double(), C.c_double()

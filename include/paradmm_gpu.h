@@ -124,6 +124,11 @@ enum { PARADMM_READ_Z = 0, PARADMM_READ_X = 1, PARADMM_READ_U = 2, PARADMM_READ_
 int paradmm_gpu_read(graph *G, int which, double *out);
 /* 1 = generic CSR layout, 2 = dense pair-tile layout; 0 before prepare */
 int paradmm_gpu_layout(graph *G);
+/* Whether the dense layout is in its contact state (one rho, alpha = 1:
+ * only pairs whose prox was active keep per-edge values), its contact tile
+ * side and the pairs its last iteration listed as active; 0, 0, 0 outside
+ * it.  No reference counterpart (observability of this engine's state). */
+int paradmm_gpu_contact_info(graph *G, int *in_contact, int *tile_side, long *listed_pairs);
 /* NEW: FP64 residuals of the last device iteration (the reference has none):
  * primal2 = sum_e ||X_e - Z_v(e)||^2, dual2 = sum_e rho_e^2 ||Z_v - Zprev_v||^2. */
 int paradmm_gpu_residuals(graph *G, double *primal2, double *dual2);

@@ -73,6 +73,17 @@ def load(fast=False):
     return _libs[name]
 
 
+def build_native(outdir):
+    """Compiles the oracle with the reference's build flags (-Ofast) plus this
+    host's instruction set (-march=native) and OpenMP into outdir, and loads
+    it: the timed CPU baseline, built on the machine it runs on."""
+    import subprocess
+    path = os.path.join(outdir, "liboracle_native.so")
+    subprocess.run(["gcc", "-std=c99", "-Ofast", "-march=native", "-fopenmp", "-fPIC", "-shared", "-o", path,
+                    os.path.join(HERE, "paradmm_oracle.c"), "-lm"], check=True)
+    return _bind(C.CDLL(path))
+
+
 def load_ref():
     """The reference engine compiled from source (this container only), or None."""
     path = os.path.join(HERE, "_ref", "libref_driver.so")
