@@ -343,6 +343,11 @@ __device__ void phase_check(const BArgs& a, const BLists& L, int it, bool final)
             const double pr = sqrt(p / (3.0 * in.E)), du = sqrt(d / (3.0 * in.E));
             bool conv = pr <= a.tol_p && du <= a.tol_d;
             if (a.tol_f >= 0.0) conv = conv && ov <= a.tol_f && bx <= a.tol_f;
+            // an instance whose state is no longer finite (a NaN or an
+            // overflow anywhere reaches its residual sums and its sum of
+            // radii) cannot recover: it retires now, not converged, with a
+            // NaN primal residual
+            const bool lost = !isfinite(pr) || !isfinite(du) || !isfinite(sr);
             BState& s = a.st[i];
             s.iterations = it;
             s.primal_rms = pr;
@@ -350,8 +355,9 @@ __device__ void phase_check(const BArgs& a, const BLists& L, int it, bool final)
             s.max_overlap = ov;
             s.max_box = bx;
             s.objective = sr;
-            if (conv || final) {
-                s.converged = conv ? 1 : 0;
+            if (lost) s.primal_rms = __builtin_nan("");
+            if (conv || final || lost) {
+                s.converged = conv && !lost ? 1 : 0;
                 s.t_end = wall_clock64();
                 s.active = 0;
             }

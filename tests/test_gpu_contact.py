@@ -323,3 +323,42 @@ def test_contact_primal_residual_near_convergence(nc, seed, rho):
     assert p2 > 0.0
     assert p2 == pytest.approx(host, rel=0.05)
     assert "contact_tile" in _kernels(g, 2)
+
+
+def test_snapshot_saved_in_contact_state_resumes(tmp_path):
+    """Saving a graph whose newest state is in the contact state writes the
+    materialised X, M, N, U and Z; a fresh graph of the same structure
+    loaded from the file continues the same trajectory (to the fp32
+    rounding of leaving and re-entering the state)."""
+    g, _ = square_pair(120, 9, rho=5.0)
+    g.iterate(100)
+    assert "contact_tile" in _kernels(g, 2)
+    path = str(tmp_path / "contact.pdm")
+    zdev = g.read("Z").ravel()
+    g.save_state(path)
+    np.testing.assert_array_equal(g.array("Z"), zdev)
+    xdev = g.read("X").ravel()
+    np.testing.assert_array_equal(g.array("X"), xdev)
+    g.iterate(60)
+    h, _ = square_pair(120, 9, rho=5.0)
+    h.load_state(path)
+    h.iterate(60)
+    err = max_rel(h.read("Z"), g.read("Z"))
+    print(f"resumed from a contact-state snapshot: Z after 60 more iterations differs by {err:.1e}")
+    assert err < 1e-5
+    assert "contact_tile" in _kernels(h, 2)
+
+
+@pytest.mark.parametrize("contact", ["1", "0"])
+def test_solve_stops_on_a_nonfinite_state(monkeypatch, contact):
+    """A NaN in the state (the uploaded N and Z of circle 2's radius edge)
+    makes the residuals NaN (in the contact state through its flag of lost
+    circle sums): the solve stops at the first check, not converged."""
+    monkeypatch.setenv("PARADMM_CONTACT", contact)
+    g, _ = square_pair(60, 4, rho=20.0)
+    g.array("N")[6:9] = np.nan  # edge 2: the radius factor of circle 2
+    g.array("Z")[6:9] = np.nan
+    g.sync_from_host()
+    r = g.solve(5000, tol=1e-4, tol_feas=1e-4, check_every=50)
+    print(f"contact={contact}: nonfinite state stopped at {r['iterations']}, primal {r['primal_rms']}")
+    assert r["iterations"] == 50 and r["converged"] == 0 and np.isnan(r["primal_rms"])
