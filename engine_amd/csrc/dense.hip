@@ -38,6 +38,7 @@
 #include "engine_internal.h"
 #include "engine_state.h"
 #include "paradmm_gpu.h"
+#include "mlp_device.h"
 #include "pairtile.h"
 
 #define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hip_fail(e_, #x, __FILE__, __LINE__); } while (0)
@@ -82,6 +83,11 @@ struct DenseState {
     double* d_Rpc = nullptr;   // [N] primal residual per circle
     float* d_Zpp = nullptr;    // [N][3] Z two iterations back
     unsigned* d_flag = nullptr; // a circle sum lost a value (non-finite or out of range) since the state was entered
+
+    // learned rho applied inside the contact state (dense_contact_policy)
+    bool c_base_ok = false;    // d_base holds every circle's unlisted pair edges' rho
+    float* d_base = nullptr;   // [N] that rho
+    void* d_cpol = nullptr;    // [N] rows, [3][N] outputs, [N] new rho, [N] factor, sums
 };
 
 // one workgroup per tile of side T; UNI: one rho and one alpha for every edge
@@ -953,6 +959,370 @@ __global__ __launch_bounds__(256) void k_contact_leave(CArgs a)
     }
 }
 
+// ------------------------------------------------------------------ learned rho in the contact state
+//
+// paradmm_gpu_solve applies the rho policy (modes 2 and 3: one rho per graph
+// from the mean of the network's log10 rho over all edges, in mode 3 plus the
+// edges whose own output exceeds that mean by more than raise_log) without
+// leaving the contact state.  At a policy point (the end of a run: lists par
+// = M_j of the last iteration and other = M_{j-1}, Z = Z_j, Z_prev = Z_{j-1},
+// Z_prev2 = Z_{j-2}) an edge's policy inputs are those of the steady state
+// k_contact_leave would materialise: X_j = M_j - U_{j-1} with
+// U_{j-1} = M_{j-1} - Z_{j-1}, where M = Z of the circle off the lists.
+// A pair edge in neither list has X_j = Z_{j-1} - (Z_{j-2} - Z_{j-1}) and, if
+// its rho is its circle's base rho (the rho every unlisted pair edge of the
+// circle took at the last application), exactly the inputs of every other
+// such edge of the circle: the network runs once per circle for them, and
+// once per edge for the listed pairs, the pairs of either list, the pairs
+// off their circles' base rho, and the box and radius edges.
+//
+// The scaled dual U_j = M_j - Z_j is rescaled by rho_old / rho_new: a listed
+// pair's M becomes Z + s (M - Z); the box and radius edges' stored U (one
+// update behind) becomes s (X + U - Z) - X + Z; an unlisted pair edge's
+// U_j = Z_{j-1} - Z_j is implicit in its circle's Z_{j-1}, so the circle's
+// Z_prev becomes Z - s (Z - Z_prev), a virtual Z_{j-1} that the next contact
+// iteration reads for every unlisted pair (and consistently in its residual
+// terms).  An unlisted pair whose edges' factors are not their circles'
+// joins the list with its rescaled M.  The virtual Z_prev is valid only for
+// the next contact iteration: the solve applies the policy only where more
+// iterations follow, and no state is left in between.
+
+constexpr float CPOL_FIX = 268435456.0f;  // network outputs summed in units of 2^-28
+
+struct CPolArgs {
+    CArgs c;
+    const float* params;   // the network's parameters
+    float frac;            // iteration fraction (network input)
+    float raise_log;       // mode 3: an edge keeps its own rho above graph rho * 10^raise_log; +inf in mode 2
+    int exceptions;        // rho may differ from a circle's base rho on unlisted pairs (per-edge rho)
+    float* base;           // [N] base rho of each circle's unlisted pair edges
+    int* rows;             // [N] pair edges of each circle evaluated on their own
+    float* yc;             // [3][N] outputs: unlisted pair edges, box edge, radius edge
+    float* rnew;           // [N] new rho of each circle's unlisted pair edges
+    float* sfac;           // [N] their factor rho_old / rho_new
+    long long* ysum;       // [1] sum of the outputs over all edges (units of 2^-28)
+    double* hist;          // [6] running sums of the per-type means (shared with the per-edge policy)
+    float* gl;             // [2] the new graph rho and its log10
+    int average;           // running mean of the estimates
+};
+
+__device__ __forceinline__ long long cpol_fix(float y) { return (long long)rintf(y * CPOL_FIX); }
+
+// policy inputs of an edge from its X, its circle's Z and Z_prev and its rho
+__device__ __forceinline__ void cpol_features(const float (&x)[3], const float* z, const float* zp, float rho, int type,
+                                              float frac, float (&f)[MLP_NIN])
+{
+    // no contraction: the per-edge application (mlp.hip) computes the same
+    // inputs bit for bit
+#pragma clang fp contract(off)
+    float r2 = 0.0f, dz2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float d = x[k] - z[k];
+        const float q = z[k] - zp[k];
+        r2 += d * d;
+        dz2 += q * q;
+    }
+    edge_features(sqrtf(r2), rho * sqrtf(dz2), rho, type, frac, f);
+}
+
+// One contact tile: the pairs evaluated on their own, in a fixed order (the
+// par list, then the pairs of the other list only, then with per-edge rho
+// the unlisted pairs off their circles' base rho), 32 pairs (64 edges) per
+// network step.  !APPLY: the outputs' sum and each circle's count of such
+// edges.  APPLY: each edge's new rho and the rescaled M (in place for the par
+// list, appended for an unlisted pair whose factors are not its circles'),
+// then every other pair of the tile takes its circles' new rho (after the
+// evaluated pairs have read their old one).
+template <int CS, bool APPLY>
+__global__ __launch_bounds__(64) void k_cpol_tile(CPolArgs p)
+{
+    constexpr int CSS = CS * CS, W = CS / 32;
+    __shared__ MlpShared w;
+    __shared__ TileScratch ts;
+    __shared__ float yv[64];
+    __shared__ float zc[2 * CS][9];       // Z, Z_prev, Z_prev2 of the tile's circles
+    __shared__ unsigned short omap[CSS];  // other list: pair -> entry
+    __shared__ unsigned omask[CS][W], pmask[CS][W];
+    __shared__ unsigned q[CSS];           // the tile's pairs to evaluate: kind << 16 | entry or pair
+    __shared__ unsigned rmask[CS][W];     // APPLY: the pairs in q
+    const CArgs& a = p.c;
+    const int lane = threadIdx.x;
+    const int t = blockIdx.x;
+    const int N = a.N;
+    const long TT = (long)a.T * a.T;
+    int P, Qb;
+    tri_tile(t, a.nbs, P, Qb);
+    const bool diag = P == Qb;
+    const long tp = (long)a.par * a.nts + t, to = (long)(a.par ^ 1) * a.nts + t;
+    float4* Lp = a.L + tp * (2 * CSS);
+    const float4* Lo = a.L + to * (2 * CSS);
+    const int cp = min(a.cnt[tp], CSS), co = min(a.cnt[to], CSS);
+    stage_weights(p.params, w);
+    for (int e = lane; e < 2 * CS; e += 64) {
+        const int cc = min(e < CS ? P * CS + e : Qb * CS + (e - CS), N - 1);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            zc[e][k] = a.Z[3 * cc + k];
+            zc[e][3 + k] = a.Zp[3 * cc + k];
+            zc[e][6 + k] = a.Zpp[3 * cc + k];
+        }
+    }
+    for (int e = lane; e < CS * W; e += 64) {
+        (&pmask[0][0])[e] = a.Lmask[tp * CS * W + e];
+        (&omask[0][0])[e] = a.Lmask[to * CS * W + e];
+    }
+    for (int e = lane; e < co; e += 64) omap[__float_as_int(Lo[2 * e].w) & (CSS - 1)] = (unsigned short)e;
+    if (APPLY)
+        for (int e = lane; e < CS * W; e += 64) (&rmask[0][0])[e] = 0u;
+    __syncthreads();
+    auto inmask = [&](const unsigned (&m)[CS][W], int idx) { return (m[idx / CS][(idx % CS) >> 5] >> (idx & 31)) & 1u; };
+    // the queue: par entries, other-only entries, off-base unlisted pairs
+    for (int e = lane; e < cp; e += 64) q[e] = (unsigned)e;
+    int n = cp;
+    for (int e0 = 0; e0 < co; e0 += 64) {
+        const int e = e0 + lane;
+        bool take = false;
+        if (e < co) take = !inmask(pmask, __float_as_int(Lo[2 * e].w) & (CSS - 1));
+        const unsigned long long b = __ballot(take);
+        const int at = n + (int)__popcll(b & ((1ull << lane) - 1ull));
+        if (take && at < CSS) q[at] = (1u << 16) | (unsigned)e;
+        n = min(n + (int)__popcll(b), CSS);
+    }
+    if (p.exceptions) {
+        for (int i0 = 0; i0 < CSS; i0 += 64) {
+            const int idx = i0 + lane;
+            const int il = idx / CS, jl = idx % CS;
+            const int ci = P * CS + il, cj = Qb * CS + jl;
+            bool take = false;
+            if (ci < N && cj < N && (!diag || il < jl) && !inmask(pmask, idx) && !inmask(omask, idx)) {
+                const long qa = dense_pair_slot(ci, cj, a.nb, a.T);
+                take = a.rho[qa] != p.base[ci] || a.rho[qa + TT] != p.base[cj];
+            }
+            const unsigned long long b = __ballot(take);
+            const int at = n + (int)__popcll(b & ((1ull << lane) - 1ull));
+            if (take && at < CSS) q[at] = (2u << 16) | (unsigned)idx;
+            n = min(n + (int)__popcll(b), CSS);
+        }
+    }
+    __syncthreads();
+    if (APPLY) {
+        for (int e = lane; e < n; e += 64) {
+            const unsigned code = q[e];
+            const int k2 = (int)(code >> 16), en = (int)(code & 0xffffu);
+            const int idx = k2 == 2 ? en : __float_as_int((k2 == 0 ? Lp : Lo)[2 * en].w) & (CSS - 1);
+            atomicOr(&rmask[idx / CS][(idx % CS) >> 5], 1u << (idx & 31));
+        }
+        __syncthreads();
+    }
+    const float g = APPLY ? p.gl[0] : 0.0f, lg = APPLY ? p.gl[1] : 0.0f;
+    long long acc = 0;
+    for (int q0 = 0; q0 < n; q0 += 32) {
+        const int qi = q0 + (lane & 31);
+        const int side = lane >> 5;  // 0: the pair's first circle (row), 1: its second (column)
+        const bool valid = qi < n;
+        float f[MLP_NIN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        int kind = 0, ent = 0, idx = 0, zr = 0, circ = 0;
+        long slot = 0;
+        float m[3] = {0.0f, 0.0f, 0.0f}, rho = 1.0f;
+        if (valid) {
+            const unsigned code = q[qi];
+            kind = (int)(code >> 16);
+            ent = (int)(code & 0xffffu);
+            idx = kind == 2 ? ent : __float_as_int((kind == 0 ? Lp : Lo)[2 * ent].w) & (CSS - 1);
+            const int il = idx / CS, jl = idx % CS;
+            zr = side ? CS + jl : il;
+            circ = side ? Qb * CS + jl : P * CS + il;
+            slot = dense_pair_slot(P * CS + il, Qb * CS + jl, a.nb, a.T) + (side ? TT : 0);
+            rho = a.rho[slot];
+            // M_j: the par entry, else Z_{j-1}; M_{j-1}: the other entry, else Z_{j-2}
+            float mo[3];
+            const bool in_o = kind == 1 || (kind == 0 && inmask(omask, idx));
+            const float4 eo = in_o ? Lo[2 * (kind == 1 ? ent : omap[idx]) + side] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 ep = kind == 0 ? Lp[2 * ent + side] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float eov[3] = {eo.x, eo.y, eo.z}, epv[3] = {ep.x, ep.y, ep.z};
+            float x[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                mo[k] = in_o ? eov[k] : zc[zr][6 + k];
+                m[k] = kind == 0 ? epv[k] : zc[zr][3 + k];
+                const float u = mo[k] - zc[zr][3 + k];
+                x[k] = m[k] - u;
+            }
+            cpol_features(x, &zc[zr][0], &zc[zr][3], rho, PDM_EDGE_PAIR, p.frac, f);
+        }
+        const float y = policy_wave_eval(w, ts, yv, f, lane);
+        if constexpr (!APPLY) {
+            if (valid) {
+                acc += cpol_fix(y);
+                atomicAdd(&p.rows[circ], 1);
+            }
+        } else {
+            float rn = 0.0f, sf = 0.0f;
+            if (valid) {
+                rn = y - lg > p.raise_log ? exp10f(y) : g;
+                sf = rho / rn;
+            }
+            // an unlisted pair stays implicit if both edges take their circles' factors
+            const bool own = valid && (rho != p.base[circ] || rn != p.rnew[circ]);
+            const bool both_own = own || __shfl_xor(own ? 1 : 0, 32, 64);
+            float mn[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) mn[k] = zc[zr][k] + sf * (m[k] - zc[zr][k]);
+            if (valid) const_cast<float*>(a.rho)[slot] = rn;  // every evaluated edge, implicit or not
+            if (valid && kind == 0) Lp[2 * ent + side] = make_float4(mn[0], mn[1], mn[2], side ? 0.0f : __int_as_float(idx));
+            // the appended entries: lane of side 0 reserves the slot
+            const bool app = valid && kind != 0 && both_own;
+            int at = 0;
+            if (app && side == 0) at = atomicAdd(&a.cnt[tp], 1);
+            at = __shfl(at, lane & 31, 64);
+            if (app && at >= CSS) {  // cannot happen for distinct pairs: the state is lost
+                if (side == 0) atomicOr(a.flag, 1u);
+            } else if (app) {
+                Lp[2 * at + side] = make_float4(mn[0], mn[1], mn[2], side ? 0.0f : __int_as_float(idx));
+                if (side == 0) atomicOr(&a.Lmask[tp * CS * W + (idx / CS) * W + ((idx % CS) >> 5)], 1u << (idx & 31));
+            }
+        }
+    }
+    if constexpr (!APPLY) {
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0 && acc) atomicAdd((unsigned long long*)p.ysum, (unsigned long long)acc);
+    } else {
+        // every pair not evaluated on its own: its circles' new rho
+        for (int idx = lane; idx < CSS; idx += 64) {
+            const int il = idx / CS, jl = idx % CS;
+            const int ci = P * CS + il, cj = Qb * CS + jl;
+            if (ci >= N || cj >= N || (diag && il >= jl) || inmask(rmask, idx)) continue;
+            const long qa = dense_pair_slot(ci, cj, a.nb, a.T);
+            const_cast<float*>(a.rho)[qa] = p.rnew[ci];
+            const_cast<float*>(a.rho)[qa + TT] = p.rnew[cj];
+        }
+    }
+}
+
+// Per circle (64 per workgroup, one per thread): the network on the circle's
+// unlisted pair edges (weighted by their count), its box edge and its
+// radius edge.  Each wave pushes its 64 lanes' rows through the network
+// three times.
+__global__ __launch_bounds__(64) void k_cpol_circles(CPolArgs p)
+{
+    __shared__ MlpShared w;
+    __shared__ TileScratch ts;
+    __shared__ float yv[64];
+    const CArgs& a = p.c;
+    stage_weights(p.params, w);
+    __syncthreads();
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    const bool valid = c < a.N;
+    const long TT = (long)a.T * a.T;
+    const long q0 = valid ? dense_unary_slot(c, a.nb, a.T) : 0;
+    float z[3] = {0.f, 0.f, 0.f}, zp[3] = {0.f, 0.f, 0.f}, zpp[3] = {0.f, 0.f, 0.f};
+    if (valid)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            z[k] = a.Z[3 * c + k];
+            zp[k] = a.Zp[3 * c + k];
+            zpp[k] = a.Zpp[3 * c + k];
+        }
+    long long acc = 0;
+    for (int kind = 0; kind < 3; ++kind) {
+        float f[MLP_NIN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        long long wgt = 0;
+        if (valid) {
+            float x[3], rho;
+            if (kind == 0) {  // an unlisted pair edge: X_j = Z_{j-1} - (Z_{j-2} - Z_{j-1})
+#pragma unroll
+                for (int k = 0; k < 3; ++k) x[k] = zp[k] - (zpp[k] - zp[k]);
+                rho = p.base[c];
+                wgt = (long long)(a.N - 1 - p.rows[c]);
+            } else {
+                const long q = q0 + (kind == 2 ? TT : 0);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) x[k] = a.X[k * a.stride + q];
+                rho = a.rho[q];
+                wgt = 1;
+            }
+            cpol_features(x, z, zp, rho, kind == 0 ? PDM_EDGE_PAIR : kind == 1 ? PDM_EDGE_CONTAINER : PDM_EDGE_RADIUS,
+                          p.frac, f);
+        }
+        const float y = policy_wave_eval(w, ts, yv, f, threadIdx.x);
+        if (valid) {
+            p.yc[(long)kind * a.N + c] = y;
+            acc += wgt * cpol_fix(y);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (threadIdx.x == 0 && acc) atomicAdd((unsigned long long*)p.ysum, (unsigned long long)acc);
+}
+
+// The graph rho: 10^(mean output over all edges), or with average the mean
+// of this and the earlier estimates (the per-type policy's running sums,
+// every type the same).
+__global__ void k_cpol_final(CPolArgs p, long E)
+{
+    if (threadIdx.x != 0) return;
+    double m = (double)(*p.ysum) / (double)CPOL_FIX / (double)E;
+    if (p.average) {
+        for (int k = 0; k < 3; ++k) {
+            p.hist[2 * k] += m;
+            p.hist[2 * k + 1] += 1.0;
+        }
+        m = p.hist[0] / p.hist[1];
+    }
+    const float g = exp10f((float)m);
+    p.gl[0] = g;
+    p.gl[1] = log10f(g);
+}
+
+// Per circle: the unlisted pair edges' new rho and factor; the box and
+// radius edges' new rho with their stored U rescaled.
+__global__ __launch_bounds__(256) void k_cpol_apply_circles(CPolArgs p)
+{
+    const CArgs& a = p.c;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= a.N) return;
+    const float g = p.gl[0], lg = p.gl[1];
+    const float y0 = p.yc[c];
+    const float rn = y0 - lg > p.raise_log ? exp10f(y0) : g;
+    p.rnew[c] = rn;
+    p.sfac[c] = p.base[c] / rn;
+    const long TT = (long)a.T * a.T, q0 = dense_unary_slot(c, a.nb, a.T);
+    for (int kind = 1; kind < 3; ++kind) {
+        const long q = q0 + (kind == 2 ? TT : 0);
+        const float y = p.yc[(long)kind * a.N + c];
+        const float r = y - lg > p.raise_log ? exp10f(y) : g;
+        const float sf = a.rho[q] / r;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float x = a.X[k * a.stride + q], z = a.Z[3 * c + k];
+            a.U[k * a.stride + q] = sf * ((x + a.U[k * a.stride + q]) - z) - x + z;
+        }
+        const_cast<float*>(a.rho)[q] = r;
+    }
+}
+
+// Per circle: the virtual Z_{j-1} that carries the unlisted pair edges'
+// rescaled U, and the base rho for the next application.
+__global__ __launch_bounds__(256) void k_cpol_zprev(CPolArgs p)
+{
+    const CArgs& a = p.c;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= a.N) return;
+    const float sf = p.sfac[c];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float z = a.Z[3 * c + k];
+        a.Zp[3 * c + k] = z - sf * (z - a.Zp[3 * c + k]);
+    }
+    p.base[c] = p.rnew[c];
+}
+
+__global__ void k_fill_f32(float* p, long n, const float* v)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = *v;
+}
+
 // ------------------------------------------------------------------ host side
 
 static inline long tile_index(int I, int J, int nb) { return (long)I * nb - (long)I * (I - 1) / 2 + (J - I); }
@@ -1286,6 +1656,7 @@ static int contact_alloc(DevGraph* s)
     RC(dmalloc(s, (void**)&d->d_Zpp, sizeof(float) * 3 * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_W, sizeof(float) * 2 * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_flag, sizeof(unsigned)));
+    RC(dmalloc(s, (void**)&d->d_base, sizeof(float) * (size_t)d->N));
     return PARADMM_OK;
 }
 
@@ -1355,6 +1726,68 @@ static int contact_iteration(DevGraph* s, bool res)
 
 int dense_contact_parity(DevGraph* s) { return s->dense ? s->dense->c_par : 0; }
 
+bool dense_contact_policy_ok(DevGraph* s)
+{
+    const DenseState* d = s->dense;
+    if (!d || !d->c_state || !d->c_base_ok) return false;
+    const char* e = getenv("PARADMM_CONTACT_POLICY");
+    return !(e && e[0] == '0');
+}
+
+int dense_contact_policy(DevGraph* s, const float* params, float frac, int mode, bool average, double* hist)
+{
+    DenseState* d = s->dense;
+    const int N = d->N;
+    if (!d->d_cpol) RC(dmalloc(s, &d->d_cpol, 16 + sizeof(int) * (size_t)N + sizeof(float) * 6 * (size_t)N));
+    CPolArgs p;
+    p.c = cargs(s);
+    p.params = params;
+    p.frac = frac;
+    p.raise_log = mode == 3 ? 0.5f : __builtin_inff();
+    p.exceptions = d->c_rhoe ? 1 : 0;
+    p.base = d->d_base;
+    p.ysum = (long long*)d->d_cpol;
+    p.gl = (float*)((char*)d->d_cpol + 8);
+    p.rows = (int*)((char*)d->d_cpol + 16);
+    p.yc = (float*)(p.rows + N);
+    p.rnew = p.yc + 3 * (size_t)N;
+    p.sfac = p.rnew + N;
+    p.hist = hist;
+    p.average = average ? 1 : 0;
+    // the sum and the row counts start at zero (fill kernels: the solve may
+    // be captured by a caller)
+    k_fill_u32<<<(unsigned)((N + 4 + 255) / 256), 256, 0, s->stream>>>((unsigned*)d->d_cpol, N + 4, 0u);
+    CK(hipGetLastError());
+    const unsigned ctg = (unsigned)((N + 63) / 64), cg = (unsigned)((N + 255) / 256);
+    prof_begin(s, "cpol_rows");
+    if (d->cs == 64) k_cpol_tile<64, false><<<(unsigned)d->nts, 64, 0, s->stream>>>(p);
+    else k_cpol_tile<32, false><<<(unsigned)d->nts, 64, 0, s->stream>>>(p);
+    prof_end(s);
+    CK(hipGetLastError());
+    prof_begin(s, "cpol_circles");
+    k_cpol_circles<<<ctg, 64, 0, s->stream>>>(p);
+    prof_end(s);
+    CK(hipGetLastError());
+    k_cpol_final<<<1, 64, 0, s->stream>>>(p, (long)s->E);
+    CK(hipGetLastError());
+    k_cpol_apply_circles<<<cg, 256, 0, s->stream>>>(p);
+    CK(hipGetLastError());
+    prof_begin(s, "cpol_apply");
+    if (d->cs == 64) k_cpol_tile<64, true><<<(unsigned)d->nts, 64, 0, s->stream>>>(p);
+    else k_cpol_tile<32, true><<<(unsigned)d->nts, 64, 0, s->stream>>>(p);
+    prof_end(s);
+    CK(hipGetLastError());
+    k_cpol_zprev<<<cg, 256, 0, s->stream>>>(p);
+    CK(hipGetLastError());
+    // rho per edge from here: the circles' sums of rho and rho^2
+    prof_begin(s, "contact_weights");
+    k_contact_weights<<<(unsigned)N, 64, 0, s->stream>>>(cargs(s));
+    prof_end(s);
+    CK(hipGetLastError());
+    d->c_rhoe = true;
+    return PARADMM_OK;
+}
+
 int dense_contact_info(DevGraph* s, int* tile_side, long* listed_pairs)
 {
     DenseState* d = s->dense;
@@ -1395,6 +1828,12 @@ int dense_contact_enter(DevGraph* s)
     CK(hipGetLastError());
     // one rho everywhere (a policy's raised edges make it per edge)
     d->c_rhoe = !(s->rho_uni && s->policy_raised == 0);
+    // with one rho it is every circle's base rho (learned rho in the state)
+    d->c_base_ok = !d->c_rhoe;
+    if (d->c_base_ok) {
+        k_fill_f32<<<(unsigned)((d->N + 255) / 256), 256, 0, s->stream>>>(d->d_base, d->N, d->d_uni);
+        CK(hipGetLastError());
+    }
     const CArgs a = cargs(s);
     if (d->c_rhoe) {
         prof_begin(s, "contact_weights");

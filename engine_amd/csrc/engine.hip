@@ -1514,10 +1514,6 @@ extern "C" int paradmm_gpu_solve(graph* G, const paradmm_solve_opts* o, paradmm_
         RC(run_fused(s, n));
         it += n;
         since_policy += n;
-        if (o->policy_every > 0 && s->rho_policy && since_policy >= o->policy_every) {
-            RC(s->rho_policy(s, (float)it / (float)o->max_iters));
-            since_policy = 0;
-        }
         RC(residual_reduce(s));
         double h[2];
         CK(hipMemcpyAsync(h, s->d_res, sizeof(h), hipMemcpyDeviceToHost, s->stream));
@@ -1541,6 +1537,15 @@ extern "C" int paradmm_gpu_solve(graph* G, const paradmm_solve_opts* o, paradmm_
                 r->converged = 1;
                 break;
             }
+        }
+        // the policy sets rho for the iterations that follow (none after the
+        // last chunk); the residuals above do not depend on it
+        if (o->policy_every > 0 && s->rho_policy && since_policy >= o->policy_every && it < o->max_iters) {
+            s->policy_in_solve = true;
+            const int prc = s->rho_policy(s, (float)it / (float)o->max_iters);
+            s->policy_in_solve = false;
+            RC(prc);
+            since_policy = 0;
         }
     }
     if (o->tol_feas >= 0 && !r->converged) {

@@ -121,6 +121,7 @@ struct DevGraph {
     float* d_policy_y = nullptr;       // per-slot network output (mode 3)
     int policy_raised = 0;             // edges raised by the last mode-3 application
     bool policy_average = false;       // average the estimates over the applications so far
+    bool policy_in_solve = false;      // applied by paradmm_gpu_solve, with iterations to follow
     void* d_policy_part = nullptr;     // per-type partial sums of that mean
     int* d_slot_var = nullptr;         // device slot -> variable (-1: padding), policy kernels
     unsigned char* d_slot_type = nullptr;
@@ -208,6 +209,12 @@ int dense_contact_leave(DevGraph* s);
 // argument, so a captured run is bound to its start parity and leaves the
 // parity flipped by its length
 int dense_contact_parity(DevGraph* s);
+// whether the contact state can take a rho policy application itself
+// (every circle's unlisted pair edges share one rho)
+bool dense_contact_policy_ok(DevGraph* s);
+// one application of the rho policy (mode 2 or 3) inside the contact state;
+// valid only when a contact iteration follows before the state is left
+int dense_contact_policy(DevGraph* s, const float* params, float frac, int mode, bool average, double* hist);
 // contact tile side and pairs in the last list (0, 0 outside the contact state)
 int dense_contact_info(DevGraph* s, int* tile_side, long* listed_pairs);
 void dense_contact_set_parity(DevGraph* s, int p);

@@ -559,6 +559,9 @@ struct PolicyArgs {
 // residuals of the edge in device slot p (coalesced in slot order)
 __device__ __forceinline__ void slot_residuals(const PolicyArgs& a, long p, int v, float& r, float& s, float& rho)
 {
+    // no contraction: the contact state's own application (dense.hip)
+    // computes the same inputs bit for bit
+#pragma clang fp contract(off)
     float r2 = 0.0f, dz2 = 0.0f;
     for (int k = 0; k < a.D; ++k) {
         const float z = a.Z[v * a.D + k];
@@ -588,6 +591,7 @@ __global__ __launch_bounds__(256) void k_build_slot_maps(int E, const int* __res
 __device__ __forceinline__ void edge_residuals(const PolicyArgs& a, int e, float& r, float& s, float& rho, int& type,
                                                int& p)
 {
+#pragma clang fp contract(off)
     p = a.pos[e];
     const int v = a.edge_var[e];
     float r2 = 0.0f, dz2 = 0.0f;
@@ -876,10 +880,31 @@ static int current_u(DevGraph* s)
     return PARADMM_OK;
 }
 
+// running sums [3][2] (FP64), partial sums [8192][3] (FP64), counts
+// [8192][3], rho per type [3], raised count
+static int ensure_policy_part(DevGraph* s)
+{
+    if (s->d_policy_part) return PARADMM_OK;
+    RC(dmalloc(s, (void**)&s->d_policy_part, sizeof(double) * (6 + 3 * 8192) + sizeof(int) * 3 * 8192 + 16));
+    CK(hipMemsetAsync(s->d_policy_part, 0, sizeof(double) * 6, s->stream));
+    return PARADMM_OK;
+}
+
 static int apply_policy(DevGraph* s, float frac)
 {
     paradmm_rho_mlp* m = (paradmm_rho_mlp*)s->policy_ctx;
     if (!m) return PARADMM_OK;
+    if (s->policy_in_solve && s->layout == 2 && (s->policy_per_type == 2 || s->policy_per_type == 3) &&
+        dense_contact_policy_ok(s)) {
+        // inside the contact state: no per-edge pass over the whole graph
+        RC(ensure_policy_part(s));
+        RC(dense_contact_policy(s, mlp_device_params(m), frac, s->policy_per_type, s->policy_average,
+                                (double*)s->d_policy_part));
+        s->rho_device_written = true;
+        s->rho_uni = false;
+        s->policy_raised = 1;  // not counted here: rho is treated as per edge
+        return PARADMM_OK;
+    }
     RC(current_u(s));
     RC(ensure_slot_maps(s));
     PolicyArgs a = policy_args(s, frac);
@@ -891,12 +916,7 @@ static int apply_policy(DevGraph* s, float frac)
     if (!s->policy_per_type) {
         k_rho_policy<<<grid, 256, 0, s->stream>>>(a, mlp_device_params(m));
     } else {
-        if (!s->d_policy_part) {
-            // running sums [3][2] (FP64), partial sums [grid][3] (FP64),
-            // counts [grid][3], rho per type [3]
-            RC(dmalloc(s, (void**)&s->d_policy_part, sizeof(double) * (6 + 3 * 8192) + sizeof(int) * 3 * 8192 + 16));
-            CK(hipMemsetAsync(s->d_policy_part, 0, sizeof(double) * 6, s->stream));
-        }
+        RC(ensure_policy_part(s));
         double* hist = (double*)s->d_policy_part;
         double* part = hist + 6;
         int* cnt = (int*)(part + 3 * 8192);

@@ -362,3 +362,57 @@ def test_solve_stops_on_a_nonfinite_state(monkeypatch, contact):
     r = g.solve(5000, tol=1e-4, tol_feas=1e-4, check_every=50)
     print(f"contact={contact}: nonfinite state stopped at {r['iterations']}, primal {r['primal_rms']}")
     assert r["iterations"] == 50 and r["converged"] == 0 and np.isnan(r["primal_rms"])
+
+
+@pytest.mark.parametrize("nc,seed", [(150, 6), (300, 2)])
+def test_contact_policy_matches_per_edge_application(monkeypatch, nc, seed):
+    """Inside paradmm_gpu_solve the learned policy (mode 3, running mean) is
+    applied in the contact state itself: the network once per circle for
+    its unlisted pair edges and once per listed edge, U rescaled through the
+    lists and a virtual Z_prev.  From the same state it computes what the
+    application over the materialised per-edge state computes
+    (PARADMM_CONTACT_POLICY=0): the same rho on every edge and, one
+    iteration later, the same Z and U to fp32 rounding.  Checked for the
+    first application (one rho everywhere) and for a second one after a
+    first in the contact state (per-edge rho, pairs off their circles' base
+    rho)."""
+    from engine_amd import rho_train as rt
+    m = rt.load_policy()
+
+    def run(second):
+        g, _ = square_pair(nc, seed, rho=rt.LEARNED_START_RHO)
+        g.prepare()
+        g.set_rho_policy(m)
+        g.set_rho_policy_mode(rt.DEFAULT_MODE, rt.DEFAULT_AVERAGE)
+        if second is not None:
+            # an identical first stretch (its application in the state)
+            monkeypatch.setenv("PARADMM_CONTACT_POLICY", "1")
+            g.solve(100, tol=-1.0, tol_feas=-1.0, check_every=50, policy_every=50)
+        monkeypatch.setenv("PARADMM_CONTACT_POLICY", second or first_env[0])
+        # one application after 50 iterations, then one iteration
+        r = g.solve(51, tol=-1.0, tol_feas=-1.0, check_every=50, policy_every=50)
+        assert r["iterations"] == 51
+        native = (second or first_env[0]) == "1"
+        assert g.contact_info()[0] or not native  # the contact-state application never leaves it
+        res = (g.read("RHO"), g.read("Z"), g.read("U"))
+        g.close()
+        return res
+
+    for which in ("first", "second"):
+        out = {}
+        for env in ("1", "0"):
+            first_env = [env]
+            out[env] = run(None if which == "first" else env)
+        (ra, za, ua), (rb, zb, ub) = out["1"], out["0"]
+        same = np.isclose(ra, rb, rtol=1e-5)
+        zerr, uerr = float(np.max(np.abs(za - zb))), float(np.max(np.abs(ua - ub)))
+        print(f"N={nc} {which} application: graph rho {np.median(ra):.5f} / {np.median(rb):.5f}; raised edges "
+              f"{np.sum(ra > 1.5 * np.median(ra))} / {np.sum(rb > 1.5 * np.median(rb))}; edges with the same rho "
+              f"{np.mean(same):.6f}; one iteration later |dZ| {zerr:.1e} |dU| {uerr:.1e}")
+        assert np.median(ra) != rt.LEARNED_START_RHO
+        assert np.all(same)
+        # a rounding-level difference can flip one overlap test, whose edge
+        # (possibly of a raised rho) then moves its circle by up to ~1e-5
+        assert zerr < 1e-4 and uerr < 1e-3
+        assert not np.array_equal(za, zb)  # the two applications ran different code
+    m.close()
