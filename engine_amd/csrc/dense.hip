@@ -988,6 +988,8 @@ __global__ __launch_bounds__(256) void k_contact_leave(CArgs a)
 // iterations follow, and no state is left in between.
 
 constexpr float CPOL_FIX = 268435456.0f;  // network outputs summed in units of 2^-28
+constexpr float CPOL_WFIX = 16777216.0f;  // sums of rho in units of 2^-24
+constexpr float CPOL_W2FIX = 4096.0f;     // sums of rho^2 in units of 2^-12
 
 struct CPolArgs {
     CArgs c;
@@ -995,18 +997,25 @@ struct CPolArgs {
     float frac;            // iteration fraction (network input)
     float raise_log;       // mode 3: an edge keeps its own rho above graph rho * 10^raise_log; +inf in mode 2
     int exceptions;        // rho may differ from a circle's base rho on unlisted pairs (per-edge rho)
+    int average;           // running mean of the estimates
     float* base;           // [N] base rho of each circle's unlisted pair edges
-    int* rows;             // [N] pair edges of each circle evaluated on their own
     float* yc;             // [3][N] outputs: unlisted pair edges, box edge, radius edge
-    float* rnew;           // [N] new rho of each circle's unlisted pair edges
-    float* sfac;           // [N] their factor rho_old / rho_new
-    long long* ysum;       // [1] sum of the outputs over all edges (units of 2^-28)
+    int* rows;             // [N] pair edges of each circle evaluated on their own
+    long long* wacc;       // [2][N] per circle, over its pair edges evaluated on their own: sum of
+                           // (rho_new - circle's new unlisted rho) (2^-24) and of the same for rho^2 (2^-12)
+    long long* ysum;       // [1] sum of the outputs of the pair edges evaluated on their own (units of 2^-28)
     double* hist;          // [6] running sums of the per-type means (shared with the per-edge policy)
     float* gl;             // [2] the new graph rho and its log10
-    int average;           // running mean of the estimates
 };
 
 __device__ __forceinline__ long long cpol_fix(float y) { return (long long)rintf(y * CPOL_FIX); }
+
+// the new rho of an edge whose network output is y: its own where it exceeds
+// the graph rho by more than the raise factor, else the graph rho
+__device__ __forceinline__ float cpol_rho(const CPolArgs& p, float y, float g, float lg)
+{
+    return y - lg > p.raise_log ? exp10f(y) : g;
+}
 
 // policy inputs of an edge from its X, its circle's Z and Z_prev and its rho
 __device__ __forceinline__ void cpol_features(const float (&x)[3], const float* z, const float* zp, float rho, int type,
@@ -1026,28 +1035,54 @@ __device__ __forceinline__ void cpol_features(const float (&x)[3], const float* 
     edge_features(sqrtf(r2), rho * sqrtf(dz2), rho, type, frac, f);
 }
 
-// One contact tile: the pairs evaluated on their own, in a fixed order (the
-// par list, then the pairs of the other list only, then with per-edge rho
-// the unlisted pairs off their circles' base rho), 32 pairs (64 edges) per
-// network step.  !APPLY: the outputs' sum and each circle's count of such
-// edges.  APPLY: each edge's new rho and the rescaled M (in place for the par
-// list, appended for an unlisted pair whose factors are not its circles'),
-// then every other pair of the tile takes its circles' new rho (after the
-// evaluated pairs have read their old one).
+// Appends the items a workgroup's threads take (in thread order) to q at n;
+// returns the new count (every thread).  Two __syncthreads.
+template <int CSS>
+__device__ __forceinline__ int cpol_append(unsigned short (&q)[CSS], int n, bool take, unsigned short code,
+                                           int (&wsum)[4])
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long b = __ballot(take);
+    if (lane == 0) wsum[wv] = (int)__popcll(b);
+    __syncthreads();
+    int base = n, total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        base += k < wv ? wsum[k] : 0;
+        total += wsum[k];
+    }
+    const int at = base + (int)__popcll(b & ((1ull << lane) - 1ull));
+    if (take && at < CSS) q[at] = code;
+    __syncthreads();
+    return min(n + total, CSS);
+}
+
+// One contact tile, four wavefronts: the pairs evaluated on their own, in a
+// fixed order (the par list, then the pairs of the other list only, then
+// with per-edge rho the unlisted pairs off their circles' base rho), 32
+// pairs (64 edges) per wavefront and network step.  !APPLY: their outputs'
+// sum and each circle's count of them; a diagonal tile also evaluates its
+// circles' unlisted pair edges, box and radius edges.  APPLY: each edge's
+// new rho and the rescaled M (in place for the par list, appended for an
+// unlisted pair whose factors are not its circles'), the circles' rho sums
+// corrected for them; then every other pair of the tile takes its circles'
+// new rho (after the evaluated pairs have read their old one), and a
+// diagonal tile also rescales its circles' box and radius edges.
 template <int CS, bool APPLY>
-__global__ __launch_bounds__(64) void k_cpol_tile(CPolArgs p)
+__global__ __launch_bounds__(256) void k_cpol_tile(CPolArgs p)
 {
     constexpr int CSS = CS * CS, W = CS / 32;
     __shared__ MlpShared w;
-    __shared__ TileScratch ts;
-    __shared__ float yv[64];
+    __shared__ TileScratch ts[4];
+    __shared__ float yv[4][64];
     __shared__ float zc[2 * CS][9];       // Z, Z_prev, Z_prev2 of the tile's circles
     __shared__ unsigned short omap[CSS];  // other list: pair -> entry
-    __shared__ unsigned omask[CS][W], pmask[CS][W];
-    __shared__ unsigned q[CSS];           // the tile's pairs to evaluate: kind << 16 | entry or pair
-    __shared__ unsigned rmask[CS][W];     // APPLY: the pairs in q
+    __shared__ unsigned omask[CS][W], pmask[CS][W], rmask[CS][W];
+    __shared__ unsigned short q[CSS];     // the tile's pairs to evaluate: kind << 14 | entry or pair
+    __shared__ int wsum[4];
     const CArgs& a = p.c;
-    const int lane = threadIdx.x;
+    float* const rhow = const_cast<float*>(a.rho);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int t = blockIdx.x;
     const int N = a.N;
     const long TT = (long)a.T * a.T;
@@ -1059,7 +1094,7 @@ __global__ __launch_bounds__(64) void k_cpol_tile(CPolArgs p)
     const float4* Lo = a.L + to * (2 * CSS);
     const int cp = min(a.cnt[tp], CSS), co = min(a.cnt[to], CSS);
     stage_weights(p.params, w);
-    for (int e = lane; e < 2 * CS; e += 64) {
+    for (int e = tid; e < 2 * CS; e += 256) {
         const int cc = min(e < CS ? P * CS + e : Qb * CS + (e - CS), N - 1);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -1068,30 +1103,25 @@ __global__ __launch_bounds__(64) void k_cpol_tile(CPolArgs p)
             zc[e][6 + k] = a.Zpp[3 * cc + k];
         }
     }
-    for (int e = lane; e < CS * W; e += 64) {
+    for (int e = tid; e < CS * W; e += 256) {
         (&pmask[0][0])[e] = a.Lmask[tp * CS * W + e];
         (&omask[0][0])[e] = a.Lmask[to * CS * W + e];
+        if (APPLY) (&rmask[0][0])[e] = 0u;
     }
-    for (int e = lane; e < co; e += 64) omap[__float_as_int(Lo[2 * e].w) & (CSS - 1)] = (unsigned short)e;
-    if (APPLY)
-        for (int e = lane; e < CS * W; e += 64) (&rmask[0][0])[e] = 0u;
+    for (int e = tid; e < co; e += 256) omap[__float_as_int(Lo[2 * e].w) & (CSS - 1)] = (unsigned short)e;
+    for (int e = tid; e < cp; e += 256) q[e] = (unsigned short)e;
     __syncthreads();
     auto inmask = [&](const unsigned (&m)[CS][W], int idx) { return (m[idx / CS][(idx % CS) >> 5] >> (idx & 31)) & 1u; };
     // the queue: par entries, other-only entries, off-base unlisted pairs
-    for (int e = lane; e < cp; e += 64) q[e] = (unsigned)e;
     int n = cp;
-    for (int e0 = 0; e0 < co; e0 += 64) {
-        const int e = e0 + lane;
-        bool take = false;
-        if (e < co) take = !inmask(pmask, __float_as_int(Lo[2 * e].w) & (CSS - 1));
-        const unsigned long long b = __ballot(take);
-        const int at = n + (int)__popcll(b & ((1ull << lane) - 1ull));
-        if (take && at < CSS) q[at] = (1u << 16) | (unsigned)e;
-        n = min(n + (int)__popcll(b), CSS);
+    for (int e0 = 0; e0 < co; e0 += 256) {
+        const int e = e0 + tid;
+        const bool take = e < co && !inmask(pmask, __float_as_int(Lo[2 * e].w) & (CSS - 1));
+        n = cpol_append(q, n, take, (unsigned short)((1u << 14) | (unsigned)e), wsum);
     }
     if (p.exceptions) {
-        for (int i0 = 0; i0 < CSS; i0 += 64) {
-            const int idx = i0 + lane;
+        for (int i0 = 0; i0 < CSS; i0 += 256) {
+            const int idx = i0 + tid;
             const int il = idx / CS, jl = idx % CS;
             const int ci = P * CS + il, cj = Qb * CS + jl;
             bool take = false;
@@ -1099,17 +1129,13 @@ __global__ __launch_bounds__(64) void k_cpol_tile(CPolArgs p)
                 const long qa = dense_pair_slot(ci, cj, a.nb, a.T);
                 take = a.rho[qa] != p.base[ci] || a.rho[qa + TT] != p.base[cj];
             }
-            const unsigned long long b = __ballot(take);
-            const int at = n + (int)__popcll(b & ((1ull << lane) - 1ull));
-            if (take && at < CSS) q[at] = (2u << 16) | (unsigned)idx;
-            n = min(n + (int)__popcll(b), CSS);
+            n = cpol_append(q, n, take, (unsigned short)((2u << 14) | (unsigned)idx), wsum);
         }
     }
-    __syncthreads();
     if (APPLY) {
-        for (int e = lane; e < n; e += 64) {
+        for (int e = tid; e < n; e += 256) {
             const unsigned code = q[e];
-            const int k2 = (int)(code >> 16), en = (int)(code & 0xffffu);
+            const int k2 = (int)(code >> 14), en = (int)(code & 0x3fffu);
             const int idx = k2 == 2 ? en : __float_as_int((k2 == 0 ? Lp : Lo)[2 * en].w) & (CSS - 1);
             atomicOr(&rmask[idx / CS][(idx % CS) >> 5], 1u << (idx & 31));
         }
@@ -1117,8 +1143,8 @@ __global__ __launch_bounds__(64) void k_cpol_tile(CPolArgs p)
     }
     const float g = APPLY ? p.gl[0] : 0.0f, lg = APPLY ? p.gl[1] : 0.0f;
     long long acc = 0;
-    for (int q0 = 0; q0 < n; q0 += 32) {
-        const int qi = q0 + (lane & 31);
+    for (int q0 = 0; q0 < n; q0 += 128) {
+        const int qi = q0 + 32 * wv + (lane & 31);
         const int side = lane >> 5;  // 0: the pair's first circle (row), 1: its second (column)
         const bool valid = qi < n;
         float f[MLP_NIN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -1127,8 +1153,8 @@ __global__ __launch_bounds__(64) void k_cpol_tile(CPolArgs p)
         float m[3] = {0.0f, 0.0f, 0.0f}, rho = 1.0f;
         if (valid) {
             const unsigned code = q[qi];
-            kind = (int)(code >> 16);
-            ent = (int)(code & 0xffffu);
+            kind = (int)(code >> 14);
+            ent = (int)(code & 0x3fffu);
             idx = kind == 2 ? ent : __float_as_int((kind == 0 ? Lp : Lo)[2 * ent].w) & (CSS - 1);
             const int il = idx / CS, jl = idx % CS;
             zr = side ? CS + jl : il;
@@ -1151,25 +1177,32 @@ __global__ __launch_bounds__(64) void k_cpol_tile(CPolArgs p)
             }
             cpol_features(x, &zc[zr][0], &zc[zr][3], rho, PDM_EDGE_PAIR, p.frac, f);
         }
-        const float y = policy_wave_eval(w, ts, yv, f, lane);
+        const float y = policy_wave_eval(w, ts[wv], yv[wv], f, lane);
         if constexpr (!APPLY) {
             if (valid) {
                 acc += cpol_fix(y);
                 atomicAdd(&p.rows[circ], 1);
             }
         } else {
-            float rn = 0.0f, sf = 0.0f;
+            float rn = 0.0f, sf = 0.0f, rv = 0.0f;
             if (valid) {
-                rn = y - lg > p.raise_log ? exp10f(y) : g;
+                rn = cpol_rho(p, y, g, lg);
+                rv = cpol_rho(p, p.yc[circ], g, lg);  // the circle's new unlisted rho
                 sf = rho / rn;
             }
             // an unlisted pair stays implicit if both edges take their circles' factors
-            const bool own = valid && (rho != p.base[circ] || rn != p.rnew[circ]);
+            const bool own = valid && (rho != p.base[circ] || rn != rv);
             const bool both_own = own || __shfl_xor(own ? 1 : 0, 32, 64);
             float mn[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) mn[k] = zc[zr][k] + sf * (m[k] - zc[zr][k]);
-            if (valid) const_cast<float*>(a.rho)[slot] = rn;  // every evaluated edge, implicit or not
+            if (valid) {
+                rhow[slot] = rn;  // every evaluated edge, implicit or not
+                const long long dw = (long long)rintf((rn - rv) * CPOL_WFIX);
+                const long long dw2 = (long long)rintf((rn * rn - rv * rv) * CPOL_W2FIX);
+                if (dw) atomicAdd((unsigned long long*)&p.wacc[circ], (unsigned long long)dw);
+                if (dw2) atomicAdd((unsigned long long*)&p.wacc[N + circ], (unsigned long long)dw2);
+            }
             if (valid && kind == 0) Lp[2 * ent + side] = make_float4(mn[0], mn[1], mn[2], side ? 0.0f : __int_as_float(idx));
             // the appended entries: lane of side 0 reserves the slot
             const bool app = valid && kind != 0 && both_own;
@@ -1185,83 +1218,83 @@ __global__ __launch_bounds__(64) void k_cpol_tile(CPolArgs p)
         }
     }
     if constexpr (!APPLY) {
+        // a diagonal tile: its circles' unlisted pair edges (wave 0), box
+        // edges (wave 1) and radius edges (wave 2)
+        if (diag && wv < 3) {
+            const int kind = wv;
+            const int c = P * CS + lane;
+            const bool valid = lane < CS && c < N;
+            float f[MLP_NIN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (valid) {
+                float x[3], rho;
+                if (kind == 0) {  // an unlisted pair edge: X_j = Z_{j-1} - (Z_{j-2} - Z_{j-1})
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) x[k] = zc[lane][3 + k] - (zc[lane][6 + k] - zc[lane][3 + k]);
+                    rho = p.base[c];
+                } else {
+                    const long qq = dense_unary_slot(c, a.nb, a.T) + (kind == 2 ? TT : 0);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) x[k] = a.X[k * a.stride + qq];
+                    rho = a.rho[qq];
+                }
+                cpol_features(x, &zc[lane][0], &zc[lane][3], rho,
+                              kind == 0 ? PDM_EDGE_PAIR : kind == 1 ? PDM_EDGE_CONTAINER : PDM_EDGE_RADIUS, p.frac, f);
+            }
+            const float y = policy_wave_eval(w, ts[wv], yv[wv], f, lane);
+            if (valid) p.yc[(long)kind * N + c] = y;
+        }
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
         if (lane == 0 && acc) atomicAdd((unsigned long long*)p.ysum, (unsigned long long)acc);
     } else {
         // every pair not evaluated on its own: its circles' new rho
-        for (int idx = lane; idx < CSS; idx += 64) {
+        for (int idx = tid; idx < CSS; idx += 256) {
             const int il = idx / CS, jl = idx % CS;
             const int ci = P * CS + il, cj = Qb * CS + jl;
             if (ci >= N || cj >= N || (diag && il >= jl) || inmask(rmask, idx)) continue;
             const long qa = dense_pair_slot(ci, cj, a.nb, a.T);
-            const_cast<float*>(a.rho)[qa] = p.rnew[ci];
-            const_cast<float*>(a.rho)[qa + TT] = p.rnew[cj];
+            rhow[qa] = cpol_rho(p, p.yc[ci], g, lg);
+            rhow[qa + TT] = cpol_rho(p, p.yc[cj], g, lg);
         }
-    }
-}
-
-// Per circle (64 per workgroup, one per thread): the network on the circle's
-// unlisted pair edges (weighted by their count), its box edge and its
-// radius edge.  Each wave pushes its 64 lanes' rows through the network
-// three times.
-__global__ __launch_bounds__(64) void k_cpol_circles(CPolArgs p)
-{
-    __shared__ MlpShared w;
-    __shared__ TileScratch ts;
-    __shared__ float yv[64];
-    const CArgs& a = p.c;
-    stage_weights(p.params, w);
-    __syncthreads();
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    const bool valid = c < a.N;
-    const long TT = (long)a.T * a.T;
-    const long q0 = valid ? dense_unary_slot(c, a.nb, a.T) : 0;
-    float z[3] = {0.f, 0.f, 0.f}, zp[3] = {0.f, 0.f, 0.f}, zpp[3] = {0.f, 0.f, 0.f};
-    if (valid)
+        // a diagonal tile: its circles' box and radius edges, their stored U
+        // (one update behind) rescaled so the pending update is s U_j
+        if (diag)
+            for (int e = tid; e < 2 * CS; e += 256) {
+                const int c = P * CS + (e % CS), kind = 1 + e / CS;
+                if (c >= N) continue;
+                const long qq = dense_unary_slot(c, a.nb, a.T) + (kind == 2 ? TT : 0);
+                const float r = cpol_rho(p, p.yc[(long)kind * N + c], g, lg);
+                const float sf = a.rho[qq] / r;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            z[k] = a.Z[3 * c + k];
-            zp[k] = a.Zp[3 * c + k];
-            zpp[k] = a.Zpp[3 * c + k];
-        }
-    long long acc = 0;
-    for (int kind = 0; kind < 3; ++kind) {
-        float f[MLP_NIN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        long long wgt = 0;
-        if (valid) {
-            float x[3], rho;
-            if (kind == 0) {  // an unlisted pair edge: X_j = Z_{j-1} - (Z_{j-2} - Z_{j-1})
-#pragma unroll
-                for (int k = 0; k < 3; ++k) x[k] = zp[k] - (zpp[k] - zp[k]);
-                rho = p.base[c];
-                wgt = (long long)(a.N - 1 - p.rows[c]);
-            } else {
-                const long q = q0 + (kind == 2 ? TT : 0);
-#pragma unroll
-                for (int k = 0; k < 3; ++k) x[k] = a.X[k * a.stride + q];
-                rho = a.rho[q];
-                wgt = 1;
+                for (int k = 0; k < 3; ++k) {
+                    const float x = a.X[k * a.stride + qq], z = a.Z[3 * c + k];
+                    a.U[k * a.stride + qq] = sf * ((x + a.U[k * a.stride + qq]) - z) - x + z;
+                }
+                rhow[qq] = r;
             }
-            cpol_features(x, z, zp, rho, kind == 0 ? PDM_EDGE_PAIR : kind == 1 ? PDM_EDGE_CONTAINER : PDM_EDGE_RADIUS,
-                          p.frac, f);
-        }
-        const float y = policy_wave_eval(w, ts, yv, f, threadIdx.x);
-        if (valid) {
-            p.yc[(long)kind * a.N + c] = y;
-            acc += wgt * cpol_fix(y);
-        }
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (threadIdx.x == 0 && acc) atomicAdd((unsigned long long*)p.ysum, (unsigned long long)acc);
 }
 
-// The graph rho: 10^(mean output over all edges), or with average the mean
-// of this and the earlier estimates (the per-type policy's running sums,
-// every type the same).
-__global__ void k_cpol_final(CPolArgs p, long E)
+// The graph rho: 10^(mean output over all edges) -- the pair edges evaluated
+// on their own (ysum), (N - 1 - rows) times each circle's unlisted pair
+// output, its box and radius outputs -- or with average the mean of this and
+// the earlier estimates (the per-type policy's running sums, every type the
+// same).  One workgroup; integer sums, so the order does not matter.
+__global__ __launch_bounds__(256) void k_cpol_final(CPolArgs p, long E)
 {
+    __shared__ long long sh[256];
+    const int N = p.c.N;
+    long long s = 0;
+    for (int c = threadIdx.x; c < N; c += 256)
+        s += (long long)(N - 1 - p.rows[c]) * cpol_fix(p.yc[c]) + cpol_fix(p.yc[(long)N + c]) +
+             cpol_fix(p.yc[2L * N + c]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
     if (threadIdx.x != 0) return;
-    double m = (double)(*p.ysum) / (double)CPOL_FIX / (double)E;
+    double m = (double)(sh[0] + *p.ysum) / (double)CPOL_FIX / (double)E;
     if (p.average) {
         for (int k = 0; k < 3; ++k) {
             p.hist[2 * k] += m;
@@ -1274,47 +1307,33 @@ __global__ void k_cpol_final(CPolArgs p, long E)
     p.gl[1] = log10f(g);
 }
 
-// Per circle: the unlisted pair edges' new rho and factor; the box and
-// radius edges' new rho with their stored U rescaled.
-__global__ __launch_bounds__(256) void k_cpol_apply_circles(CPolArgs p)
-{
-    const CArgs& a = p.c;
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= a.N) return;
-    const float g = p.gl[0], lg = p.gl[1];
-    const float y0 = p.yc[c];
-    const float rn = y0 - lg > p.raise_log ? exp10f(y0) : g;
-    p.rnew[c] = rn;
-    p.sfac[c] = p.base[c] / rn;
-    const long TT = (long)a.T * a.T, q0 = dense_unary_slot(c, a.nb, a.T);
-    for (int kind = 1; kind < 3; ++kind) {
-        const long q = q0 + (kind == 2 ? TT : 0);
-        const float y = p.yc[(long)kind * a.N + c];
-        const float r = y - lg > p.raise_log ? exp10f(y) : g;
-        const float sf = a.rho[q] / r;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const float x = a.X[k * a.stride + q], z = a.Z[3 * c + k];
-            a.U[k * a.stride + q] = sf * ((x + a.U[k * a.stride + q]) - z) - x + z;
-        }
-        const_cast<float*>(a.rho)[q] = r;
-    }
-}
-
-// Per circle: the virtual Z_{j-1} that carries the unlisted pair edges'
-// rescaled U, and the base rho for the next application.
+// Per circle, last: the virtual Z_{j-1} that carries the unlisted pair
+// edges' rescaled U, the base rho for the next application, and the
+// circle's sums of rho and rho^2 over its N + 1 edges for the per-edge-rho
+// consensus: (N - 1) times the unlisted rho, corrected for the pair edges
+// evaluated on their own, plus the box and radius edges.
 __global__ __launch_bounds__(256) void k_cpol_zprev(CPolArgs p)
 {
     const CArgs& a = p.c;
     const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= a.N) return;
-    const float sf = p.sfac[c];
+    const int N = a.N;
+    if (c >= N) return;
+    const float g = p.gl[0], lg = p.gl[1];
+    const float rv = cpol_rho(p, p.yc[c], g, lg);
+    const float sf = p.base[c] / rv;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float z = a.Z[3 * c + k];
         a.Zp[3 * c + k] = z - sf * (z - a.Zp[3 * c + k]);
     }
-    p.base[c] = p.rnew[c];
+    p.base[c] = rv;
+    const long TT = (long)a.T * a.T, q0 = dense_unary_slot(c, a.nb, a.T);
+    const float rb = a.rho[q0], rr = a.rho[q0 + TT];
+    const double w = (double)(N - 1) * rv + (double)p.wacc[c] / (double)CPOL_WFIX + rb + rr;
+    const double w2 = (double)(N - 1) * ((double)rv * rv) + (double)p.wacc[N + c] / (double)CPOL_W2FIX +
+                      (double)rb * rb + (double)rr * rr;
+    a.W[c] = (float)w;
+    a.W[N + c] = (float)w2;
 }
 
 __global__ void k_fill_f32(float* p, long n, const float* v)
@@ -1738,53 +1757,49 @@ int dense_contact_policy(DevGraph* s, const float* params, float frac, int mode,
 {
     DenseState* d = s->dense;
     const int N = d->N;
-    if (!d->d_cpol) RC(dmalloc(s, &d->d_cpol, 16 + sizeof(int) * (size_t)N + sizeof(float) * 6 * (size_t)N));
+    // [0, 16): output sum, graph rho; then rows [N] (int), the rho sums [2][N]
+    // (int64), the outputs [3][N] (float): the first three start at zero
+    const size_t zwords = 4 + (size_t)N + 4 * (size_t)N;
+    if (!d->d_cpol) RC(dmalloc(s, &d->d_cpol, 4 * zwords + sizeof(float) * 3 * (size_t)N + 8));
     CPolArgs p;
     p.c = cargs(s);
     p.params = params;
     p.frac = frac;
     p.raise_log = mode == 3 ? 0.5f : __builtin_inff();
     p.exceptions = d->c_rhoe ? 1 : 0;
+    p.average = average ? 1 : 0;
     p.base = d->d_base;
     p.ysum = (long long*)d->d_cpol;
     p.gl = (float*)((char*)d->d_cpol + 8);
     p.rows = (int*)((char*)d->d_cpol + 16);
-    p.yc = (float*)(p.rows + N);
-    p.rnew = p.yc + 3 * (size_t)N;
-    p.sfac = p.rnew + N;
+    p.wacc = (long long*)((char*)d->d_cpol + 16 + 8 * (((size_t)N + 1) / 2));
+    p.yc = (float*)((char*)d->d_cpol + 4 * zwords + 8);
     p.hist = hist;
-    p.average = average ? 1 : 0;
-    // the sum and the row counts start at zero (fill kernels: the solve may
-    // be captured by a caller)
-    k_fill_u32<<<(unsigned)((N + 4 + 255) / 256), 256, 0, s->stream>>>((unsigned*)d->d_cpol, N + 4, 0u);
+    // a fill kernel, not a memset: the solve may be captured by a caller
+    prof_begin(s, "cpol_fill");
+    k_fill_u32<<<(unsigned)((zwords + 4 + 255) / 256), 256, 0, s->stream>>>((unsigned*)d->d_cpol, (int)(zwords + 4), 0u);
+    prof_end(s);
     CK(hipGetLastError());
-    const unsigned ctg = (unsigned)((N + 63) / 64), cg = (unsigned)((N + 255) / 256);
+    const unsigned cg = (unsigned)((N + 255) / 256);
     prof_begin(s, "cpol_rows");
-    if (d->cs == 64) k_cpol_tile<64, false><<<(unsigned)d->nts, 64, 0, s->stream>>>(p);
-    else k_cpol_tile<32, false><<<(unsigned)d->nts, 64, 0, s->stream>>>(p);
+    if (d->cs == 64) k_cpol_tile<64, false><<<(unsigned)d->nts, 256, 0, s->stream>>>(p);
+    else k_cpol_tile<32, false><<<(unsigned)d->nts, 256, 0, s->stream>>>(p);
     prof_end(s);
     CK(hipGetLastError());
-    prof_begin(s, "cpol_circles");
-    k_cpol_circles<<<ctg, 64, 0, s->stream>>>(p);
+    prof_begin(s, "cpol_final");
+    k_cpol_final<<<1, 256, 0, s->stream>>>(p, (long)s->E);
     prof_end(s);
-    CK(hipGetLastError());
-    k_cpol_final<<<1, 64, 0, s->stream>>>(p, (long)s->E);
-    CK(hipGetLastError());
-    k_cpol_apply_circles<<<cg, 256, 0, s->stream>>>(p);
     CK(hipGetLastError());
     prof_begin(s, "cpol_apply");
-    if (d->cs == 64) k_cpol_tile<64, true><<<(unsigned)d->nts, 64, 0, s->stream>>>(p);
-    else k_cpol_tile<32, true><<<(unsigned)d->nts, 64, 0, s->stream>>>(p);
+    if (d->cs == 64) k_cpol_tile<64, true><<<(unsigned)d->nts, 256, 0, s->stream>>>(p);
+    else k_cpol_tile<32, true><<<(unsigned)d->nts, 256, 0, s->stream>>>(p);
     prof_end(s);
     CK(hipGetLastError());
+    prof_begin(s, "cpol_zprev");
     k_cpol_zprev<<<cg, 256, 0, s->stream>>>(p);
-    CK(hipGetLastError());
-    // rho per edge from here: the circles' sums of rho and rho^2
-    prof_begin(s, "contact_weights");
-    k_contact_weights<<<(unsigned)N, 64, 0, s->stream>>>(cargs(s));
     prof_end(s);
     CK(hipGetLastError());
-    d->c_rhoe = true;
+    d->c_rhoe = true;  // rho per edge from here (the circles' sums are set)
     return PARADMM_OK;
 }
 

@@ -1645,6 +1645,14 @@ extern "C" int paradmm_gpu_profile_kernels(graph* G, int iters, int max_kernels,
     int rc = PARADMM_OK;
     // the kernel sequence of a captured run of iters iterations
     for (int i = 0; i < iters && !rc; ++i) rc = fused_iteration(s, 0, mphase_of(s, i, iters));
+    // with a rho policy attached: one application as a solve makes it
+    // (iterations follow it), then one more iteration
+    if (!rc && s->rho_policy && iters > 0) {
+        s->policy_in_solve = true;
+        rc = s->rho_policy(s, 0.5f);
+        s->policy_in_solve = false;
+        if (!rc) rc = fused_iteration(s, 0, DENSE_PLAIN);
+    }
     s->prof = false;
     RC(rc);
     CK(hipStreamSynchronize(s->stream));

@@ -138,7 +138,9 @@ int paradmm_gpu_time_iterations(graph *G, int iters, double *ms);
 /* Runs iters iterations launched one kernel at a time with HIP events around
  * every kernel on the engine's stream; per distinct kernel name returns the
  * mean duration (avg_ms[j], ms) and launch count.  names receives up to
- * max_kernels NUL-terminated names of 64 bytes each. */
+ * max_kernels NUL-terminated names of 64 bytes each.  With a rho policy
+ * attached (paradmm_gpu_set_rho_policy) the sequence also holds one
+ * application of it, as a solve makes it, and one more iteration. */
 int paradmm_gpu_profile_kernels(graph *G, int iters, int max_kernels, char *names, double *avg_ms,
                                 int *count, int *nkernels);
 /* ---- solving to tolerance (NEW: the reference runs a fixed iteration count,
