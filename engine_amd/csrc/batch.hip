@@ -43,6 +43,7 @@
 #include "gridbar.h"
 #include "mlp_device.h"
 #include "pairtile.h"
+#include "contact_device.h"
 #include "paradmm_gpu.h"
 
 #define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hip_fail(e_, #x, __FILE__, __LINE__); } while (0)
@@ -107,7 +108,8 @@ enum { FIRST_NBUF = 0, FIRST_CURRENT = 1, FIRST_FUSED = 2 };
 
 // Grid-wide barrier between phases; false when the two-level barrier gave up
 // (every workgroup then leaves the kernel).
-__device__ __forceinline__ bool phase_barrier(const BArgs& a, cg::grid_group& grid, unsigned& gen)
+template <class A>
+__device__ __forceinline__ bool phase_barrier(const A& a, cg::grid_group& grid, unsigned& gen)
 {
     if (a.barrier == 0) {
         grid.sync();
@@ -116,13 +118,16 @@ __device__ __forceinline__ bool phase_barrier(const BArgs& a, cg::grid_group& gr
     return grid_barrier(a.bar, gen, a.spin_limit);
 }
 
-struct BLists {
+// the active instances (at most CAP) with prefix sums, in LDS
+template <int CAP>
+struct BListsT {
     int n;
-    int inst[BATCH_MAX];
-    int tpre[BATCH_MAX + 1];
-    int cpre[BATCH_MAX + 1];
+    int inst[CAP];
+    int tpre[CAP + 1];
+    int cpre[CAP + 1];
     int scan[3][256];
 };
+using BLists = BListsT<BATCH_MAX>;
 
 __device__ __forceinline__ DArgs inst_view(const BArgs& a, const BInst& in, const int i)
 {
@@ -147,19 +152,21 @@ __device__ __forceinline__ DArgs inst_view(const BArgs& a, const BInst& in, cons
     return v;
 }
 
-// Active instances in the launch's order with prefix sums of their tile and
-// circle counts; every workgroup builds the same lists from st[].active.
-__device__ void rebuild_lists(const BArgs& a, BLists& L)
+// Active instances in the launch's order with prefix sums of their work
+// items (items(i): tiles, or contact blocks and tiles) and circle counts;
+// every workgroup builds the same lists from st[].active.
+template <class Items, class Lists>
+__device__ void rebuild_lists_of(int ninst, const int* order, BState* st, const BInst* inst, Items items, Lists& L)
 {
     const int tid = threadIdx.x;
-    const int per = (a.ninst + 255) / 256;
-    const int lo = min(tid * per, a.ninst), hi = min(lo + per, a.ninst);
+    const int per = (ninst + 255) / 256;
+    const int lo = min(tid * per, ninst), hi = min(lo + per, ninst);
     int na = 0, nt = 0, nc = 0;
     for (int j = lo; j < hi; ++j)
-        if (const int i = a.order[j]; __hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        if (const int i = order[j]; __hip_atomic_load(&st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             ++na;
-            nt += a.inst[i].nt;
-            nc += a.inst[i].N;
+            nt += items(i);
+            nc += inst[i].N;
         }
     L.scan[0][tid] = na;
     L.scan[1][tid] = nt;
@@ -176,13 +183,13 @@ __device__ void rebuild_lists(const BArgs& a, BLists& L)
     __syncthreads();
     int k = L.scan[0][tid], tp = L.scan[1][tid], cp = L.scan[2][tid];
     for (int j = lo; j < hi; ++j)
-        if (const int i = a.order[j]; __hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        if (const int i = order[j]; __hip_atomic_load(&st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             L.inst[k] = i;
             L.tpre[k] = tp;
             L.cpre[k] = cp;
             ++k;
-            tp += a.inst[i].nt;
-            cp += a.inst[i].N;
+            tp += items(i);
+            cp += inst[i].N;
         }
     if (tid == 255) {
         L.n = k;
@@ -190,6 +197,12 @@ __device__ void rebuild_lists(const BArgs& a, BLists& L)
         L.cpre[k] = cp;
     }
     __syncthreads();
+}
+
+__device__ void rebuild_lists(const BArgs& a, BLists& L)
+{
+    const BInst* inst = a.inst;
+    rebuild_lists_of(a.ninst, a.order, a.st, inst, [inst](int i) { return inst[i].nt; }, L);
 }
 
 // position of w in a prefix array: pre[k] <= w < pre[k+1]
@@ -437,6 +450,320 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
+// ---- the contact state on the batch
+//
+// With alpha = 1 and one rho per instance (the plain fixed-rho batch) every
+// instance runs the contact state of the single-instance dense layout
+// (contact_device.h): per contact tile of CS x CS circles a list of the
+// pairs whose prox is active; every other pair edge carries no state (its M
+// is its circle's Z).  The launch sequence of a solve:
+//   1. one k_batch_solve iteration (the after-upload pass, per edge);
+//   2. k_batch_contact_enter: every instance's lists from its X, U;
+//   3. k_batch_contact, one cooperative launch over the remaining
+//      iterations: per iteration one phase over the work items of every
+//      active instance (its CS-circle blocks' box and radius edges, then its
+//      contact tiles), one grid barrier; Z_{k+1} = Z_k + sum / (N + 1) from
+//      the integer circle sums is applied by the next phase as it reads the
+//      circles, as between the single-instance launches, so a run of
+//      iterations needs one barrier each.  At a check (every check_every
+//      iterations) the run ends: the per-circle consensus and residuals into
+//      the canonical Z arrays, the quality of that Z per contact tile, one
+//      wavefront per instance reduces and retires;
+//   4. k_batch_contact_leave: every instance's X and U from its lists, at
+//      the parity of its own last iteration.
+// The circle sums are integers (2^-31 fixed point), so an instance's
+// iterations do not depend on the tiling or on what else shares the launch:
+// they equal the single-instance contact iterations bit for bit wherever the
+// after-upload pass does.
+
+struct BCInst {
+    int nbs, nts;  // contact blocks of CS circles, contact tiles
+    long l0;       // the instance's first contact tile
+};
+
+struct BCArgs {
+    int ninst, max_iters, check_every;
+    int it0, it_end;  // iterations [it0, it_end) of this launch (Z of it0 canonical)
+    int par0;         // the list the last iteration wrote, at it0
+    double tol_p, tol_d, tol_f;
+    int T;            // dense tile side (the box and radius slots)
+    long stride;
+    float *X, *U, *Z, *Zprev, *Zpp, *Zb;
+    const float *rho, *aexp, *inst_rho;
+    const BInst* inst;
+    const BCInst* cinst;
+    float4* L;                // per instance [2][nts][CS*CS][2]
+    unsigned* Lmask;          // per instance [2][nts][CS][CS/32]
+    int* cnt;                 // per instance [2][nts]
+    unsigned long long* Ac;   // per instance [3][CQ][N]
+    double *Rpc, *Rd, *cq;    // per circle primal and dual residual; per contact tile max overlap
+    unsigned* flags;          // per instance: a circle sum could not take a value
+    int* cpar;                // per instance: the list its last iteration wrote
+    BState* st;
+    long long* t_start;
+    unsigned* bar;
+    const int* order;
+    unsigned spin_limit;
+    int barrier;
+};
+
+// instance i's contact state as the single-instance kernels see it
+template <int CS>
+__device__ __forceinline__ CArgs cview(const BCArgs& a, const BInst& in, const BCInst& ci, const int i, const int par)
+{
+    constexpr long CSS = (long)CS * CS;
+    CArgs v;
+    v.N = in.N;
+    v.nbs = ci.nbs;
+    v.nts = ci.nts;
+    v.nb = in.nb;
+    v.T = a.T;
+    v.stride = a.stride;
+    v.X = a.X + in.s0;
+    v.U = a.U + in.s0;
+    v.Z = a.Z + 3L * in.c0;
+    v.Zp = a.Zprev + 3L * in.c0;
+    v.Zpp = a.Zpp + 3L * in.c0;
+    v.aexp = a.aexp + in.c0;
+    v.uni = a.inst_rho + 2 * i;
+    v.rho = a.rho + in.s0;
+    v.W = nullptr;
+    v.par = par;
+    v.L = a.L + 2 * ci.l0 * (2 * CSS);
+    v.Lmask = a.Lmask + 2 * ci.l0 * (CS * (CS / 32));
+    v.cnt = a.cnt + 2 * ci.l0;
+    v.Ac = v.Acl = nullptr;
+    v.Ap = nullptr;
+    v.Zin = v.Zpin = nullptr;
+    v.Zo = nullptr;
+    v.Rpc = a.Rpc + in.c0;
+    v.Rd = a.Rd + in.c0;
+    v.flag = a.flags + i;
+    return v;
+}
+
+// the sum buffers and Z ring of launch j of a run (as contact_iteration)
+__device__ __forceinline__ void cview_run(CArgs& v, const BCArgs& a, const BInst& in, const int j)
+{
+    const long AQ = (long)CQ * in.N, ZN = 3L * in.N;
+    unsigned long long* A = a.Ac + 3L * CQ * in.c0;
+    float* Zb = a.Zb + 6L * in.c0;
+    v.Ac = A + (j % 3) * AQ;
+    v.Ap = j ? A + ((j - 1) % 3) * AQ : nullptr;
+    v.Acl = A + ((j + 1) % 3) * AQ;
+    v.Zin = j == 0 ? nullptr : j == 1 ? v.Z : Zb + ((j - 1) % 2) * ZN;
+    v.Zo = j ? Zb + (j % 2) * ZN : nullptr;
+}
+
+// the end of a run whose last launch was j (k_contact_zreduce's arguments)
+__device__ __forceinline__ void cview_end(CArgs& f, const BCArgs& a, const BInst& in, const int j)
+{
+    const long AQ = (long)CQ * in.N, ZN = 3L * in.N;
+    unsigned long long* A = a.Ac + 3L * CQ * in.c0;
+    float* Zb = a.Zb + 6L * in.c0;
+    f.Ac = A + (j % 3) * AQ;
+    f.Acl = A + ((j + 2) % 3) * AQ;
+    f.Zin = j == 0 ? f.Z : Zb + (j % 2) * ZN;
+    f.Zpin = j == 0 ? f.Zp : j == 1 ? f.Z : Zb + ((j - 1) % 2) * ZN;
+}
+
+// one iteration's work items, one wavefront each: an instance's nbs blocks
+// (box and radius edges) then its nts contact tiles
+template <int CS, bool RES, class Lists>
+__device__ __forceinline__ void bc_phase_items(const BCArgs& a, const Lists& L, ContactWave<CS, RES ? CQ : 3>& cw,
+                                               const int par, const int j)
+{
+    const int nitems = L.tpre[L.n];
+    const int nw = gridDim.x * 4;
+    // the wavefront's index is uniform: the item's instance and pointers live in scalar registers
+    for (int w = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); w < nitems; w += nw) {
+        const int k = find_prefix(L.tpre, L.n, w);
+        const int i = L.inst[k];
+        const BInst in = a.inst[i];
+        const BCInst ci = a.cinst[i];
+        const int li = w - L.tpre[k];
+        CArgs v = cview<CS>(a, in, ci, i, par);
+        cview_run(v, a, in, j);
+        if (li < ci.nbs) contact_unary<CS, RES, false>(v, li);
+        else contact_tile_body<CS, RES, false>(v, li - ci.nbs, cw);
+    }
+}
+
+// the end of a run: every active circle's consensus into the canonical Z
+// arrays (RES: with its residuals), one thread per circle
+template <int CS, bool RES, class Lists>
+__device__ void bc_phase_zreduce(const BCArgs& a, const Lists& L, const int j)
+{
+    const int nc = L.cpre[L.n];
+    for (int w = blockIdx.x * 256 + threadIdx.x; w < nc; w += gridDim.x * 256) {
+        const int k = find_prefix(L.cpre, L.n, w);
+        const int i = L.inst[k];
+        const BInst in = a.inst[i];
+        CArgs f = cview<CS>(a, in, a.cinst[i], i, 0);
+        cview_end(f, a, in, j);
+        contact_zreduce_circle<RES, false>(f, w - L.cpre[k]);
+    }
+}
+
+// the largest pair overlap r_i + r_j - |c_i - c_j| of Z over each contact
+// tile's pairs (FP64, as the per-edge batch's quality partials), one
+// wavefront per tile
+template <int CS, class Lists>
+__device__ void bc_phase_quality(const BCArgs& a, const Lists& L)
+{
+    const int lane = threadIdx.x & 63;
+    const int nitems = L.tpre[L.n];
+    const int nw = gridDim.x * 4;
+    // the wavefront's index is uniform: the item's instance and pointers live in scalar registers
+    for (int w = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); w < nitems; w += nw) {
+        const int k = find_prefix(L.tpre, L.n, w);
+        const int i = L.inst[k];
+        const BCInst ci = a.cinst[i];
+        const int li = w - L.tpre[k];
+        if (li < ci.nbs) continue;
+        const BInst in = a.inst[i];
+        const int t = li - ci.nbs;
+        int P, Qb;
+        tri_tile(t, ci.nbs, P, Qb);
+        const float* Z = a.Z + 3L * in.c0;
+        double mo = 0.0;
+        for (int idx = lane; idx < CS * CS; idx += 64) {
+            const int il = idx / CS, jl = idx % CS;
+            const int ii = P * CS + il, jj = Qb * CS + jl;
+            if (ii >= in.N || jj >= in.N || (P == Qb && il >= jl)) continue;
+            const double dx = (double)Z[3 * ii] - (double)Z[3 * jj], dy = (double)Z[3 * ii + 1] - (double)Z[3 * jj + 1];
+            const double ov = (double)Z[3 * ii + 2] + (double)Z[3 * jj + 2] - sqrt(dx * dx + dy * dy);
+            mo = ov > mo ? ov : mo;
+        }
+        mo = wmax64d(mo);
+        if (lane == 0) a.cq[ci.l0 + t] = mo;
+    }
+}
+
+// one wavefront per active instance: the residuals and quality of the
+// run's last iteration (`done` iterations in all), retirement as phase_check
+// one wavefront: instance i's check
+__device__ __forceinline__ void bc_check_instance(const BCArgs& a, const int i, const int done, const bool final,
+                                                  const int par)
+{
+    const int lane = threadIdx.x & 63;
+    {
+        const BInst in = a.inst[i];
+        const BCInst ci = a.cinst[i];
+        double p = 0.0, d = 0.0, ov = 0.0, bx = 0.0, sr = 0.0;
+        for (int t = lane; t < ci.nts; t += 64) ov = fmax(ov, a.cq[ci.l0 + t]);
+        for (int c = lane; c < in.N; c += 64) {
+            const long g = in.c0 + c;
+            p += a.Rpc[g];
+            d += a.Rd[g];
+            const double cx = a.Z[3 * g], cy = a.Z[3 * g + 1], r = a.Z[3 * g + 2];
+            const double v[5] = {r - cx, cx + r - 1.0, r - cy, cy + r - 1.0, -r};
+#pragma unroll
+            for (int q = 0; q < 5; ++q) bx = v[q] > bx ? v[q] : bx;
+            sr += r;
+        }
+        p = wsum64d(p);
+        d = wsum64d(d);
+        sr = wsum64d(sr);
+        ov = wmax64d(ov);
+        bx = wmax64d(bx);
+        if (lane == 0) {
+            const double pr = sqrt(p / (3.0 * in.E)), du = sqrt(d / (3.0 * in.E));
+            bool conv = pr <= a.tol_p && du <= a.tol_d;
+            if (a.tol_f >= 0.0) conv = conv && ov <= a.tol_f && bx <= a.tol_f;
+            // a circle sum that could not take a value (a non-finite state or
+            // an overflow) or a non-finite residual: retire now, not converged
+            const bool lost = a.flags[i] != 0u || !isfinite(pr) || !isfinite(du) || !isfinite(sr);
+            BState& s = a.st[i];
+            s.iterations = done;
+            s.primal_rms = lost ? __builtin_nan("") : pr;
+            s.dual_rms = du;
+            s.max_overlap = ov;
+            s.max_box = bx;
+            s.objective = sr;
+            if (conv || final || lost) {
+                s.converged = conv && !lost ? 1 : 0;
+                s.t_end = wall_clock64();
+                a.cpar[i] = par;
+                s.active = 0;
+            }
+        }
+    }
+}
+
+template <class Lists>
+__device__ void bc_phase_check(const BCArgs& a, const Lists& L, const int done, const bool final, const int par)
+{
+    const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int k = wave; k < L.n; k += gridDim.x * 4) bc_check_instance(a, L.inst[k], done, final, par);
+}
+
+// CAP: the most instances the launch lists (256 or BATCH_MAX; fewer lists
+// leave LDS for more resident workgroups)
+template <int CS, int CAP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CS == 32 ? 4 : 2))) void k_batch_contact(BCArgs a)
+{
+    cg::grid_group grid = cg::this_grid();
+    __shared__ BListsT<CAP> L;
+    // one contact scratch per wavefront (the iterations without residual
+    // sums use the smaller layout in the same bytes)
+    __shared__ __align__(16) unsigned char cw[4][sizeof(ContactWave<CS, CQ>)];
+    unsigned gen = 0;
+    const BInst* inst = a.inst;
+    const BCInst* cinst = a.cinst;
+    auto items = [cinst](int i) { return cinst[i].nbs + cinst[i].nts; };
+    rebuild_lists_of(a.ninst, a.order, a.st, inst, items, L);
+    unsigned char* mine = cw[threadIdx.x >> 6];
+    int par = a.par0, j = 0;
+    for (int it = a.it0; it < a.it_end && L.n > 0; ++it) {
+        const int done = it + 1;
+        const bool final = done >= a.max_iters;
+        const bool check = final || done % a.check_every == 0;
+        const bool end = check || done >= a.it_end;
+        if (check) bc_phase_items<CS, true>(a, L, *reinterpret_cast<ContactWave<CS, CQ>*>(mine), par, j);
+        else bc_phase_items<CS, false>(a, L, *reinterpret_cast<ContactWave<CS, 3>*>(mine), par, j);
+        if (!phase_barrier(a, grid, gen)) return;
+        par ^= 1;
+        if (!end) {
+            ++j;
+            continue;
+        }
+        if (check) bc_phase_zreduce<CS, true>(a, L, j);
+        else bc_phase_zreduce<CS, false>(a, L, j);
+        if (!phase_barrier(a, grid, gen)) return;
+        j = 0;
+        if (!check) continue;
+        bc_phase_quality<CS>(a, L);
+        if (!phase_barrier(a, grid, gen)) return;
+        bc_phase_check(a, L, done, final, par);
+        if (!phase_barrier(a, grid, gen)) return;
+        rebuild_lists_of(a.ninst, a.order, a.st, inst, items, L);
+    }
+    // instances still active at the launch's end (it_end before their last
+    // check) record the parity here
+    if (blockIdx.x == 0)
+        for (int k = threadIdx.x; k < L.n; k += 256) a.cpar[L.inst[k]] = par;
+}
+
+// one workgroup per contact tile of every instance (ctile: instance, tile)
+template <int CS>
+__global__ __launch_bounds__(256) void k_batch_contact_enter(BCArgs a, const int2* ctile)
+{
+    __shared__ ContactEnterShared<CS> sh;
+    const int2 e = ctile[blockIdx.x];
+    const CArgs v = cview<CS>(a, a.inst[e.x], a.cinst[e.x], e.x, 0);
+    contact_enter_tile<CS>(v, e.y, sh);
+}
+
+template <int CS>
+__global__ __launch_bounds__(256) void k_batch_contact_leave(BCArgs a, const int2* ctile)
+{
+    __shared__ ContactLeaveShared<CS> sh;
+    const int2 e = ctile[blockIdx.x];
+    const CArgs v = cview<CS>(a, a.inst[e.x], a.cinst[e.x], e.x, a.cpar[e.x]);
+    contact_leave_tile<CS>(v, e.y, sh);
+}
+
 // ---- learned rho on the batch: one rho per instance, 10^(mean of the
 // network's log10 rho over the instance's edges), between launches
 
@@ -465,6 +792,98 @@ __device__ __forceinline__ PolicySlot policy_slot(const int q, const int I, cons
         s.type = side ? PDM_EDGE_RADIUS : PDM_EDGE_CONTAINER;
     }
     return s;
+}
+
+// ---- the contact batch with one launch per iteration: the same work items
+// and phases as k_batch_contact, each phase a kernel of its own (the
+// launch boundary orders them), so the hardware keeps as many item
+// wavefronts resident as fit instead of a fixed persistent grid.  Items of
+// retired instances return at once.
+
+// items: (instance, local item), every instance's blocks then its tiles
+template <int CS, bool RES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_contact_step(
+    BCArgs a, const int2* items, int par, int j)
+{
+    __shared__ ContactWave<CS, RES ? CQ : 3> sh;
+    const int2 e = items[blockIdx.x];
+    if (!__hip_atomic_load(&a.st[e.x].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const BInst in = a.inst[e.x];
+    const BCInst ci = a.cinst[e.x];
+    CArgs v = cview<CS>(a, in, ci, e.x, par);
+    cview_run(v, a, in, j);
+    if (e.y < ci.nbs) contact_unary<CS, RES, false>(v, e.y);
+    else contact_tile_body<CS, RES, false>(v, e.y - ci.nbs, sh);
+}
+
+// the end of a run: every circle of an active instance (circ_inst: its instance)
+template <int CS, bool RES>
+__global__ __launch_bounds__(256) void k_batch_contact_zreduce(BCArgs a, const int* circ_inst, int ncirc, int j)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= ncirc) return;
+    const int i = circ_inst[g];
+    if (!__hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const BInst in = a.inst[i];
+    CArgs f = cview<CS>(a, in, a.cinst[i], i, 0);
+    cview_end(f, a, in, j);
+    contact_zreduce_circle<RES, false>(f, g - in.c0);
+}
+
+// quality partials: one wavefront per contact tile (ctile: instance, tile)
+template <int CS>
+__global__ __launch_bounds__(64) void k_batch_contact_quality(BCArgs a, const int2* ctile)
+{
+    const int2 e = ctile[blockIdx.x];
+    if (!__hip_atomic_load(&a.st[e.x].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const int lane = threadIdx.x;
+    const BInst in = a.inst[e.x];
+    const BCInst ci = a.cinst[e.x];
+    int P, Qb;
+    tri_tile(e.y, ci.nbs, P, Qb);
+    const float* Z = a.Z + 3L * in.c0;
+    double mo = 0.0;
+    for (int idx = lane; idx < CS * CS; idx += 64) {
+        const int il = idx / CS, jl = idx % CS;
+        const int ii = P * CS + il, jj = Qb * CS + jl;
+        if (ii >= in.N || jj >= in.N || (P == Qb && il >= jl)) continue;
+        const double dx = (double)Z[3 * ii] - (double)Z[3 * jj], dy = (double)Z[3 * ii + 1] - (double)Z[3 * jj + 1];
+        const double ov = (double)Z[3 * ii + 2] + (double)Z[3 * jj + 2] - sqrt(dx * dx + dy * dy);
+        mo = ov > mo ? ov : mo;
+    }
+    mo = wmax64d(mo);
+    if (lane == 0) a.cq[ci.l0 + e.y] = mo;
+}
+
+// the check: one wavefront per active instance
+__global__ __launch_bounds__(64) void k_batch_contact_check(BCArgs a, int done, int final, int par)
+{
+    const int i = blockIdx.x;
+    if (!__hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    bc_check_instance(a, i, done, final != 0, par);
+}
+
+// The pending U update of every edge of every instance, U = (X + U) - Z
+// (alpha = 1, u_update's rounding): after the contact state's exit the
+// edges hold X_j and U_{j-1}, as after every steady iteration, and a solve
+// hands back U_j, as the per-edge batch's last pass and the single-instance
+// host sync do.  One workgroup per dense tile.
+template <int T>
+__global__ __launch_bounds__(256) void k_batch_finish_u(const int4* tiles, const BInst* inst, long stride,
+                                                        const float* X, float* U, const float* Z)
+{
+    constexpr int NS = 2 * T * T;
+    const int4 td = tiles[blockIdx.x];
+    const BInst in = inst[td.x];
+    const long base = in.s0 + (long)td.w * NS;
+    for (int q = threadIdx.x; q < NS; q += 256) {
+        const PolicySlot ps = policy_slot<T>(q, td.y, td.z, in.N);
+        if (!ps.valid) continue;
+        const long p = base + q;
+        const float* z = Z + 3L * (in.c0 + ps.c);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) U[k * stride + p] = u_update(U[k * stride + p], X[k * stride + p], z[k], 1.0f, 0.0f);
+    }
 }
 
 struct BPolicyArgs {
@@ -663,10 +1082,41 @@ struct paradmm_batch {
     double* phist = nullptr;    // [ninst][2] (average)
     std::vector<int> uni0;      // per instance: one rho and alpha on every edge at upload
     int alpha1 = 0;             // alpha = 1 on every edge of every instance
+    // the contact state (alpha = 1, one rho per instance, no policy)
+    int ccs = 32;               // contact tile side
+    int cbpc = -1, cgrid = 0;   // its persistent grid (0: a launch per iteration, the default; -1: not configured)
+    int cbarrier = 1;
+    bool contact_off = false;   // paradmm_batch_configure_contact(b, 0, ...)
+    int ccs_alloc = 0;          // side the contact buffers are laid out for
+    long cts = 0;               // contact tiles of all instances
+    std::vector<BCInst> cinst;
+    std::vector<void*> callocs;
+    BCInst* cinst_d = nullptr;
+    int2* ctile_d = nullptr;    // [cts] (instance, tile)
+    int2* citems_d = nullptr;   // every instance's blocks, then every instance's tiles (one launch per iteration)
+    int nitems = 0;
+    int* circ_inst_d = nullptr; // [ncirc] instance of each circle
+    float4* cL = nullptr;
+    unsigned* cLmask = nullptr;
+    int* ccnt = nullptr;
+    unsigned long long* cAc = nullptr;
+    float *cZb = nullptr, *cZpp = nullptr;
+    double *cRpc = nullptr, *ccq = nullptr;
+    unsigned* cflags = nullptr;
+    int* cpar = nullptr;
+    float last_ms[4] = {0, 0, 0, 0};  // last contact solve: after-upload pass, entry, iterations, exit
 };
+
+static void batch_contact_free(paradmm_batch* b)
+{
+    for (void* p : b->callocs) (void)hipFree(p);
+    b->callocs.clear();
+    b->ccs_alloc = 0;
+}
 
 static void batch_free_dev(paradmm_batch* b)
 {
+    batch_contact_free(b);
     for (void* p : b->allocs) (void)hipFree(p);
     b->allocs.clear();
     b->yout = nullptr;  // allocated on first use
@@ -917,6 +1367,264 @@ static int batch_launch(paradmm_batch* b, const paradmm_solve_opts* o, float* ms
     return PARADMM_OK;
 }
 
+// ---- the contact state on the batch: host side
+
+// the contact kernel for contact tile side CS and a batch of ninst instances
+static const void* contact_kernel(int CS, int ninst)
+{
+    if (ninst <= 256) return CS == 64 ? (const void*)k_batch_contact<64, 256> : (const void*)k_batch_contact<32, 256>;
+    return CS == 64 ? (const void*)k_batch_contact<64, BATCH_MAX> : (const void*)k_batch_contact<32, BATCH_MAX>;
+}
+
+static int contact_occupancy(const paradmm_batch* b, int CS, int* per_cu)
+{
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, contact_kernel(CS, b->ninst), 256, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor", __FILE__, __LINE__);
+    if (*per_cu < 1) {
+        pdm_set_error(PARADMM_ERR_HIP, "batch contact kernel does not fit a compute unit");
+        return PARADMM_ERR_HIP;
+    }
+    return PARADMM_OK;
+}
+
+// The plain fixed-rho batch runs in the contact state: alpha = 1 and one rho
+// on every edge of each instance (as uploaded), no policy, at least two
+// iterations; PARADMM_CONTACT=0 or paradmm_batch_configure_contact(b, 0, ..)
+// keep the per-edge kernel.
+static bool batch_contact_applies(const paradmm_batch* b, const paradmm_solve_opts* o)
+{
+    if (b->contact_off || o->policy_every > 0 || !b->alpha1 || o->max_iters < 2) return false;
+    const char* e = getenv("PARADMM_CONTACT");
+    if (e && e[0] == '0') return false;
+    for (int i = 0; i < b->ninst; ++i)
+        if (!b->uni0[i]) return false;
+    return true;
+}
+
+static int cmalloc(paradmm_batch* b, void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+    if (e != hipSuccess) {
+        pdm_set_error(PARADMM_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        *p = nullptr;
+        return PARADMM_ERR_NOMEM;
+    }
+    b->callocs.push_back(*p);
+    return PARADMM_OK;
+}
+
+// Contact tiling of every instance for side CS and its buffers (the lists
+// hold every pair of a tile at worst: 32 B per pair and parity).
+static int batch_contact_alloc(paradmm_batch* b, int CS)
+{
+    if (b->ccs_alloc == CS) return PARADMM_OK;
+    batch_contact_free(b);
+    b->cinst.assign(b->ninst, BCInst{});
+    std::vector<int2> ct;
+    long l0 = 0;
+    for (int i = 0; i < b->ninst; ++i) {
+        const int nbs = (b->inst[i].N + CS - 1) / CS, nts = nbs * (nbs + 1) / 2;
+        b->cinst[i] = BCInst{nbs, nts, l0};
+        for (int t = 0; t < nts; ++t) ct.push_back(make_int2(i, t));
+        l0 += nts;
+    }
+    b->cts = l0;
+    const size_t css = (size_t)CS * CS, words = (size_t)CS * (CS / 32), nc = (size_t)b->ncirc;
+    RC(cmalloc(b, (void**)&b->cL, sizeof(float4) * 2 * 2 * (size_t)l0 * css));
+    RC(cmalloc(b, (void**)&b->cLmask, sizeof(unsigned) * 2 * (size_t)l0 * words));
+    RC(cmalloc(b, (void**)&b->ccnt, sizeof(int) * 2 * (size_t)l0));
+    RC(cmalloc(b, (void**)&b->cAc, sizeof(unsigned long long) * 3 * CQ * nc));
+    RC(cmalloc(b, (void**)&b->cZb, sizeof(float) * 6 * nc));
+    RC(cmalloc(b, (void**)&b->cZpp, sizeof(float) * 3 * nc));
+    RC(cmalloc(b, (void**)&b->cRpc, sizeof(double) * nc));
+    RC(cmalloc(b, (void**)&b->ccq, sizeof(double) * (size_t)l0));
+    RC(cmalloc(b, (void**)&b->cflags, sizeof(unsigned) * (size_t)b->ninst));
+    RC(cmalloc(b, (void**)&b->cpar, sizeof(int) * (size_t)b->ninst));
+    RC(cmalloc(b, (void**)&b->cinst_d, sizeof(BCInst) * (size_t)b->ninst));
+    RC(cmalloc(b, (void**)&b->ctile_d, sizeof(int2) * ct.size()));
+    CK(hipMemcpy(b->cinst_d, b->cinst.data(), sizeof(BCInst) * (size_t)b->ninst, hipMemcpyHostToDevice));
+    CK(hipMemcpy(b->ctile_d, ct.data(), sizeof(int2) * ct.size(), hipMemcpyHostToDevice));
+    // work items of the launch-per-iteration form: the blocks' box and
+    // radius edges of every instance first, then the tiles, larger
+    // instances first
+    std::vector<int> ord(b->ninst);
+    for (int i = 0; i < b->ninst; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return b->cinst[x].nts > b->cinst[y].nts; });
+    std::vector<int2> items;
+    for (int i : ord)
+        for (int k = 0; k < b->cinst[i].nbs; ++k) items.push_back(make_int2(i, k));
+    for (int i : ord)
+        for (int t = 0; t < b->cinst[i].nts; ++t) items.push_back(make_int2(i, b->cinst[i].nbs + t));
+    b->nitems = (int)items.size();
+    std::vector<int> ci(nc);
+    for (int i = 0; i < b->ninst; ++i)
+        for (int c = 0; c < b->inst[i].N; ++c) ci[(size_t)b->inst[i].c0 + c] = i;
+    RC(cmalloc(b, (void**)&b->citems_d, sizeof(int2) * items.size()));
+    RC(cmalloc(b, (void**)&b->circ_inst_d, sizeof(int) * nc));
+    CK(hipMemcpy(b->citems_d, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
+    CK(hipMemcpy(b->circ_inst_d, ci.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+    CK(hipMemset(b->ccq, 0, sizeof(double) * (size_t)l0));
+    CK(hipMemset(b->cRpc, 0, sizeof(double) * nc));
+    b->ccs_alloc = CS;
+    return PARADMM_OK;
+}
+
+static BCArgs batch_contact_args(paradmm_batch* b, const paradmm_solve_opts* o, int it0, int it_end, int par0)
+{
+    BCArgs a;
+    a.ninst = b->ninst;
+    a.max_iters = o->max_iters;
+    a.check_every = o->check_every;
+    a.it0 = it0;
+    a.it_end = it_end;
+    a.par0 = par0;
+    a.tol_p = o->tol_primal;
+    a.tol_d = o->tol_dual;
+    a.tol_f = o->tol_feas;
+    a.T = b->T;
+    a.stride = b->slots;
+    a.X = b->X;
+    a.U = b->U;
+    a.Z = b->Z;
+    a.Zprev = b->Zprev;
+    a.Zpp = b->cZpp;
+    a.Zb = b->cZb;
+    a.rho = b->rho;
+    a.aexp = b->aexp_d;
+    a.inst_rho = b->inst_rho;
+    a.inst = b->inst_d;
+    a.cinst = b->cinst_d;
+    a.L = b->cL;
+    a.Lmask = b->cLmask;
+    a.cnt = b->ccnt;
+    a.Ac = b->cAc;
+    a.Rpc = b->cRpc;
+    a.Rd = b->Rd;
+    a.cq = b->ccq;
+    a.flags = b->cflags;
+    a.cpar = b->cpar;
+    a.st = b->st;
+    a.t_start = b->t_start;
+    a.bar = b->bar;
+    a.order = b->order_d;
+    a.spin_limit = 1u << 24;
+    a.barrier = b->cbarrier;
+    return a;
+}
+
+static int contact_grid(paradmm_batch* b, int CS, int bpc, int* grid)
+{
+    if (bpc == 0) {  // one launch per iteration, no persistent grid
+        *grid = 0;
+        return PARADMM_OK;
+    }
+    int per_cu = 0;
+    RC(contact_occupancy(b, CS, &per_cu));
+    if (bpc < 0 || bpc > per_cu) {
+        pdm_set_error(PARADMM_ERR_ARG, "blocks_per_cu %d outside 0..%d for contact tile %d", bpc, per_cu, CS);
+        return PARADMM_ERR_ARG;
+    }
+    *grid = bpc * b->cus;
+    return PARADMM_OK;
+}
+
+// The contact iterations 1 .. max_iters with one launch per iteration: runs
+// of step launches up to each check, then the run's end, the quality and
+// the check kernels; the host stops once no instance is active.
+template <int CS>
+static int batch_contact_steps(paradmm_batch* b, const paradmm_solve_opts* o, const BCArgs& a)
+{
+    const unsigned ni = (unsigned)b->nitems, zg = (unsigned)((b->ncirc + 255) / 256);
+    std::vector<BState> st(b->ninst);
+    int par = 0, it = 1;
+    while (it < o->max_iters) {
+        const int end = std::min(o->max_iters, (it / o->check_every + 1) * o->check_every);
+        for (int j = 0; it + j < end; ++j) {
+            if (it + j + 1 == end) k_batch_contact_step<CS, true><<<ni, 64, 0, b->stream>>>(a, b->citems_d, par, j);
+            else k_batch_contact_step<CS, false><<<ni, 64, 0, b->stream>>>(a, b->citems_d, par, j);
+            par ^= 1;
+        }
+        k_batch_contact_zreduce<CS, true><<<zg, 256, 0, b->stream>>>(a, b->circ_inst_d, b->ncirc, end - it - 1);
+        k_batch_contact_quality<CS><<<(unsigned)b->cts, 64, 0, b->stream>>>(a, b->ctile_d);
+        k_batch_contact_check<<<(unsigned)b->ninst, 64, 0, b->stream>>>(a, end, end >= o->max_iters ? 1 : 0, par);
+        CK(hipGetLastError());
+        it = end;
+        if (it >= o->max_iters) break;
+        CK(hipMemcpyAsync(st.data(), b->st, sizeof(BState) * st.size(), hipMemcpyDeviceToHost, b->stream));
+        CK(hipStreamSynchronize(b->stream));
+        bool any = false;
+        for (const BState& x : st) any = any || x.active;
+        if (!any) break;
+    }
+    return PARADMM_OK;
+}
+
+// A contact solve from the uploaded state: the after-upload pass, the
+// entry, one cooperative launch over iterations 1 .. max_iters, the exit
+// and the pending U update.
+// *ms: the four together (each timed with events; b->last_ms has them;
+// the exit includes the pending U update of every edge).
+static int batch_contact_solve(paradmm_batch* b, const paradmm_solve_opts* o, float* ms)
+{
+    if (b->cbpc < 0) {  // not configured: one launch per iteration (measured faster than the persistent grid)
+        b->cbpc = 0;
+        b->cgrid = 0;
+    }
+    RC(batch_contact_alloc(b, b->ccs));
+    CK(hipMemsetAsync(b->cAc, 0, sizeof(unsigned long long) * 3 * CQ * (size_t)b->ncirc, b->stream));
+    CK(hipMemsetAsync(b->cflags, 0, sizeof(unsigned) * (size_t)b->ninst, b->stream));
+    float m0 = 0.0f;
+    RC(batch_launch(b, o, &m0, 0, 1, FIRST_NBUF));
+    BCArgs a = batch_contact_args(b, o, 1, o->max_iters, 0);
+    const unsigned nt = (unsigned)b->cts;
+    float me = 0.0f, mi = 0.0f, ml = 0.0f;
+    CK(hipEventRecord(b->ev0, b->stream));
+    if (b->ccs == 64) k_batch_contact_enter<64><<<nt, 256, 0, b->stream>>>(a, b->ctile_d);
+    else k_batch_contact_enter<32><<<nt, 256, 0, b->stream>>>(a, b->ctile_d);
+    CK(hipGetLastError());
+    CK(hipEventRecord(b->ev1, b->stream));
+    CK(hipEventSynchronize(b->ev1));
+    CK(hipEventElapsedTime(&me, b->ev0, b->ev1));
+    if (b->cbpc == 0) {  // one launch per iteration
+        CK(hipEventRecord(b->ev0, b->stream));
+        RC(b->ccs == 64 ? batch_contact_steps<64>(b, o, a) : batch_contact_steps<32>(b, o, a));
+        CK(hipEventRecord(b->ev1, b->stream));
+        CK(hipEventSynchronize(b->ev1));
+        CK(hipEventElapsedTime(&mi, b->ev0, b->ev1));
+    } else {  // one cooperative launch
+        CK(hipMemsetAsync(b->bar, 0, sizeof(unsigned) * b->bar_words, b->stream));
+        void* args[] = {&a};
+        const void* fn = contact_kernel(b->ccs, b->ninst);
+        CK(hipEventRecord(b->ev0, b->stream));
+        CK(hipLaunchCooperativeKernel(fn, dim3(b->cgrid), dim3(256), args, 0, b->stream));
+        CK(hipEventRecord(b->ev1, b->stream));
+        CK(hipEventSynchronize(b->ev1));
+        CK(hipEventElapsedTime(&mi, b->ev0, b->ev1));
+        unsigned aborted = 0;
+        CK(hipMemcpy(&aborted, b->bar + (GRIDBAR_GROUPS + 2) * GRIDBAR_LINE, sizeof aborted, hipMemcpyDeviceToHost));
+        if (aborted) {
+            pdm_set_error(PARADMM_ERR_HIP, "batch: a grid barrier wait timed out (workgroups not co-resident?)");
+            return PARADMM_ERR_HIP;
+        }
+    }
+    CK(hipEventRecord(b->ev0, b->stream));
+    if (b->ccs == 64) k_batch_contact_leave<64><<<nt, 256, 0, b->stream>>>(a, b->ctile_d);
+    else k_batch_contact_leave<32><<<nt, 256, 0, b->stream>>>(a, b->ctile_d);
+    CK(hipGetLastError());
+    if (b->T == 64) k_batch_finish_u<64><<<(unsigned)b->ntiles, 256, 0, b->stream>>>(b->tiles_d, b->inst_d, b->slots, b->X, b->U, b->Z);
+    else k_batch_finish_u<32><<<(unsigned)b->ntiles, 256, 0, b->stream>>>(b->tiles_d, b->inst_d, b->slots, b->X, b->U, b->Z);
+    CK(hipGetLastError());
+    CK(hipEventRecord(b->ev1, b->stream));
+    CK(hipEventSynchronize(b->ev1));
+    CK(hipEventElapsedTime(&ml, b->ev0, b->ev1));
+    b->last_ms[0] = m0;
+    b->last_ms[1] = me;
+    b->last_ms[2] = mi;
+    b->last_ms[3] = ml;
+    *ms = m0 + me + mi + ml;
+    return PARADMM_OK;
+}
+
 // The learned-rho batch: launches of policy_every iterations; after each,
 // the pending U update and the check of every active instance (a
 // finish-only launch), then the policy on the instances still active.
@@ -1076,6 +1784,55 @@ extern "C" int paradmm_batch_configure(paradmm_batch* b, int tile, int blocks_pe
     return PARADMM_OK;
 }
 
+extern "C" int paradmm_batch_configure_contact(paradmm_batch* b, int contact_tile, int blocks_per_cu, int barrier)
+{
+    if (!b || (contact_tile != 0 && contact_tile != 32 && contact_tile != 64) || (barrier != 0 && barrier != 1)) {
+        pdm_set_error(PARADMM_ERR_ARG,
+                      "paradmm_batch_configure_contact: contact tile 0 (off), 32 or 64; barrier 0 or 1");
+        return PARADMM_ERR_ARG;
+    }
+    if (contact_tile == 0) {
+        b->contact_off = true;
+        return PARADMM_OK;
+    }
+    int grid = 0;
+    RC(contact_grid(b, contact_tile, blocks_per_cu, &grid));
+    b->contact_off = false;
+    b->ccs = contact_tile;
+    b->cbpc = blocks_per_cu;
+    b->cgrid = grid;
+    b->cbarrier = barrier;
+    return PARADMM_OK;
+}
+
+extern "C" int paradmm_batch_contact_info(paradmm_batch* b, int* contact_tile, int* blocks_per_cu, int* grid,
+                                          long* contact_tiles, long* listed_pairs, double* phase_ms)
+{
+    if (!b) {
+        pdm_set_error(PARADMM_ERR_ARG, "paradmm_batch_contact_info: NULL batch");
+        return PARADMM_ERR_ARG;
+    }
+    if (contact_tile) *contact_tile = b->contact_off ? 0 : b->ccs;
+    if (blocks_per_cu) *blocks_per_cu = b->cbpc;
+    if (grid) *grid = b->cgrid;
+    if (contact_tiles) *contact_tiles = b->ccs_alloc ? b->cts : 0;
+    if (phase_ms)
+        for (int k = 0; k < 4; ++k) phase_ms[k] = b->last_ms[k];
+    if (listed_pairs) {  // the last iteration's list of every instance
+        *listed_pairs = 0;
+        if (b->ccs_alloc && b->last_ms[2] > 0.0f) {
+            std::vector<int> cnt(2 * (size_t)b->cts), par(b->ninst);
+            CK(hipMemcpy(cnt.data(), b->ccnt, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost));
+            CK(hipMemcpy(par.data(), b->cpar, sizeof(int) * par.size(), hipMemcpyDeviceToHost));
+            for (int i = 0; i < b->ninst; ++i) {
+                const BCInst& c = b->cinst[i];
+                for (int t = 0; t < c.nts; ++t) *listed_pairs += cnt[2 * (size_t)c.l0 + (size_t)par[i] * c.nts + t];
+            }
+        }
+    }
+    return PARADMM_OK;
+}
+
 extern "C" int paradmm_batch_autotune(paradmm_batch* b, int iters, paradmm_batch_config* chosen,
                                       paradmm_batch_config* tried, int max_tried, int* ntried)
 {
@@ -1092,8 +1849,45 @@ extern "C" int paradmm_batch_autotune(paradmm_batch* b, int iters, paradmm_batch
     o.check_every = iters;
     o.tol_primal = o.tol_dual = -1.0;
     o.tol_feas = -1.0;
-    paradmm_batch_config best{0, 0, 0, 0.0};
+    paradmm_batch_config best{0, 0, 0, 0.0, 0};
     int n = 0;
+    // the plain fixed-rho batch runs in the contact state when it can: then
+    // its candidates are the contact kernel's (contact tile side x blocks
+    // per CU x barrier), each timed from a fresh upload over iters - 1
+    // contact iterations, with the dense layout at tile side 32 (which only
+    // the after-upload pass, the entry and the exit use)
+    if (!b->T) RC(paradmm_batch_configure(b, 32, 1, 1));
+    RC(batch_upload(b));
+    if (iters >= 3 && batch_contact_applies(b, &o)) {
+        if (b->T != 32) {
+            RC(paradmm_batch_configure(b, 32, 1, b->barrier));
+        }
+        const int sides[2] = {32, 64};
+        for (int si = 0; si < 2; ++si) {
+            const int CS = sides[si];
+            int per_cu = 0;
+            RC(contact_occupancy(b, CS, &per_cu));
+            for (int bpc = 0; bpc <= per_cu && bpc <= 4; ++bpc)
+                for (int bar = 0; bar < 2; ++bar) {
+                    if (bpc == 0 && bar == 1) continue;  // no grid barrier in the launch-per-iteration form
+                    RC(paradmm_batch_configure_contact(b, CS, bpc, bar));
+                    float ms = 0.0f;
+                    RC(batch_upload(b));
+                    RC(batch_contact_solve(b, &o, &ms));  // warm
+                    RC(batch_upload(b));
+                    RC(batch_contact_solve(b, &o, &ms));
+                    const double per = b->last_ms[2] / (iters - 1);
+                    const paradmm_batch_config c{b->T, bpc, bar, per, CS};
+                    if (tried && n < max_tried) tried[n] = c;
+                    ++n;
+                    if (best.contact_tile == 0 || per < best.ms_per_iter) best = c;
+                }
+        }
+        if (ntried) *ntried = n;
+        RC(paradmm_batch_configure_contact(b, best.contact_tile, best.blocks_per_cu, best.barrier));
+        if (chosen) *chosen = best;
+        return PARADMM_OK;
+    }
     const int tiles[2] = {32, 64};
     for (int ti = 0; ti < 2; ++ti) {
         const int T = tiles[ti];
@@ -1113,9 +1907,9 @@ extern "C" int paradmm_batch_autotune(paradmm_batch* b, int iters, paradmm_batch
                 RC(batch_reset_flags(b));      // the last check retired every instance
                 RC(batch_launch(b, &o, &ms));
                 const double per = ms / iters;
-                if (tried && n < max_tried) tried[n] = paradmm_batch_config{T, bpc, bar, per};
+                if (tried && n < max_tried) tried[n] = paradmm_batch_config{T, bpc, bar, per, 0};
                 ++n;
-                if (best.tile == 0 || per < best.ms_per_iter) best = paradmm_batch_config{T, bpc, bar, per};
+                if (best.tile == 0 || per < best.ms_per_iter) best = paradmm_batch_config{T, bpc, bar, per, 0};
             }
     }
     if (ntried) *ntried = n;
@@ -1141,7 +1935,9 @@ extern "C" int paradmm_batch_solve(paradmm_batch* b, const paradmm_solve_opts* o
     RC(batch_upload(b));
     float ms = 0.0f;
     std::vector<BState> st(b->ninst);
-    if (o->policy_every <= 0) {
+    if (batch_contact_applies(b, o)) {
+        RC(batch_contact_solve(b, o, &ms));
+    } else if (o->policy_every <= 0) {
         RC(batch_launch(b, o, &ms));
     } else {
         RC(batch_policy_solve(b, o, st, &ms));

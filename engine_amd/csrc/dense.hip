@@ -40,6 +40,7 @@
 #include "paradmm_gpu.h"
 #include "mlp_device.h"
 #include "pairtile.h"
+#include "contact_device.h"
 
 #define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hip_fail(e_, #x, __FILE__, __LINE__); } while (0)
 #define RC(x) do { int r_ = (x); if (r_) return r_; } while (0)
@@ -197,548 +198,19 @@ __global__ void k_fill_u32(unsigned* p, int n, unsigned v)
     if (i < n) p[i] = v;
 }
 
-// ------------------------------------------------------------------ contact state
-//
-// With alpha = 1 the U update is U_{k+1} = M_k - Z_{k+1}.  A pair factor
-// whose prox is inactive returns X = N = Z_k - U_k, so M_k = X + U = Z_k:
-// its edges carry no state of their own, and at the next iteration
-// U = Z_k - Z_{k+1}, N = Z_{k+1} - U, the same for every inactive pair of a
-// circle.  So only the pairs whose prox was active (the contacts, about 1%
-// of all pairs for N = 5000 circles) need per-edge values.  The contact
-// state covers the pairs with contact tiles of CS x CS circles (its own
-// tiling, independent of the dense layout's); a contact tile keeps a list
-// of its active pairs, each entry the M of both edges and the pair's slot,
-// and an iteration reads that list, tests every pair of the tile against
-// the circles' common N, runs the prox on the colliding ones and writes the
-// new list.  The box and radius edges stay in their dense X and U slots.
-//
-// Per circle a tile sums, over its pair edges that are in either list,
-//   a = M - Z_k                      (consensus: Z_{k+1} = Z_k + sum a / (N + 1)),
-//   g = (X - Z_k) - (Z_k - Z_{k-1})  and |g|^2,
-// where a and g are 0 on every other edge.  The primal residual
-// sum |X - Z_{k+1}|^2 is then, per circle, |g - e|^2 summed over all N + 1
-// edges with e = Z_{k+1} - 2 Z_k + Z_{k-1}: S2 - 2 e.S1 + (N + 1)|e|^2.
-// The sums run in fixed point, each value rounded to the nearest multiple
-// of 2^-31 (64-bit LDS atomics inside a tile, then 64-bit global atomics of
-// the tile's nonzero sums into the circle's accumulator), so they do not
-// depend on the order the lanes or the tiles add them: runs are bitwise
-// reproducible.  |g|^2 is held exactly to 2^-62 as two sums, its multiple
-// of 2^-31 and the remainder in units of 2^-62, so the primal residual does
-// not lose the small terms of a nearly converged iterate.  A value the
-// fixed point cannot hold (non-finite, or |v| >= 2^30) is not added: it
-// sets the state's flag, and the residuals read NaN from then on.
-//
-// The per-pair test replaces 24 bytes of M per edge read and written by
-// ~10 VALU operations; the state is Z, Z_prev, Z_prev2 and the lists.
-
-constexpr int CQ = 8;         // per-circle sums: a (3), g (3), |g|^2 (multiple of 2^-31, remainder in 2^-62)
-constexpr double CFIX_INV = 1.0 / 2147483648.0;  // 2^-31
-constexpr double CFIX2_INV = CFIX_INV * CFIX_INV; // 2^-62
-
-struct CArgs {
-    int N;
-    int nbs, nts;          // contact tiling: blocks of CS circles, tiles
-    int nb, T;             // dense layout: blocks, tile side (box and radius slots)
-    long stride;
-    float* X;
-    float* U;
-    float* Z;
-    float* Zp;
-    float* Zpp;
-    const float* aexp;
-    const float* uni;      // {rho, alpha}
-    const float* rho;      // per-edge rho (dense slots), read when rho differs between edges
-    float* W;              // [2][N] sum rho, sum rho^2 over each circle's edges (per-edge rho)
-    int par;               // the list the last iteration wrote
-    float4* L;             // [2][nts][CS*CS][2]
-    unsigned* Lmask;       // [2][nts][CS][CS/32] pairs of each list, per tile row
-    int* cnt;              // [2][nts]
-    unsigned long long* Ac; // [CQ][N] this launch's circle sums, 2^-31 fixed point (starts at zero)
-    const unsigned long long* Ap; // [CQ][N] the previous launch's sums, not yet applied to Z (or null)
-    unsigned long long* Acl; // [CQ][N] the sums two launches back, cleared here (or null)
-    const float* Zin;      // the Z the pending sums Ap apply to
-    const float* Zpin;     // (run end) the Z before Zin
-    float* Zo;             // Zin + Ap / (N + 1): this launch's Z, stored by the box-and-radius wavefronts
-    double* Rpc;
-    double* Rd;
-    unsigned* flag;        // set when a circle sum could not take a value
-};
-
-// v rounded to the nearest multiple of 2^-31, in units of 2^-31:
-// v = k + f with k = rint(v) and f = v - k exact (|f| <= 1/2), so
-// v 2^31 = k 2^31 + rint(f 2^31) needs no FP64.  A value outside
-// |v| < 2^30 (NaN included) gives 0 and sets bad.
-__device__ __forceinline__ long long cfix(float v, unsigned& bad)
-{
-    if (!(fabsf(v) < 1073741824.0f)) {
-        bad = 1u;
-        return 0;
-    }
-    const float k = rintf(v);
-    const float f = v - k;
-    return (long long)(int)k * 2147483648LL + (long long)(int)rintf(f * 2147483648.0f);
-}
-
-// v >= 0 exactly to 2^-62 as hi 2^-31 + lo 2^-62: hi = the multiple of
-// 2^-31 below v, lo = rint((v - hi 2^-31) 2^62) with the remainder exact
-__device__ __forceinline__ void cfix2(float v, long long& hi, long long& lo, unsigned& bad)
-{
-    if (!(fabsf(v) < 1073741824.0f)) {
-        bad = 1u;
-        hi = lo = 0;
-        return;
-    }
-    const float t = v * 2147483648.0f;  // exact
-    const float k = floorf(t);
-    hi = (long long)k;
-    lo = (long long)rintf((t - k) * 2147483648.0f);  // t - k exact, in [0, 1)
-}
-
-// a circle sum (units of 2^-31) as fp32: one rounding, no FP64
-__device__ __forceinline__ float csum_f32(unsigned long long v) { return (float)(long long)v * 4.656612873077393e-10f; }
-
-template <typename SH>
-__device__ __forceinline__ void cadd(SH& sh, unsigned long long* acc, float v)
-{
-    atomicAdd(acc, (unsigned long long)cfix(v, sh.bad));
-}
-
-template <typename SH>
-__device__ __forceinline__ void cadd2(SH& sh, unsigned long long* acc, float v)
-{
-    long long hi, lo;
-    cfix2(v, hi, lo, sh.bad);
-    atomicAdd(acc, (unsigned long long)hi);
-    atomicAdd(acc + 1, (unsigned long long)lo);
-}
-
-// (I, J) of tile t of a triangular tiling with nb blocks: row I starts at
-// I nb - I (I - 1) / 2 (the float estimate is corrected by the loops)
-__device__ __forceinline__ void tri_tile(int t, int nb, int& I, int& J)
-{
-    const float b = 2.0f * nb + 1.0f;
-    I = (int)((b - sqrtf(fmaxf(b * b - 8.0f * (float)t, 0.0f))) * 0.5f);
-    I = max(0, min(I, nb - 1));
-    while (I > 0 && I * nb - I * (I - 1) / 2 > t) --I;
-    while (I + 1 < nb && (I + 1) * nb - (I + 1) * I / 2 <= t) ++I;
-    J = I + (t - (I * nb - I * (I - 1) / 2));
-}
-
-// dense-layout slot of the edge of circle i in pair (i, j), i < j (the
-// edge of j is T*T further)
-__device__ __forceinline__ long dense_pair_slot(int i, int j, int nb, int T)
-{
-    const int I = i / T, J = j / T;
-    const long t = (long)I * nb - (long)I * (I - 1) / 2 + (J - I);
-    return t * (2L * T * T) + (long)(i % T) * T + (j % T);
-}
-
-// box slot of circle c (its radius slot is T*T further)
-__device__ __forceinline__ long dense_unary_slot(int c, int nb, int T)
-{
-    const int I = c / T;
-    const long t = (long)I * nb - (long)I * (I - 1) / 2;
-    return t * (2L * T * T) + (long)(c % T) * T + (c % T);
-}
-
-// Exclusive position of this thread's n items among the workgroup's (in
-// thread order) and the workgroup total.  Contains one __syncthreads.
-__device__ __forceinline__ int block_offset(int n, int (&wsum)[4], int& total)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int s = n;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(s, o, 64);
-        s += lane >= o ? y : 0;
-    }
-    if (lane == 63) wsum[w] = s;
-    __syncthreads();
-    int base = 0;
-    total = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        base += q < w ? wsum[q] : 0;
-        total += wsum[q];
-    }
-    return base + s - n;
-}
-
-// Orders one wavefront's LDS accesses across its lanes.  A wavefront's LDS
-// operations execute in issue order, so a wavefront-scope fence (a compiler
-// ordering point, no instruction) suffices; s_waitcnt 0 would also wait for
-// the wavefront's list stores to reach memory.
-__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
-
-// Shared scratch of one contact tile (one wavefront).  The pairs to project
-// form one stream per tile: the previous list's pairs, then the colliding
-// pairs found by the tests, which pass through a ring in LDS.
-constexpr int CRING = 2048;  // > 63 pending + the 1024 pairs one test step can add
-
-template <int CS, int Q>
-struct ContactWave {
-    unsigned long long acc[2 * CS][Q];  // fixed-point circle sums: rows (block P), then columns (block Q)
-    float zc[2 * CS][6];                // Z_k and c = Z_k - Z_{k-1} of the tile's 2 CS circles
-    unsigned short ring[CRING];         // colliding pairs not in the previous list (slot)
-    unsigned mask[CS][CS / 32];         // pairs in the previous list, per row
-    unsigned nmask[CS][CS / 32];        // pairs in the new list
-    unsigned bad;                       // a circle sum could not take a value
-};
-
-// One pair of the contact tile through the prox: N from its list entry (ex)
-// or the circles' common N, the circle sums; returns whether the prox was
-// active, with the new M of both edges.
-template <int CS, int Q, bool RE>
-__device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bool ex, float4 pa, float4 pb,
-                                             bool diag, float rA, float rB, float (&mA)[3], float (&mB)[3])
-{
-    const int rl = idx / CS, cl = idx % CS;
-    const int cs = diag ? cl : CS + cl;
-    float zA[3], zB[3], cA[3], cB[3], uA[3], uB[3], nA[3], nB[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        zA[k] = sh.zc[rl][k];
-        cA[k] = sh.zc[rl][3 + k];
-        zB[k] = sh.zc[CS + cl][k];
-        cB[k] = sh.zc[CS + cl][3 + k];
-    }
-    if (ex) {
-        const float ma[3] = {pa.x, pa.y, pa.z}, mb[3] = {pb.x, pb.y, pb.z};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            uA[k] = ma[k] - zA[k];
-            uB[k] = mb[k] - zB[k];
-            nA[k] = zA[k] - uA[k];
-            nB[k] = zB[k] - uB[k];
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            uA[k] = -cA[k];  // Z_{k-1} - Z_k, exactly
-            uB[k] = -cB[k];
-            nA[k] = zA[k] + cA[k];  // Z_k - (Z_{k-1} - Z_k), exactly
-            nB[k] = zB[k] + cB[k];
-        }
-    }
-    float xA[3] = {nA[0], nA[1], nA[2]}, xB[3] = {nB[0], nB[1], nB[2]};
-    const bool act = pair_project(nA[0], nA[1], nA[2], nB[0], nB[1], nB[2], rA, rA, rB, rB, xA[0], xA[1], xA[2],
-                                  xB[0], xB[1], xB[2]);
-    if constexpr (Q == CQ) if (act || ex) {  // g = 0 on the other edges
-        float gA2 = 0.0f, gB2 = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const float gA = (xA[k] - zA[k]) - cA[k];
-            const float gB = (xB[k] - zB[k]) - cB[k];
-            cadd(sh, &sh.acc[rl][3 + k], gA);
-            cadd(sh, &sh.acc[cs][3 + k], gB);
-            gA2 += gA * gA;
-            gB2 += gB * gB;
-        }
-        cadd2(sh, &sh.acc[rl][6], gA2);
-        cadd2(sh, &sh.acc[cs][6], gB2);
-    }
-    if (act) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            mA[k] = xA[k] + uA[k];
-            mB[k] = xB[k] + uB[k];
-            // per-edge rho: the consensus weighs each deviation with its edge's rho
-            cadd(sh, &sh.acc[rl][k], RE ? rA * (mA[k] - zA[k]) : mA[k] - zA[k]);
-            cadd(sh, &sh.acc[cs][k], RE ? rB * (mB[k] - zB[k]) : mB[k] - zB[k]);
-        }
-    }
-    return act;
-}
-
-// One round of the tile's pair stream: items d .. d + 63 (item < cp: entry
-// of the previous list, else ring slot item - cp); the active pairs are
-// appended to the new list in lane order.
-template <int CS, int Q, bool RE>
-__device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArgs& a, int P, int Qb, int d, int avail,
-                                              int cp, const float4* Lp, float4 pre0, float4 pre1, float4* Ln,
-                                              int& base, bool diag, float rho)
-{
-    constexpr int CSS = CS * CS;
-    const int lane = threadIdx.x;
-    const int item = d + lane;
-    bool act = false;
-    int idx = 0;
-    float mA[3], mB[3];
-    if (item < avail) {
-        const bool ex = item < cp;
-        float4 pa = pre0, pb = pre1;
-        if (ex) {
-            if (d > 0) {
-                pa = Lp[2 * item];
-                pb = Lp[2 * item + 1];
-            }
-            idx = __float_as_int(pa.w) & (CSS - 1);
-        } else {
-            idx = sh.ring[(item - cp) & (CRING - 1)];
-        }
-        float rA = rho, rB = rho;
-        if constexpr (RE) {  // the pair's own rho from its dense slots
-            const long q = dense_pair_slot(P * CS + idx / CS, Qb * CS + idx % CS, a.nb, a.T);
-            rA = a.rho[q];
-            rB = a.rho[q + (long)a.T * a.T];
-        }
-        act = contact_pair<CS, Q, RE>(sh, idx, ex, pa, pb, diag, rA, rB, mA, mB);
-    }
-    const unsigned long long bal = __ballot(act);
-    if (act) {
-        const int slot = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
-        Ln[2 * slot] = make_float4(mA[0], mA[1], mA[2], __int_as_float(idx));
-        Ln[2 * slot + 1] = make_float4(mB[0], mB[1], mB[2], 0.0f);
-        atomicOr(&sh.nmask[idx / CS][(idx % CS) >> 5], 1u << (idx & 31));
-    }
-    base += (int)__popcll(bal);
-}
-
-// One contact iteration of one contact tile (P, Q) of side CS: a wavefront
-// of 64 lanes, lane l owning rows RL (l / 8) .. +RL-1 and columns
-// RL (l % 8) .. +RL-1 (RL = CS / 8).  The tile's pair stream is the
-// previous list's pairs, then, two rows of every lane at a time, the pairs
-// that collide with the circles' common N and are not in that list (lane
-// order, then pair order); whenever 64 items are pending they go through
-// the prox as one round.  The new list is written in stream order.  RES:
-// also the residual sums (the last iteration before residuals are read).
-// RE: rho differs between edges; the pairs taken through the prox read
-// theirs from the dense slots (inactive pairs need none).
-// Circle cc's consensus of this launch, z = Z_k, and c = Z_k - Z_{k-1}.
-// Inside a run the previous launch's sums are applied here, by every
-// wavefront that needs the circle (the same operations as the run's last
-// update in k_contact_zreduce, so the same bits): the Z update needs no
-// launch of its own between iterations.
-template <bool RE>
-__device__ __forceinline__ void contact_zc(const CArgs& a, int cc, float (&z)[3], float (&c)[3])
-{
-    if (a.Ap) {
-        const float den = RE ? a.W[cc] : (float)(a.N + 1);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const float zk = a.Zin[3 * cc + k];
-            z[k] = zk + csum_f32(a.Ap[(long)k * a.N + cc]) / den;
-            c[k] = z[k] - zk;
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            z[k] = a.Z[3 * cc + k];
-            c[k] = z[k] - a.Zp[3 * cc + k];
-        }
-    }
-}
-
-// The box and radius edges of the CS circles of block P, one circle per
-// lane: the plain U update, prox and X, U stores of the fused iteration,
-// their sums added to the circles' accumulators.  These are wavefronts of
-// their own in the tile launch, so the diagonal tiles are no longer than
-// the rest.
-template <int CS, bool RES, bool RE>
-__device__ __forceinline__ void contact_unary(const CArgs& a, int P)
-{
-    constexpr int Q = RES ? CQ : 3;
-    const long S = a.stride, TT = (long)a.T * a.T;
-    for (int e = threadIdx.x; e < CS; e += 64) {
-        const int cc = P * CS + e;
-        if (cc >= a.N) break;
-        float z[3], cz[3];
-        contact_zc<RE>(a, cc, z, cz);
-        if (a.Zo)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) a.Zo[3 * cc + k] = z[k];
-        if (a.Acl)
-#pragma unroll
-            for (int q = 0; q < CQ; ++q) a.Acl[(long)q * a.N + cc] = 0ull;
-        const long p0 = dense_unary_slot(cc, a.nb, a.T);
-        long long s[Q];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) s[q] = 0;
-        float g2 = 0.0f;
-        unsigned bad = 0u;
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const long q = p0 + side * TT;
-            const float r = RE ? a.rho[q] : a.uni[0];
-            float x[3], u[3], nn[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                x[k] = a.X[k * S + q];
-                u[k] = (x[k] + a.U[k * S + q]) - z[k];
-                nn[k] = z[k] - u[k];
-            }
-            if (side == 0) {
-                box3(x[0], x[1], x[2], nn[0], nn[1], nn[2]);
-            } else {
-                x[0] = nn[0];
-                x[1] = nn[1];
-                x[2] = max_alpha_radius(nn[2], r, a.aexp[cc]);
-            }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                a.X[k * S + q] = x[k];
-                a.U[k * S + q] = u[k];
-                s[k] += cfix(RE ? r * ((x[k] + u[k]) - z[k]) : (x[k] + u[k]) - z[k], bad);
-                if constexpr (RES) {
-                    const float g = (x[k] - z[k]) - cz[k];
-                    s[3 + k] += cfix(g, bad);
-                    g2 += g * g;
-                }
-            }
-            if constexpr (RES) {
-                long long hi, lo;
-                cfix2(g2, hi, lo, bad);
-                s[6] += hi;
-                s[7] += lo;
-                g2 = 0.0f;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
-            if (s[q]) atomicAdd(&a.Ac[(long)q * a.N + cc], (unsigned long long)s[q]);
-        if (bad) atomicOr(a.flag, 1u);
-    }
-}
-
 // Launch: nbs + nts wavefronts, the first nbs the blocks' box and radius
 // edges (contact_unary), then the contact tiles.
 template <int CS, bool RES, bool RE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_contact_tile(CArgs a)
 {
-    constexpr int Q = RES ? CQ : 3;
-    constexpr int CSS = CS * CS, RL = CS / 8, W = CS / 32;
-    __shared__ ContactWave<CS, Q> sh;
-    const int lane = threadIdx.x;
-    const int N = a.N, nbs = a.nbs;
-    if ((int)blockIdx.x < nbs) {
+    __shared__ ContactWave<CS, RES ? CQ : 3> sh;
+    if ((int)blockIdx.x < a.nbs) {
         contact_unary<CS, RES, RE>(a, blockIdx.x);
         return;
     }
-    const int t = blockIdx.x - nbs;
-    int P, Qb;
-    tri_tile(t, nbs, P, Qb);
-    const bool diag = P == Qb;
-    const long tp = (long)a.par * a.nts + t, tn = (long)(a.par ^ 1) * a.nts + t;
-    const float4* Lp = a.L + tp * (2 * CSS);
-    float4* Ln = a.L + tn * (2 * CSS);
-    const int cp = min(a.cnt[tp], CSS);
-    const float rho = a.uni[0];
-
-    // circle table: entries 0..CS-1 the rows' circles, CS..2CS-1 the columns'
-    for (int e = lane; e < 2 * CS; e += 64) {
-        const int cc = min((e < CS ? P * CS + e : Qb * CS + (e - CS)), N - 1);
-        float z[3], c[3];
-        contact_zc<RE>(a, cc, z, c);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            sh.zc[e][k] = z[k];
-            sh.zc[e][3 + k] = c[k];
-        }
-    }
-    for (int e = lane; e < CS * W; e += 64) {
-        (&sh.mask[0][0])[e] = a.Lmask[tp * CS * W + e];
-        (&sh.nmask[0][0])[e] = 0u;
-    }
-    for (int k = lane; k < 2 * CS * Q; k += 64) (&sh.acc[0][0])[k] = 0ull;
-    if (lane == 0) sh.bad = 0u;
-    wave_lds_fence();
-    // the first 64 entries of the previous list, in flight during the tests
-    float4 pre0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pre1 = pre0;
-    if (lane < cp) {
-        pre0 = Lp[2 * lane];
-        pre1 = Lp[2 * lane + 1];
-    }
-
-    const int r0 = (lane >> 3) * RL, c0 = (lane & 7) * RL;
-    const bool interior = !diag && (Qb + 1) * CS <= N;
-    float nc[RL][3];
-#pragma unroll
-    for (int c = 0; c < RL; ++c)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) nc[c][k] = sh.zc[CS + c0 + c][k] + sh.zc[CS + c0 + c][3 + k];
-    int base = 0;           // entries of the new list so far
-    int done = 0;           // stream items taken through the prox
-    int avail = cp;         // stream items so far (previous list + ring)
-    while (avail - done >= 64) {
-        contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
-        done += 64;
-    }
-#pragma unroll 1
-    for (int h = 0; h < RL / 2; ++h) {
-        // the tests of rows r0 + 2h, r0 + 2h + 1
-        unsigned qm = 0u;
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int rl = r0 + 2 * h + r;
-            float nr[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) nr[k] = sh.zc[rl][k] + sh.zc[rl][3 + k];
-#pragma unroll
-            for (int c = 0; c < RL; ++c) {
-                const bool hit = pair_hits(nr[0], nr[1], nr[2], nc[c][0], nc[c][1], nc[c][2]);
-                qm |= hit ? 1u << (RL * r + c) : 0u;
-            }
-            const unsigned ex = (sh.mask[rl][c0 >> 5] >> (c0 & 31)) & ((1u << RL) - 1u);
-            qm &= ~(ex << (RL * r));
-        }
-        if (!interior) {  // pairs of the tile: not below the diagonal, not past N
-            unsigned vmask = 0u;
-#pragma unroll
-            for (int r = 0; r < 2; ++r)
-#pragma unroll
-                for (int c = 0; c < RL; ++c) {
-                    const int rl = r0 + 2 * h + r, cl = c0 + c;
-                    const bool vm = (P * CS + rl < N) && (Qb * CS + cl < N) && (!diag || rl < cl);
-                    vmask |= vm ? 1u << (RL * r + c) : 0u;
-                }
-            qm &= vmask;
-        }
-        const int n = __popc(qm);
-        int s = n;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(s, o, 64);
-            s += lane >= o ? y : 0;
-        }
-        int off = avail - cp + s - n;
-        avail += __shfl(s, 63, 64);
-        unsigned m = qm;
-        while (m) {
-            const int b = __ffs(m) - 1;
-            m &= m - 1u;
-            sh.ring[(off++) & (CRING - 1)] = (unsigned short)((r0 + 2 * h + b / RL) * CS + c0 + b % RL);
-        }
-        wave_lds_fence();
-        while (avail - done >= 64) {
-            contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
-            done += 64;
-        }
-    }
-    if (avail > done) contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
-    if (lane == 0) a.cnt[tn] = base;
-
-    wave_lds_fence();
-    for (int e = lane; e < CS * W; e += 64) a.Lmask[tn * CS * W + e] = (&sh.nmask[0][0])[e];
-    // the tile's nonzero circle sums into the circles' accumulators (most
-    // circles of a tile have no active pair in it)
-    for (int e = lane; e < (diag ? CS : 2 * CS); e += 64) {
-        const int circ = e < CS ? P * CS + e : Qb * CS + (e - CS);
-        if (circ < N)
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const unsigned long long v = sh.acc[e][q];
-                if (v) atomicAdd(&a.Ac[(long)q * N + circ], v);
-            }
-    }
-    if (lane == 0 && sh.bad) atomicOr(a.flag, 1u);
+    contact_tile_body<CS, RES, RE>(a, (int)blockIdx.x - a.nbs, sh);
 }
 
-// The end of a run: Z_{k+1} = Z_k + sum a / (N + 1) from the last launch's
-// sums (RE, rho per edge: Z_k + sum rho a / sum rho) into the canonical Z,
-// Z_prev, Z_prev2 (Zin = Z_k, Zpin = Z_{k-1}: canonical or inside the run's
-// ring), the clearing of the sum buffers and, RES, the per-circle primal
-// and dual residuals.  One thread per circle.
-constexpr int CZ_THREADS = 256;
 
 // Per-edge rho: W[c] = sum rho and W[N + c] = sum rho^2 over circle c's N + 1
 // edges, FP64 in a fixed order (lane j's pairs j, j + 64, ..., then the
@@ -776,187 +248,24 @@ __global__ __launch_bounds__(64) void k_contact_weights(CArgs a)
 template <bool RES, bool RE>
 __global__ __launch_bounds__(CZ_THREADS) void k_contact_zreduce(CArgs a)
 {
-    constexpr int Q = RES ? CQ : 3;
     const int c = blockIdx.x * CZ_THREADS + threadIdx.x;
-    const int N = a.N;
-    if (c >= N) return;
-    unsigned long long sa[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) sa[k] = a.Ac[(long)k * N + c];
-    double s[Q];
-#pragma unroll
-    for (int q = 3; q < Q; ++q) s[q] = (double)(long long)a.Ac[(long)q * N + c] * (q == 7 ? CFIX2_INV : CFIX_INV);
-    float zk[3], zp[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        zk[k] = a.Zin[3 * c + k];
-        zp[k] = a.Zpin[3 * c + k];
-    }
-    // every sum buffer of the run is zero for the next one
-#pragma unroll
-    for (int q = 0; q < CQ; ++q) {
-        a.Ac[(long)q * N + c] = 0ull;
-        a.Acl[(long)q * N + c] = 0ull;
-    }
-    const float deg = (float)(N + 1);
-    double p = 0.0, dz2 = 0.0;
-    if constexpr (RES) p = s[6] + s[7];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        // the same operations as contact_zc's, so the same bits
-        const float zn = zk[k] + csum_f32(sa[k]) / (RE ? a.W[c] : deg);
-        if constexpr (RES) {
-            const float dz = zn - zk[k];
-            const double e = (double)(dz - (zk[k] - zp[k]));
-            p += -2.0 * e * s[3 + k] + (double)deg * e * e;
-            dz2 += (double)dz * dz;
-        }
-        a.Zpp[3 * c + k] = zp[k];
-        a.Zp[3 * c + k] = zk[k];
-        a.Z[3 * c + k] = zn;
-    }
-    if constexpr (RES) {
-        const float rho = a.uni[0];
-        // S2 - 2 e.S1 + (N + 1)|e|^2 >= 0; the roundings of the two sides can
-        // leave a tiny negative
-        a.Rpc[c] = p > 0.0 ? p : 0.0;
-        a.Rd[c] = RE ? (double)a.W[a.N + c] * dz2 : (double)rho * rho * (double)deg * dz2;
-    }
+    if (c < a.N) contact_zreduce_circle<RES, RE>(a, c);
 }
 
-// Enters the contact state from the steady X, U state (U one update behind,
-// Z = Z_k, Z_prev = Z_{k-1}): list 0 of contact tile blockIdx.x receives
-// every pair with M = X + U != Z_{k-1} on either edge, exactly, so the first
-// contact iteration computes the same U = M - Z_k as the fused one.  256
-// threads; thread: column tid % CS of rows tid / CS, + 256 / CS, ... (a
-// wavefront reads consecutive slots of one row of the dense tiles).
+// Enters the contact state (contact_enter_tile), one workgroup per contact tile.
 template <int CS>
 __global__ __launch_bounds__(256) void k_contact_enter(CArgs a)
 {
-    constexpr int CSS = CS * CS, W = CS / 32, RS = 256 / CS, CPT = CSS / 256;
-    __shared__ int wsum[4];
-    __shared__ unsigned emask[CS][W];
-    const int tid = threadIdx.x;
-    const int t = blockIdx.x;
-    const int N = a.N;
-    const long S = a.stride, TT = (long)a.T * a.T;
-    int P, Qb;
-    tri_tile(t, a.nbs, P, Qb);
-    if (tid < CS * W) (&emask[0][0])[tid] = 0u;
-    __syncthreads();
-    float4* L0 = a.L + (long)t * (2 * CSS);
-    const int jl = tid % CS, r0 = tid / CS;
-    const int j = Qb * CS + jl;
-    const int jc = min(j, N - 1);
-    float zpj[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) zpj[k] = a.Zp[3 * jc + k];
-    unsigned exm = 0u;
-#pragma unroll 4
-    for (int c = 0; c < CPT; ++c) {
-        const int il = r0 + c * RS, i = P * CS + il;
-        const bool vm = (i < N) && (j < N) && (P != Qb || il < jl);
-        if (!vm) continue;
-        const long pa = dense_pair_slot(i, j, a.nb, a.T), pb = pa + TT;
-        bool ex = false;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const float mA = a.X[k * S + pa] + a.U[k * S + pa];
-            const float mB = a.X[k * S + pb] + a.U[k * S + pb];
-            ex = ex || mA != a.Zp[3 * i + k] || mB != zpj[k];
-        }
-        if (ex) {
-            exm |= 1u << c;
-            atomicOr(&emask[il][jl >> 5], 1u << (jl & 31));
-        }
-    }
-    int total = 0;
-    int slot = block_offset(__popc(exm), wsum, total);
-    while (exm) {
-        const int c = __ffs(exm) - 1;
-        exm &= exm - 1u;
-        const int il = r0 + c * RS, i = P * CS + il;
-        const long pa = dense_pair_slot(i, j, a.nb, a.T), pb = pa + TT;
-        float mA[3], mB[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            mA[k] = a.X[k * S + pa] + a.U[k * S + pa];
-            mB[k] = a.X[k * S + pb] + a.U[k * S + pb];
-        }
-        L0[2 * slot] = make_float4(mA[0], mA[1], mA[2], __int_as_float(il * CS + jl));
-        L0[2 * slot + 1] = make_float4(mB[0], mB[1], mB[2], 0.0f);
-        ++slot;
-    }
-    if (tid == 0) a.cnt[t] = total;
-    if (tid < CS * W) a.Lmask[(long)t * CS * W + tid] = (&emask[0][0])[tid];
+    __shared__ ContactEnterShared<CS> sh;
+    contact_enter_tile<CS>(a, (int)blockIdx.x, sh);
 }
 
-// Leaves the contact state after iteration k (Z_prev = Z_k, Z_prev2 =
-// Z_{k-1}, list par = L_k, the other = L_{k-1}): every pair edge gets
-//   U_k = M_{k-1} - Z_k,  X_k = M_k - U_k
-// with M = Z of its circle off the lists: the steady X, U state.  256
-// threads per contact tile, as k_contact_enter.
+// Leaves the contact state (contact_leave_tile), one workgroup per contact tile.
 template <int CS>
 __global__ __launch_bounds__(256) void k_contact_leave(CArgs a)
 {
-    constexpr int CSS = CS * CS, W = CS / 32, RS = 256 / CS, CPT = CSS / 256;
-    __shared__ unsigned short map[2][CSS];
-    __shared__ unsigned mask[2][CS][W];
-    const int tid = threadIdx.x;
-    const int t = blockIdx.x;
-    const int N = a.N;
-    const long S = a.stride, TT = (long)a.T * a.T;
-    int P, Qb;
-    tri_tile(t, a.nbs, P, Qb);
-    const long lt[2] = {(long)a.par * a.nts + t, (long)(a.par ^ 1) * a.nts + t};
-    const float4* Ls[2] = {a.L + lt[0] * (2 * CSS), a.L + lt[1] * (2 * CSS)};
-    const int cn[2] = {min(a.cnt[lt[0]], CSS), min(a.cnt[lt[1]], CSS)};
-    if (tid < 2 * CS * W) (&mask[0][0][0])[tid] = 0u;
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-        for (int e = tid; e < cn[b]; e += 256) {
-            const int idx = __float_as_int(Ls[b][2 * e].w) & (CSS - 1);
-            map[b][idx] = (unsigned short)e;
-            atomicOr(&mask[b][idx / CS][(idx % CS) >> 5], 1u << (idx & 31));
-        }
-    __syncthreads();
-    const int jl = tid % CS, r0 = tid / CS;
-    const int j = Qb * CS + jl;
-#pragma unroll 2
-    for (int c = 0; c < CPT; ++c) {
-        const int il = r0 + c * RS, i = P * CS + il;
-        const bool vm = (i < N) && (j < N) && (P != Qb || il < jl);
-        if (!vm) continue;
-        const int ic = i;
-        const int idx = il * CS + jl;
-        const bool in_k = (mask[0][il][jl >> 5] >> (jl & 31)) & 1u;
-        const bool in_k1 = (mask[1][il][jl >> 5] >> (jl & 31)) & 1u;
-        float4 ek[2] = {}, ek1[2] = {};
-        if (in_k) {
-            ek[0] = Ls[0][2 * map[0][idx]];
-            ek[1] = Ls[0][2 * map[0][idx] + 1];
-        }
-        if (in_k1) {
-            ek1[0] = Ls[1][2 * map[1][idx]];
-            ek1[1] = Ls[1][2 * map[1][idx] + 1];
-        }
-        const long pa = dense_pair_slot(i, j, a.nb, a.T), pb = pa + TT;
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const int v = side == 0 ? ic : j;
-            const long q = side == 0 ? pa : pb;
-            const float mk[3] = {ek[side].x, ek[side].y, ek[side].z};
-            const float mk1[3] = {ek1[side].x, ek1[side].y, ek1[side].z};
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float zk = a.Zp[3 * v + k];
-                const float u = (in_k1 ? mk1[k] : a.Zpp[3 * v + k]) - zk;
-                a.U[k * S + q] = u;
-                a.X[k * S + q] = (in_k ? mk[k] : zk) - u;
-            }
-        }
-    }
+    __shared__ ContactLeaveShared<CS> sh;
+    contact_leave_tile<CS>(a, (int)blockIdx.x, sh);
 }
 
 // ------------------------------------------------------------------ learned rho in the contact state

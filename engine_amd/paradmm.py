@@ -65,7 +65,8 @@ class SolveResult(C.Structure):
 
 class BatchConfig(C.Structure):
     """paradmm_batch_config of paradmm_gpu.h"""
-    _fields_ = [("tile", C.c_int), ("blocks_per_cu", C.c_int), ("barrier", C.c_int), ("ms_per_iter", C.c_double)]
+    _fields_ = [("tile", C.c_int), ("blocks_per_cu", C.c_int), ("barrier", C.c_int), ("ms_per_iter", C.c_double),
+                ("contact_tile", C.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -163,6 +164,8 @@ def load():
         "paradmm_batch_set_rho_policy": (C.c_int, [V, V]),
         "paradmm_batch_set_rho_policy_mode": (C.c_int, [V, C.c_int, C.c_int]),
         "paradmm_batch_info": (C.c_int, [V, _IP, _IP, _IP, C.POINTER(C.c_long), _IP]),
+        "paradmm_batch_configure_contact": (C.c_int, [V, C.c_int, C.c_int, C.c_int]),
+        "paradmm_batch_contact_info": (C.c_int, [V, _IP, _IP, _IP, C.POINTER(C.c_long), C.POINTER(C.c_long), _DP]),
         "paradmm_rho_mlp_get_params": (C.c_int, [V, _FP, C.c_int]),
         "paradmm_rho_mlp_forward": (C.c_int, [V, _FP, C.c_int, _FP]),
         "paradmm_rho_mlp_gradient": (C.c_int, [V, _FP, _FP, C.c_int, _FP, _DP]),
@@ -514,6 +517,22 @@ class Batch:
 
     def configure(self, tile, blocks_per_cu=1, barrier=1):
         check(self.lib.paradmm_batch_configure(self.h, tile, blocks_per_cu, barrier), "paradmm_batch_configure")
+
+    def configure_contact(self, contact_tile, blocks_per_cu=1, barrier=1):
+        """Contact-state kernel of the plain fixed-rho batch: contact tile 32
+        or 64, or 0 to keep the per-edge kernel."""
+        check(self.lib.paradmm_batch_configure_contact(self.h, contact_tile, blocks_per_cu, barrier),
+              "paradmm_batch_configure_contact")
+
+    def contact_info(self):
+        t, b, gr = C.c_int(), C.c_int(), C.c_int()
+        nt, lp = C.c_long(), C.c_long()
+        ph = (C.c_double * 4)()
+        check(self.lib.paradmm_batch_contact_info(self.h, C.byref(t), C.byref(b), C.byref(gr), C.byref(nt),
+                                                  C.byref(lp), ph), "paradmm_batch_contact_info")
+        return {"contact_tile": t.value, "blocks_per_cu": b.value, "grid": gr.value, "contact_tiles": nt.value,
+                "listed_pairs": lp.value,
+                "phase_ms": {"after_upload": ph[0], "enter": ph[1], "iterations": ph[2], "leave": ph[3]}}
 
     def autotune(self, iters=20):
         chosen = BatchConfig()

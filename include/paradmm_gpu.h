@@ -262,6 +262,8 @@ typedef struct {
     int blocks_per_cu;   /* persistent workgroups per compute unit */
     int barrier;         /* grid barrier: 0 cooperative-groups sync, 1 two-level (bounded wait) */
     double ms_per_iter;  /* measured by the autotuner */
+    int contact_tile;    /* 0: the per-edge kernel; 32 or 64: the contact-state kernel with that contact
+                            tile side (blocks_per_cu and barrier are then its own) */
 } paradmm_batch_config;
 
 int paradmm_batch_create(paradmm_batch **out, graph *const *graphs, int count);
@@ -269,9 +271,30 @@ void paradmm_batch_destroy(paradmm_batch *b);
 /* Fixes the tile side, grid and barrier (default on first solve: tile 64,
  * 1 block/CU, two-level barrier). */
 int paradmm_batch_configure(paradmm_batch *b, int tile, int blocks_per_cu, int barrier);
-/* Times `iters` iterations of every candidate (tile 32/64 x blocks per CU up
- * to the kernel's occupancy x barrier kind) and configures the fastest;
- * *chosen gets it, tried[0..max_tried) the candidates measured. */
+/* The plain fixed-rho batch (alpha = 1 and one rho on every edge of each
+ * instance, no policy) runs in the contact state of the dense layout: per
+ * instance only the pairs whose prox is active keep per-edge values, and
+ * each instance's iterations equal the single-instance contact iterations.
+ * Either one cooperative launch iterates every instance (blocks_per_cu >=
+ * 1 workgroups per CU, one grid barrier per iteration) or, blocks_per_cu =
+ * 0, one launch per iteration does (the hardware keeps as many work items
+ * resident as fit; barrier ignored).  This fixes the contact tile side (32
+ * or 64; 0 keeps the per-edge kernel for every solve), the form and the
+ * barrier. */
+int paradmm_batch_configure_contact(paradmm_batch *b, int contact_tile, int blocks_per_cu, int barrier);
+/* The contact kernel's configuration (blocks_per_cu 0: one launch per
+ * iteration), its contact tiles over all instances, the pairs listed by
+ * every instance's last iteration of the last contact solve (synchronises),
+ * and phase_ms[4] of that solve: the after-upload pass, the entry, the
+ * remaining iterations, the exit (with the pending U update). */
+int paradmm_batch_contact_info(paradmm_batch *b, int *contact_tile, int *blocks_per_cu, int *grid,
+                               long *contact_tiles, long *listed_pairs, double *phase_ms);
+/* Times every candidate and configures the fastest; *chosen gets it,
+ * tried[0..max_tried) the candidates measured.  Where the contact state
+ * applies the candidates are its kernel's (contact tile 32/64 x blocks per
+ * CU x barrier kind, `iters` iterations from a fresh upload each); else the
+ * per-edge kernel's (tile 32/64 x blocks per CU up to the occupancy x
+ * barrier kind, `iters` iterations). */
 int paradmm_batch_autotune(paradmm_batch *b, int iters, paradmm_batch_config *chosen,
                            paradmm_batch_config *tried, int max_tried, int *ntried);
 /* Uploads every graph's host state (X, U, N, Z, RHOS, ALPHAS) and solves;

@@ -163,20 +163,37 @@ def test_batch_retires_instances_independently(monkeypatch):
         assert r1["converged"] == 1
 
 
-def test_batch_autotune_and_rerun():
+@pytest.mark.parametrize("contact", [False, True])
+def test_batch_autotune_and_rerun(monkeypatch, contact):
+    """The autotuner times the per-edge kernel's candidates (tile x blocks
+    per CU x barrier) or, for the plain fixed-rho batch, the contact
+    kernel's (contact tile x blocks per CU x barrier), and configures the
+    fastest; a solve after tuning starts from the graphs' state again."""
+    if not contact:
+        monkeypatch.setenv("PARADMM_CONTACT", "0")
     sizes = P.batch_sizes(12, 5, lo=16, hi=200)
     gs = [P.packing_square(n, 100 + k, rho=50.0) for k, n in enumerate(sizes)]
     b = P.Batch(gs)
     chosen, tried = b.autotune(10)
     print("autotune:", chosen, tried)
     assert chosen in tried and len(tried) >= 4
-    assert {t["tile"] for t in tried} == {32, 64} and {t["barrier"] for t in tried} == {0, 1}
-    # every candidate timed real iterations: none is faster than the
-    # chosen configuration's own fixed-iteration run by a large factor
-    _, ms_fixed = b.solve(10, tol=-1.0, tol_feas=-1.0, check_every=1000)
-    assert chosen["ms_per_iter"] > 0.3 * ms_fixed / 10
+    assert {t["barrier"] for t in tried} == {0, 1}
+    if contact:
+        assert {t["contact_tile"] for t in tried} == {32, 64} and chosen["contact_tile"] in (32, 64)
+        ci = b.contact_info()
+        assert ci["contact_tile"] == chosen["contact_tile"] and ci["blocks_per_cu"] == chosen["blocks_per_cu"]
+        _, ms_fixed = b.solve(10, tol=-1.0, tol_feas=-1.0, check_every=1000)
+        assert chosen["ms_per_iter"] > 0.3 * b.contact_info()["phase_ms"]["iterations"] / 9
+    else:
+        assert {t["tile"] for t in tried} == {32, 64} and all(t["contact_tile"] == 0 for t in tried)
+        # every candidate timed real iterations: none is faster than the
+        # chosen configuration's own fixed-iteration run by a large factor
+        _, ms_fixed = b.solve(10, tol=-1.0, tol_feas=-1.0, check_every=1000)
+        assert chosen["ms_per_iter"] > 0.3 * ms_fixed / 10
     info = b.info()
-    assert info["tile"] == chosen["tile"] and info["blocks_per_cu"] == chosen["blocks_per_cu"]
+    assert info["tile"] == chosen["tile"]
+    if not contact:
+        assert info["blocks_per_cu"] == chosen["blocks_per_cu"]
     # a solve after tuning starts from the graphs' state again
     r1, _ = b.solve(30, tol=-1.0, tol_feas=-1.0, check_every=1000)
     b.sync_to_host()
@@ -184,7 +201,10 @@ def test_batch_autotune_and_rerun():
     # the graphs now hold the solved state: rebuild fresh ones for the rerun
     gs2 = [P.packing_square(n, 100 + k, rho=50.0) for k, n in enumerate(sizes)]
     b2 = P.Batch(gs2)
-    b2.configure(chosen["tile"], chosen["blocks_per_cu"])
+    if contact:  # the contact tiling and grid do not change the bits
+        b2.configure(chosen["tile"], 1)
+    else:
+        b2.configure(chosen["tile"], chosen["blocks_per_cu"])
     r2, _ = b2.solve(30, tol=-1.0, tol_feas=-1.0, check_every=1000)
     b2.sync_to_host()
     for a, g in zip(z1, gs2):
@@ -210,10 +230,11 @@ def test_batch_rejects_other_graphs():
         b.configure(32, 1)
 
 
-def test_batch_barrier_kinds_agree():
+def test_batch_barrier_kinds_agree(monkeypatch):
     """The cooperative-groups grid sync and the two-level barrier give the
     same iterations bit for bit (the barrier only orders the phases), with
-    one and with two workgroups per compute unit."""
+    one and with two workgroups per compute unit (the per-edge kernel)."""
+    monkeypatch.setenv("PARADMM_CONTACT", "0")
     sizes = [16, 40, 100, 130]
     zs = []
     for bpc, barrier in ((1, 0), (1, 1), (2, 1)):
@@ -420,3 +441,99 @@ def test_batch_retires_nonfinite_instance():
     for k in (0, 2):
         assert res[k]["iterations"] == 400 and np.isfinite(res[k]["primal_rms"])
     b.close()
+
+
+# ---- the contact state on the batch (the plain fixed-rho batch's default)
+
+
+@pytest.mark.parametrize("iters", [3, 120])
+def test_batch_contact_equals_single_instance_contact(monkeypatch, iters):
+    """Each instance of the contact batch equals the single-instance contact
+    iterations bit for bit -- Z, and X and U after both leave the state: the
+    after-upload pass is the same tile code (dense tile side 64, at most two
+    blocks per instance, N <= 128) and every later iteration sums integers.
+    120 iterations cross two checks (runs end at 50 and 100); the residuals
+    of the last iteration are the same per-circle sums in another order."""
+    monkeypatch.setenv("PARADMM_DENSE_TILE", "64")
+    monkeypatch.delenv("PARADMM_CONTACT", raising=False)
+    sizes = [16, 70, 100, 128]
+    gs = _instances(sizes, 10.0)
+    b = P.Batch(gs)
+    b.configure(64, 1)
+    res, ms = b.solve(iters, tol=-1.0, tol_feas=-1.0, check_every=50)
+    b.sync_to_host()
+    ci = b.contact_info()
+    print(f"{iters} iterations: {ci}")
+    assert ci["contact_tile"] == 32 and ci["phase_ms"]["iterations"] > 0.0
+    for k, n in enumerate(sizes):
+        s = P.packing_square(n, 11 + k, rho=10.0)
+        s.iterate(iters)
+        assert s.contact_info()[0]  # the single instance ran in the contact state
+        p2, d2 = s.residuals()
+        s.sync_to_host()
+        for name in ("Z", "X", "U"):
+            assert np.array_equal(gs[k].array(name), s.array(name)), (n, name)
+        e3 = 3.0 * s.nedge
+        assert res[k]["iterations"] == iters
+        assert res[k]["primal_rms"] == pytest.approx(np.sqrt(p2 / e3), rel=1e-9, abs=1e-12)
+        assert res[k]["dual_rms"] == pytest.approx(np.sqrt(d2 / e3), rel=1e-9, abs=1e-12)
+        s.close()
+    b.close()
+
+
+def test_batch_contact_tilings_and_grids_agree():
+    """Contact tile side 32 or 64, one persistent launch with one to four
+    workgroups per CU and either barrier, or one launch per iteration
+    (blocks_per_cu 0): the same iterations bit for bit (integer circle
+    sums), checks included."""
+    sizes = [16, 40, 100, 300, 512]
+    zs = []
+    for cs, bpc, bar in ((32, 1, 0), (32, 4, 1), (64, 1, 1), (64, 2, 0), (32, 0, 0), (64, 0, 0)):
+        gs = _instances(sizes, 20.0)
+        b = P.Batch(gs)
+        b.configure(32, 1)
+        b.configure_contact(cs, bpc, bar)
+        res, ms = b.solve(60, tol=-1.0, tol_feas=-1.0, check_every=25)
+        b.sync_to_host()
+        zs.append([g.array("Z").copy() for g in gs])
+        print(f"contact tile {cs}, {bpc} per CU, barrier {bar}: {ms:.2f} ms for 60 iterations, {b.contact_info()}")
+        b.close()
+    for other in zs[1:]:
+        for a, c in zip(zs[0], other):
+            assert np.array_equal(a, c)
+
+
+def test_batch_contact_against_per_edge_batch(monkeypatch):
+    """The contact batch and the per-edge batch (PARADMM_CONTACT=0) compute
+    the same iteration with other roundings.  On the first 24 instances of
+    the 256-instance workload (rho = clip(N/2, 35, 300)) the two agree to
+    1e-5 after 3 iterations; by 30 iterations both have left the FP64
+    trajectory by up to a few 1e-3 on some instances (the early iterations
+    amplify any rounding), and the oracle must sit as close to the contact
+    batch as to the per-edge one."""
+    sizes = P.batch_sizes(256, 0)[:24]
+    for iters in (3, 30):
+        zs = {}
+        for mode in ("1", "0"):
+            monkeypatch.setenv("PARADMM_CONTACT", mode)
+            gs = [P.packing_square(n, k, rho=_batch_rho(n)) for k, n in enumerate(sizes)]
+            b = P.Batch(gs)
+            b.configure(32, 1)
+            b.solve(iters, tol=-1.0, tol_feas=-1.0, check_every=1000)
+            b.sync_to_host()
+            zs[mode] = [g.array("Z").copy() for g in gs]
+            b.close()
+        worst, oc, op = 0.0, [], []
+        for k, n in enumerate(sizes):
+            worst = max(worst, max_rel(zs["1"][k], zs["0"][k]))
+            o = po.Graph.square(n, k, _batch_rho(n))
+            o.iterate(iters)
+            oc.append(max_rel(zs["1"][k], o.array("Z")))
+            op.append(max_rel(zs["0"][k], o.array("Z")))
+            o.close()
+        print(f"{iters} iterations: contact vs per-edge batch {worst:.1e}; oracle error contact max {max(oc):.1e} "
+              f"median {np.median(oc):.1e}, per-edge max {max(op):.1e} median {np.median(op):.1e}")
+        if iters == 3:
+            assert worst < 1e-5 and max(oc) < 1e-5
+        assert np.median(oc) <= 3.0 * np.median(op) + 1e-7
+        assert max(oc) <= 3.0 * max(op) + 1e-7
