@@ -44,6 +44,7 @@
 #include "mlp_device.h"
 #include "pairtile.h"
 #include "contact_device.h"
+#include "contact_policy.h"
 #include "paradmm_gpu.h"
 
 #define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hip_fail(e_, #x, __FILE__, __LINE__); } while (0)
@@ -489,6 +490,7 @@ struct BCArgs {
     int T;            // dense tile side (the box and radius slots)
     long stride;
     float *X, *U, *Z, *Zprev, *Zpp, *Zb;
+    float* W;                 // per instance [2][N]: sums of rho and rho^2 per circle (per-edge rho, after a policy)
     const float *rho, *aexp, *inst_rho;
     const BInst* inst;
     const BCInst* cinst;
@@ -527,7 +529,7 @@ __device__ __forceinline__ CArgs cview(const BCArgs& a, const BInst& in, const B
     v.aexp = a.aexp + in.c0;
     v.uni = a.inst_rho + 2 * i;
     v.rho = a.rho + in.s0;
-    v.W = nullptr;
+    v.W = a.W ? a.W + 2L * in.c0 : nullptr;
     v.par = par;
     v.L = a.L + 2 * ci.l0 * (2 * CSS);
     v.Lmask = a.Lmask + 2 * ci.l0 * (CS * (CS / 32));
@@ -801,7 +803,8 @@ __device__ __forceinline__ PolicySlot policy_slot(const int q, const int I, cons
 // retired instances return at once.
 
 // items: (instance, local item), every instance's blocks then its tiles
-template <int CS, bool RES>
+// RE: rho per edge (after a policy application)
+template <int CS, bool RES, bool RE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_contact_step(
     BCArgs a, const int2* items, int par, int j)
 {
@@ -812,12 +815,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const BCInst ci = a.cinst[e.x];
     CArgs v = cview<CS>(a, in, ci, e.x, par);
     cview_run(v, a, in, j);
-    if (e.y < ci.nbs) contact_unary<CS, RES, false>(v, e.y);
-    else contact_tile_body<CS, RES, false>(v, e.y - ci.nbs, sh);
+    if (e.y < ci.nbs) contact_unary<CS, RES, RE>(v, e.y);
+    else contact_tile_body<CS, RES, RE>(v, e.y - ci.nbs, sh);
 }
 
 // the end of a run: every circle of an active instance (circ_inst: its instance)
-template <int CS, bool RES>
+template <int CS, bool RES, bool RE>
 __global__ __launch_bounds__(256) void k_batch_contact_zreduce(BCArgs a, const int* circ_inst, int ncirc, int j)
 {
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -827,7 +830,91 @@ __global__ __launch_bounds__(256) void k_batch_contact_zreduce(BCArgs a, const i
     const BInst in = a.inst[i];
     CArgs f = cview<CS>(a, in, a.cinst[i], i, 0);
     cview_end(f, a, in, j);
-    contact_zreduce_circle<RES, false>(f, g - in.c0);
+    contact_zreduce_circle<RES, RE>(f, g - in.c0);
+}
+
+// ---- the learned rho policy inside the batch's contact state: each active
+// instance takes the application of contact_policy.h (modes 2 and 3, as
+// paradmm_gpu_solve applies it to one graph in the contact state), on its
+// own regions of the policy buffers
+
+struct BCPol {
+    const float* params;  // the network
+    float frac, raise_log;
+    int exceptions;       // rho already per edge (an earlier application)
+    int average;
+    float* base;          // [circles]
+    float* yc;            // per instance [3][N]
+    int* rows;            // [circles]
+    long long* wacc;      // per instance [2][N]
+    long long* ysum;      // [instances]
+    double* hist;         // [instances][6]
+    float* gl;            // [instances][2]
+};
+
+template <int CS>
+__device__ __forceinline__ CPolArgs cpol_view(const BCArgs& a, const BCPol& pp, const BInst& in, const int i,
+                                              const int par)
+{
+    CPolArgs p;
+    p.c = cview<CS>(a, in, a.cinst[i], i, par);
+    p.params = pp.params;
+    p.frac = pp.frac;
+    p.raise_log = pp.raise_log;
+    p.exceptions = pp.exceptions;
+    p.average = pp.average;
+    p.base = pp.base + in.c0;
+    p.yc = pp.yc + 3L * in.c0;
+    p.rows = pp.rows + in.c0;
+    p.wacc = pp.wacc + 2L * in.c0;
+    p.ysum = pp.ysum + i;
+    p.hist = pp.hist + 6L * i;
+    p.gl = pp.gl + 2L * i;
+    return p;
+}
+
+// one workgroup per contact tile of an active instance
+template <int CS, bool APPLY>
+__global__ __launch_bounds__(256) void k_batch_cpol_tile(BCArgs a, BCPol pp, const int2* ctile, int par)
+{
+    __shared__ CPolShared<CS> sh;
+    const int2 e = ctile[blockIdx.x];
+    if (!__hip_atomic_load(&a.st[e.x].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const CPolArgs p = cpol_view<CS>(a, pp, a.inst[e.x], e.x, par);
+    cpol_tile_body<CS, APPLY>(p, e.y, sh);
+}
+
+// one workgroup per active instance: its rho
+template <int CS>
+__global__ __launch_bounds__(256) void k_batch_cpol_final(BCArgs a, BCPol pp, int par)
+{
+    __shared__ long long sh[256];
+    const int i = blockIdx.x;
+    if (!__hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const BInst in = a.inst[i];
+    const CPolArgs p = cpol_view<CS>(a, pp, in, i, par);
+    cpol_final_body(p, (long)in.E, sh);
+}
+
+// one thread per circle of an active instance
+template <int CS>
+__global__ __launch_bounds__(256) void k_batch_cpol_zprev(BCArgs a, BCPol pp, const int* circ_inst, int ncirc, int par)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= ncirc) return;
+    const int i = circ_inst[g];
+    if (!__hip_atomic_load(&a.st[i].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const BInst in = a.inst[i];
+    const CPolArgs p = cpol_view<CS>(a, pp, in, i, par);
+    cpol_zprev_circle(p, g - in.c0);
+}
+
+// every circle's base rho: its instance's rho (one rho per instance at upload)
+__global__ __launch_bounds__(256) void k_batch_cpol_base(const float* inst_rho, const int* circ_inst, int ncirc,
+                                                         float* base)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g < ncirc) base[g] = inst_rho[2 * circ_inst[g]];
 }
 
 // quality partials: one wavefront per contact tile (ctile: instance, tile)
@@ -1096,6 +1183,11 @@ struct paradmm_batch {
     int2* citems_d = nullptr;   // every instance's blocks, then every instance's tiles (one launch per iteration)
     int nitems = 0;
     int* circ_inst_d = nullptr; // [ncirc] instance of each circle
+    // the policy inside the contact state
+    float *cW = nullptr, *cbase = nullptr, *cyc = nullptr, *cgl = nullptr;
+    int* crows = nullptr;
+    long long *cwacc = nullptr, *cysum = nullptr;
+    double* chist = nullptr;
     float4* cL = nullptr;
     unsigned* cLmask = nullptr;
     int* ccnt = nullptr;
@@ -1387,15 +1479,20 @@ static int contact_occupancy(const paradmm_batch* b, int CS, int* per_cu)
     return PARADMM_OK;
 }
 
-// The plain fixed-rho batch runs in the contact state: alpha = 1 and one rho
-// on every edge of each instance (as uploaded), no policy, at least two
-// iterations; PARADMM_CONTACT=0 or paradmm_batch_configure_contact(b, 0, ..)
-// keep the per-edge kernel.
+// The batch runs in the contact state when alpha = 1 and one rho is on every
+// edge of each instance (as uploaded), for at least two iterations, with or
+// without a rho policy; PARADMM_CONTACT=0 or paradmm_batch_configure_contact
+// (b, 0, ..) keep the per-edge kernel (PARADMM_CONTACT_POLICY=0: with a
+// policy).
 static bool batch_contact_applies(const paradmm_batch* b, const paradmm_solve_opts* o)
 {
-    if (b->contact_off || o->policy_every > 0 || !b->alpha1 || o->max_iters < 2) return false;
+    if (b->contact_off || !b->alpha1 || o->max_iters < 2) return false;
     const char* e = getenv("PARADMM_CONTACT");
     if (e && e[0] == '0') return false;
+    if (o->policy_every > 0) {  // the policy inside the contact state (PARADMM_CONTACT_POLICY=0: per edge)
+        const char* p = getenv("PARADMM_CONTACT_POLICY");
+        if (!b->policy || (p && p[0] == '0')) return false;
+    }
     for (int i = 0; i < b->ninst; ++i)
         if (!b->uni0[i]) return false;
     return true;
@@ -1459,6 +1556,14 @@ static int batch_contact_alloc(paradmm_batch* b, int CS)
     std::vector<int> ci(nc);
     for (int i = 0; i < b->ninst; ++i)
         for (int c = 0; c < b->inst[i].N; ++c) ci[(size_t)b->inst[i].c0 + c] = i;
+    RC(cmalloc(b, (void**)&b->cW, sizeof(float) * 2 * nc));
+    RC(cmalloc(b, (void**)&b->cbase, sizeof(float) * nc));
+    RC(cmalloc(b, (void**)&b->cyc, sizeof(float) * 3 * nc));
+    RC(cmalloc(b, (void**)&b->crows, sizeof(int) * nc));
+    RC(cmalloc(b, (void**)&b->cwacc, sizeof(long long) * 2 * nc));
+    RC(cmalloc(b, (void**)&b->cysum, sizeof(long long) * (size_t)b->ninst));
+    RC(cmalloc(b, (void**)&b->chist, sizeof(double) * 6 * (size_t)b->ninst));
+    RC(cmalloc(b, (void**)&b->cgl, sizeof(float) * 2 * (size_t)b->ninst));
     RC(cmalloc(b, (void**)&b->citems_d, sizeof(int2) * items.size()));
     RC(cmalloc(b, (void**)&b->circ_inst_d, sizeof(int) * nc));
     CK(hipMemcpy(b->citems_d, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
@@ -1489,6 +1594,7 @@ static BCArgs batch_contact_args(paradmm_batch* b, const paradmm_solve_opts* o, 
     a.Zprev = b->Zprev;
     a.Zpp = b->cZpp;
     a.Zb = b->cZb;
+    a.W = b->cW;
     a.rho = b->rho;
     a.aexp = b->aexp_d;
     a.inst_rho = b->inst_rho;
@@ -1530,26 +1636,68 @@ static int contact_grid(paradmm_batch* b, int CS, int bpc, int* grid)
 
 // The contact iterations 1 .. max_iters with one launch per iteration: runs
 // of step launches up to each check, then the run's end, the quality and
-// the check kernels; the host stops once no instance is active.
+// the check kernels; with a rho policy (check_every == policy_every) every
+// instance still active after a check that more iterations follow takes
+// the policy inside the contact state, and the later iterations read rho per
+// edge.  The host stops once no instance is active.
 template <int CS>
 static int batch_contact_steps(paradmm_batch* b, const paradmm_solve_opts* o, const BCArgs& a)
 {
     const unsigned ni = (unsigned)b->nitems, zg = (unsigned)((b->ncirc + 255) / 256);
     std::vector<BState> st(b->ninst);
+    const bool policy = o->policy_every > 0 && b->policy;
+    BCPol pp{};
+    if (policy) {
+        pp.params = mlp_device_params(b->policy);
+        pp.raise_log = b->policy_mode == 3 ? 0.5f : __builtin_inff();
+        pp.average = b->policy_average ? 1 : 0;
+        pp.base = b->cbase;
+        pp.yc = b->cyc;
+        pp.rows = b->crows;
+        pp.wacc = b->cwacc;
+        pp.ysum = b->cysum;
+        pp.hist = b->chist;
+        pp.gl = b->cgl;
+        CK(hipMemsetAsync(b->chist, 0, sizeof(double) * 6 * (size_t)b->ninst, b->stream));
+        k_batch_cpol_base<<<zg, 256, 0, b->stream>>>(b->inst_rho, b->circ_inst_d, b->ncirc, b->cbase);
+        CK(hipGetLastError());
+    }
+    bool re = false;  // rho per edge: after the first policy application
     int par = 0, it = 1;
     while (it < o->max_iters) {
         const int end = std::min(o->max_iters, (it / o->check_every + 1) * o->check_every);
         for (int j = 0; it + j < end; ++j) {
-            if (it + j + 1 == end) k_batch_contact_step<CS, true><<<ni, 64, 0, b->stream>>>(a, b->citems_d, par, j);
-            else k_batch_contact_step<CS, false><<<ni, 64, 0, b->stream>>>(a, b->citems_d, par, j);
+            const bool res = it + j + 1 == end;
+            if (re) {
+                if (res) k_batch_contact_step<CS, true, true><<<ni, 64, 0, b->stream>>>(a, b->citems_d, par, j);
+                else k_batch_contact_step<CS, false, true><<<ni, 64, 0, b->stream>>>(a, b->citems_d, par, j);
+            } else {
+                if (res) k_batch_contact_step<CS, true, false><<<ni, 64, 0, b->stream>>>(a, b->citems_d, par, j);
+                else k_batch_contact_step<CS, false, false><<<ni, 64, 0, b->stream>>>(a, b->citems_d, par, j);
+            }
             par ^= 1;
         }
-        k_batch_contact_zreduce<CS, true><<<zg, 256, 0, b->stream>>>(a, b->circ_inst_d, b->ncirc, end - it - 1);
+        if (re) k_batch_contact_zreduce<CS, true, true><<<zg, 256, 0, b->stream>>>(a, b->circ_inst_d, b->ncirc, end - it - 1);
+        else k_batch_contact_zreduce<CS, true, false><<<zg, 256, 0, b->stream>>>(a, b->circ_inst_d, b->ncirc, end - it - 1);
         k_batch_contact_quality<CS><<<(unsigned)b->cts, 64, 0, b->stream>>>(a, b->ctile_d);
         k_batch_contact_check<<<(unsigned)b->ninst, 64, 0, b->stream>>>(a, end, end >= o->max_iters ? 1 : 0, par);
         CK(hipGetLastError());
         it = end;
         if (it >= o->max_iters) break;
+        if (policy && it % o->policy_every == 0) {
+            pp.frac = (float)it / (float)o->max_iters;
+            pp.exceptions = re ? 1 : 0;
+            CK(hipMemsetAsync(b->crows, 0, sizeof(int) * (size_t)b->ncirc, b->stream));
+            CK(hipMemsetAsync(b->cwacc, 0, sizeof(long long) * 2 * (size_t)b->ncirc, b->stream));
+            CK(hipMemsetAsync(b->cysum, 0, sizeof(long long) * (size_t)b->ninst, b->stream));
+            const unsigned nt = (unsigned)b->cts;
+            k_batch_cpol_tile<CS, false><<<nt, 256, 0, b->stream>>>(a, pp, b->ctile_d, par);
+            k_batch_cpol_final<CS><<<(unsigned)b->ninst, 256, 0, b->stream>>>(a, pp, par);
+            k_batch_cpol_tile<CS, true><<<nt, 256, 0, b->stream>>>(a, pp, b->ctile_d, par);
+            k_batch_cpol_zprev<CS><<<zg, 256, 0, b->stream>>>(a, pp, b->circ_inst_d, b->ncirc, par);
+            CK(hipGetLastError());
+            re = true;
+        }
         CK(hipMemcpyAsync(st.data(), b->st, sizeof(BState) * st.size(), hipMemcpyDeviceToHost, b->stream));
         CK(hipStreamSynchronize(b->stream));
         bool any = false;
@@ -1585,7 +1733,7 @@ static int batch_contact_solve(paradmm_batch* b, const paradmm_solve_opts* o, fl
     CK(hipEventRecord(b->ev1, b->stream));
     CK(hipEventSynchronize(b->ev1));
     CK(hipEventElapsedTime(&me, b->ev0, b->ev1));
-    if (b->cbpc == 0) {  // one launch per iteration
+    if (b->cbpc == 0 || o->policy_every > 0) {  // one launch per iteration (always with a policy)
         CK(hipEventRecord(b->ev0, b->stream));
         RC(b->ccs == 64 ? batch_contact_steps<64>(b, o, a) : batch_contact_steps<32>(b, o, a));
         CK(hipEventRecord(b->ev1, b->stream));
