@@ -233,11 +233,26 @@ def batch_rho(n):
     return float(min(300.0, max(35.0, 0.5 * n)))
 
 
+def batch_alg_bytes(sizes, cs, listed):
+    """contact_alg_bytes summed over the instances of a batch: the list term
+    over the batch's listed pairs, the tile-mask and per-circle terms per
+    instance."""
+    tot = 64 * listed
+    for n in sizes:
+        nbs = (n + cs - 1) // cs
+        tot += nbs * (nbs + 1) // 2 * cs * cs // 4 + n * (36 + 12 + 64 + 24) + 96 * n
+    return tot
+
+
 def batch_bench(P, count, seed, iters, cap):
     """The batched workload: `count` unit-square packings with N ~ U{16..512}
-    drawn from the batch seed, one persistent launch.  Returns the
-    throughput of `iters` iterations without retirement and the
-    time-to-tolerance solve with per-instance retirement."""
+    drawn from the batch seed.  The autotuner picks the form (the contact
+    state with one launch per iteration, or one cooperative launch) and the
+    contact tile side.  Returns the throughput of `iters` iterations without
+    retirement (whole solve, and the iterations alone with the contact
+    roofline on the batch's own algorithmic bytes), the time-to-tolerance
+    solve with per-instance retirement, and the learned policy on the same
+    batch against fixed rho = 1."""
     t0 = time.time()
     sizes = P.batch_sizes(count, seed)
     gs = [P.packing_square(n, 1000 * seed + k, rho=batch_rho(n), alpha=1.0, a_exp=1.0) for k, n in enumerate(sizes)]
@@ -249,7 +264,21 @@ def batch_bench(P, count, seed, iters, cap):
           file=sys.stderr, flush=True)
     _, ms = b.solve(iters, tol=-1.0, tol_feas=-1.0, check_every=iters + 1)
     thr = E * iters / (ms * 1e-3)
-    print(f"# batch throughput: {ms / iters:.4f} ms/iter, {thr:.4e} edge-updates/s", file=sys.stderr, flush=True)
+    ci = b.contact_info()
+    it_ms = ci["phase_ms"]["iterations"]
+    per_iter = it_ms / (iters - 1) if chosen.get("contact_tile") else ms / iters
+    thr_it = E / (per_iter * 1e-3)
+    print(f"# batch throughput: whole solve {ms / iters:.4f} ms/iter, {thr:.4e} edge-updates/s; iterations alone "
+          f"{per_iter:.4f} ms/iter, {thr_it:.4e} edge-updates/s; {ci}", file=sys.stderr, flush=True)
+    roof = None
+    if chosen.get("contact_tile"):
+        alg = batch_alg_bytes(sizes, chosen["contact_tile"], ci["listed_pairs"])
+        t = per_iter * 1e-3
+        roof = {"bound": "hbm", "kernel": "batch_contact_step", "achieved": alg / t / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": alg / t / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_iteration": alg,
+                "listed_pairs": ci["listed_pairs"], "ms_per_iteration": per_iter,
+                "edge_update_unit_equivalent_gbs": EDGE_UPDATE_BYTES * E / t / 1e9,
+                "limiter": "latency: more work items than fit resident at once, each waiting on its first loads"}
     res, ms2 = b.solve(cap, tol=1e-4, tol_feas=1e-4, check_every=50)
     conv = [r for r in res if r["converged"]]
     done_edge_iters = sum(r["iterations"] * g.nedge for r, g in zip(res, gs))
@@ -267,7 +296,9 @@ def batch_bench(P, count, seed, iters, cap):
     for name, rho0, pol in (("fixed_rho1", 1.0, None), ("learned", rt.LEARNED_START_RHO, rt.load_policy())):
         gs = [P.packing_square(n, 1000 * seed + k, rho=rho0, alpha=1.0, a_exp=1.0) for k, n in enumerate(sizes)]
         b = P.Batch(gs)
-        b.configure(info["tile"], info["blocks_per_cu"])
+        b.configure(info["tile"], 1)
+        if chosen.get("contact_tile"):
+            b.configure_contact(chosen["contact_tile"], chosen["blocks_per_cu"], chosen["barrier"])
         if pol is not None:
             b.set_rho_policy(pol)
             b.set_rho_policy_mode(rt.DEFAULT_MODE, rt.DEFAULT_AVERAGE)
@@ -284,11 +315,12 @@ def batch_bench(P, count, seed, iters, cap):
             g.close()
     return {
         "workload": f"{count} unit-square packings, N ~ U{{16..512}} from batch seed {seed}, rho = clip(N/2, 35, 300), "
-                    f"alpha 1, one persistent launch",
-        "instances": count, "edges": E, "tile": info["tile"], "blocks_per_cu": info["blocks_per_cu"],
+                    f"alpha 1",
+        "instances": count, "edges": E, "tile": info["tile"], "chosen": chosen,
         "autotune_tried": tried,
         "iters": iters, "ms_per_iter": ms / iters, "edge_updates_per_s": thr,
-        "roofline_frac_72B": thr * EDGE_UPDATE_BYTES / 1e9 / HBM_PEAK_GBS,
+        "iterations_alone": {"ms_per_iter": per_iter, "edge_updates_per_s": thr_it, "phase_ms": ci["phase_ms"]},
+        "roofline": roof,
         "rho_policy_vs_fixed1": learned,
         "to_tol": {"cap": cap, "check_every": 50, "converged": len(conv), "launch_ms": ms2,
                    "edge_updates_per_s": done_edge_iters / (ms2 * 1e-3),
@@ -375,7 +407,7 @@ def main():
     ap.add_argument("--no-n1000", action="store_true", help="skip the N = 1000 leg")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batched workload (0: skip)")
     ap.add_argument("--batch-seed", type=int, default=0)
-    ap.add_argument("--batch-iters", type=int, default=200)
+    ap.add_argument("--batch-iters", type=int, default=1000)
     ap.add_argument("--batch-cap", type=int, default=20000)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-fixed time to tolerance")
     ap.add_argument("--host-loop-iters", type=int, default=0,

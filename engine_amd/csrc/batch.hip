@@ -491,6 +491,7 @@ struct BCArgs {
     long stride;
     float *X, *U, *Z, *Zprev, *Zpp, *Zb;
     float* W;                 // per instance [2][N]: sums of rho and rho^2 per circle (per-edge rho, after a policy)
+    float* Xu;                // per instance [CUNARY][N]: box and radius edges inside the contact state
     const float *rho, *aexp, *inst_rho;
     const BInst* inst;
     const BCInst* cinst;
@@ -541,6 +542,7 @@ __device__ __forceinline__ CArgs cview(const BCArgs& a, const BInst& in, const B
     v.Rpc = a.Rpc + in.c0;
     v.Rd = a.Rd + in.c0;
     v.flag = a.flags + i;
+    v.Xu = a.Xu + (long)CUNARY * in.c0;
     return v;
 }
 
@@ -1184,7 +1186,7 @@ struct paradmm_batch {
     int nitems = 0;
     int* circ_inst_d = nullptr; // [ncirc] instance of each circle
     // the policy inside the contact state
-    float *cW = nullptr, *cbase = nullptr, *cyc = nullptr, *cgl = nullptr;
+    float *cW = nullptr, *cbase = nullptr, *cyc = nullptr, *cgl = nullptr, *cXu = nullptr;
     int* crows = nullptr;
     long long *cwacc = nullptr, *cysum = nullptr;
     double* chist = nullptr;
@@ -1557,6 +1559,7 @@ static int batch_contact_alloc(paradmm_batch* b, int CS)
     for (int i = 0; i < b->ninst; ++i)
         for (int c = 0; c < b->inst[i].N; ++c) ci[(size_t)b->inst[i].c0 + c] = i;
     RC(cmalloc(b, (void**)&b->cW, sizeof(float) * 2 * nc));
+    RC(cmalloc(b, (void**)&b->cXu, sizeof(float) * CUNARY * nc));
     RC(cmalloc(b, (void**)&b->cbase, sizeof(float) * nc));
     RC(cmalloc(b, (void**)&b->cyc, sizeof(float) * 3 * nc));
     RC(cmalloc(b, (void**)&b->crows, sizeof(int) * nc));
@@ -1595,6 +1598,7 @@ static BCArgs batch_contact_args(paradmm_batch* b, const paradmm_solve_opts* o, 
     a.Zpp = b->cZpp;
     a.Zb = b->cZb;
     a.W = b->cW;
+    a.Xu = b->cXu;
     a.rho = b->rho;
     a.aexp = b->aexp_d;
     a.inst_rho = b->inst_rho;

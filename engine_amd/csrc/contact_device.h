@@ -46,6 +46,14 @@ constexpr int CQ = 8;         // per-circle sums: a (3), g (3), |g|^2 (multiple 
 constexpr double CFIX_INV = 1.0 / 2147483648.0;  // 2^-31
 constexpr double CFIX2_INV = CFIX_INV * CFIX_INV; // 2^-62
 
+// Inside the contact state the box and radius edges of circle c live in a
+// compact per-circle array, component f at Xu[f N + c]: f = 0..2 the box
+// edge's X, 3..5 its U (one update behind), 6..8 and 9..11 the radius
+// edge's, 12 and 13 their rho.  (In the dense layout they sit on the
+// diagonals of the dense tiles, T + 1 floats apart: one cache line per
+// component and circle.)  The entry copies them in, the exit back.
+constexpr int CUNARY = 14;
+
 struct CArgs {
     int N;
     int nbs, nts;          // contact tiling: blocks of CS circles, tiles
@@ -73,6 +81,7 @@ struct CArgs {
     double* Rpc;
     double* Rd;
     unsigned* flag;        // set when a circle sum could not take a value
+    float* Xu;             // [CUNARY][N] the box and radius edges' state inside the contact state
 };
 
 // v rounded to the nearest multiple of 2^-31, in units of 2^-31:
@@ -352,7 +361,6 @@ template <int CS, bool RES, bool RE>
 __device__ __forceinline__ void contact_unary(const CArgs& a, int P)
 {
     constexpr int Q = RES ? CQ : 3;
-    const long S = a.stride, TT = (long)a.T * a.T;
     for (int e = (int)(threadIdx.x & 63); e < CS; e += 64) {
         const int cc = P * CS + e;
         if (cc >= a.N) break;
@@ -364,7 +372,7 @@ __device__ __forceinline__ void contact_unary(const CArgs& a, int P)
         if (a.Acl)
 #pragma unroll
             for (int q = 0; q < CQ; ++q) a.Acl[(long)q * a.N + cc] = 0ull;
-        const long p0 = dense_unary_slot(cc, a.nb, a.T);
+        const long N = a.N;
         long long s[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) s[q] = 0;
@@ -372,13 +380,13 @@ __device__ __forceinline__ void contact_unary(const CArgs& a, int P)
         unsigned bad = 0u;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
-            const long q = p0 + side * TT;
-            const float r = RE ? a.rho[q] : a.uni[0];
+            float* xu = a.Xu + 6 * side * N + cc;
+            const float r = RE ? a.Xu[(12 + side) * N + cc] : a.uni[0];
             float x[3], u[3], nn[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                x[k] = a.X[k * S + q];
-                u[k] = (x[k] + a.U[k * S + q]) - z[k];
+                x[k] = xu[k * N];
+                u[k] = (x[k] + xu[(3 + k) * N]) - z[k];
                 nn[k] = z[k] - u[k];
             }
             if (side == 0) {
@@ -390,8 +398,8 @@ __device__ __forceinline__ void contact_unary(const CArgs& a, int P)
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                a.X[k * S + q] = x[k];
-                a.U[k * S + q] = u[k];
+                xu[k * N] = x[k];
+                xu[(3 + k) * N] = u[k];
                 s[k] += cfix(RE ? r * ((x[k] + u[k]) - z[k]) : (x[k] + u[k]) - z[k], bad);
                 if constexpr (RES) {
                     const float g = (x[k] - z[k]) - cz[k];
@@ -670,6 +678,23 @@ __device__ __forceinline__ void contact_enter_tile(const CArgs& a, const int t, 
     }
     if (tid == 0) a.cnt[t] = total;
     if (tid < CS * W) a.Lmask[(long)t * CS * W + tid] = (&emask[0][0])[tid];
+    // a diagonal tile: its circles' box and radius edges into the compact array
+    if (P == Qb && tid < CS) {
+        const int c = P * CS + tid;
+        if (c < N) {
+            const long q0 = dense_unary_slot(c, a.nb, a.T);
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const long q = q0 + side * TT;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    a.Xu[(6 * side + k) * (long)N + c] = a.X[k * S + q];
+                    a.Xu[(6 * side + 3 + k) * (long)N + c] = a.U[k * S + q];
+                }
+                a.Xu[(12 + side) * (long)N + c] = a.rho[q];
+            }
+        }
+    }
 }
 
 
@@ -741,6 +766,22 @@ __device__ __forceinline__ void contact_leave_tile(const CArgs& a, const int t, 
                 const float u = (in_k1 ? mk1[k] : a.Zpp[3 * v + k]) - zk;
                 a.U[k * S + q] = u;
                 a.X[k * S + q] = (in_k ? mk[k] : zk) - u;
+            }
+        }
+    }
+    // a diagonal tile: its circles' box and radius edges back to their slots
+    if (P == Qb && tid < CS) {
+        const int c = P * CS + tid;
+        if (c < N) {
+            const long q0 = dense_unary_slot(c, a.nb, a.T);
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const long q = q0 + side * TT;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    a.X[k * S + q] = a.Xu[(6 * side + k) * (long)N + c];
+                    a.U[k * S + q] = a.Xu[(6 * side + 3 + k) * (long)N + c];
+                }
             }
         }
     }

@@ -295,11 +295,11 @@ __device__ __forceinline__ void cpol_tile_body(const CPolArgs& p, const int t, C
 #pragma unroll
                     for (int k = 0; k < 3; ++k) x[k] = zc[lane][3 + k] - (zc[lane][6 + k] - zc[lane][3 + k]);
                     rho = p.base[c];
-                } else {
-                    const long qq = dense_unary_slot(c, a.nb, a.T) + (kind == 2 ? TT : 0);
+                } else {  // the compact box (kind 1) or radius (kind 2) edge
+                    const int side = kind - 1;
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) x[k] = a.X[k * a.stride + qq];
-                    rho = a.rho[qq];
+                    for (int k = 0; k < 3; ++k) x[k] = a.Xu[(6 * side + k) * (long)N + c];
+                    rho = a.Xu[(12 + side) * (long)N + c];
                 }
                 cpol_features(x, &zc[lane][0], &zc[lane][3], rho,
                               kind == 0 ? PDM_EDGE_PAIR : kind == 1 ? PDM_EDGE_CONTAINER : PDM_EDGE_RADIUS, p.frac, f);
@@ -326,14 +326,17 @@ __device__ __forceinline__ void cpol_tile_body(const CPolArgs& p, const int t, C
                 const int c = P * CS + (e % CS), kind = 1 + e / CS;
                 if (c >= N) continue;
                 const long qq = dense_unary_slot(c, a.nb, a.T) + (kind == 2 ? TT : 0);
+                const int side = kind - 1;
+                float* xu = a.Xu + 6L * side * N + c;
                 const float r = cpol_rho(p, p.yc[(long)kind * N + c], g, lg);
-                const float sf = a.rho[qq] / r;
+                const float sf = a.Xu[(12 + side) * (long)N + c] / r;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const float x = a.X[k * a.stride + qq], z = a.Z[3 * c + k];
-                    a.U[k * a.stride + qq] = sf * ((x + a.U[k * a.stride + qq]) - z) - x + z;
+                    const float x = xu[k * (long)N], z = a.Z[3 * c + k];
+                    xu[(3 + k) * (long)N] = sf * ((x + xu[(3 + k) * (long)N]) - z) - x + z;
                 }
-                rhow[qq] = r;
+                rhow[qq] = r;  // the dense slot (read after the state is left) and the compact copy
+                a.Xu[(12 + side) * (long)N + c] = r;
             }
     }
 }
@@ -391,8 +394,7 @@ __device__ __forceinline__ void cpol_zprev_circle(const CPolArgs& p, const int c
         a.Zp[3 * c + k] = z - sf * (z - a.Zp[3 * c + k]);
     }
     p.base[c] = rv;
-    const long TT = (long)a.T * a.T, q0 = dense_unary_slot(c, a.nb, a.T);
-    const float rb = a.rho[q0], rr = a.rho[q0 + TT];
+    const float rb = a.Xu[12L * N + c], rr = a.Xu[13L * N + c];
     const double w = (double)(N - 1) * rv + (double)p.wacc[c] / (double)CPOL_WFIX + rb + rr;
     const double w2 = (double)(N - 1) * ((double)rv * rv) + (double)p.wacc[N + c] / (double)CPOL_W2FIX +
                       (double)rb * rb + (double)rr * rr;

@@ -83,6 +83,7 @@ struct DenseState {
     float* d_Zb = nullptr;               // [2][N][3] Z inside a run (its canonical arrays hold the run's start)
     int c_run = 0;                       // contact launches since the last run end
     double* d_Rpc = nullptr;   // [N] primal residual per circle
+    float* d_Xu = nullptr;     // [CUNARY][N] box and radius edges inside the contact state
     float* d_Zpp = nullptr;    // [N][3] Z two iterations back
     unsigned* d_flag = nullptr; // a circle sum lost a value (non-finite or out of range) since the state was entered
 
@@ -582,6 +583,7 @@ static CArgs cargs(DevGraph* s)
     a.Rpc = d->d_Rpc;
     a.Rd = d->d_Rd;
     a.flag = d->d_flag;
+    a.Xu = d->d_Xu;
     return a;
 }
 
@@ -633,6 +635,7 @@ static int contact_alloc(DevGraph* s)
     RC(dmalloc(s, (void**)&d->d_W, sizeof(float) * 2 * (size_t)d->N));
     RC(dmalloc(s, (void**)&d->d_flag, sizeof(unsigned)));
     RC(dmalloc(s, (void**)&d->d_base, sizeof(float) * (size_t)d->N));
+    RC(dmalloc(s, (void**)&d->d_Xu, sizeof(float) * CUNARY * (size_t)d->N));
     return PARADMM_OK;
 }
 
