@@ -244,7 +244,7 @@ def batch_alg_bytes(sizes, cs, listed):
     return tot
 
 
-def batch_bench(P, count, seed, iters, cap):
+def batch_bench(P, count, seed, iters, cap, per_edge=False):
     """The batched workload: `count` unit-square packings with N ~ U{16..512}
     drawn from the batch seed.  The autotuner picks the form (the contact
     state with one launch per iteration, or one cooperative launch) and the
@@ -252,12 +252,15 @@ def batch_bench(P, count, seed, iters, cap):
     retirement (whole solve, and the iterations alone with the contact
     roofline on the batch's own algorithmic bytes), the time-to-tolerance
     solve with per-instance retirement, and the learned policy on the same
-    batch against fixed rho = 1."""
+    batch against fixed rho = 1.  per_edge: the per-edge kernel (one
+    cooperative launch, no contact state) throughout."""
     t0 = time.time()
     sizes = P.batch_sizes(count, seed)
     gs = [P.packing_square(n, 1000 * seed + k, rho=batch_rho(n), alpha=1.0, a_exp=1.0) for k, n in enumerate(sizes)]
     E = sum(g.nedge for g in gs)
     b = P.Batch(gs)
+    if per_edge:
+        b.configure_contact(0, 0, 0)
     chosen, tried = b.autotune(20)
     t_setup = time.time() - t0
     print(f"# batch: {count} instances, E={E}, setup {t_setup:.1f} s, autotune {chosen} of {tried}",
@@ -297,7 +300,9 @@ def batch_bench(P, count, seed, iters, cap):
         gs = [P.packing_square(n, 1000 * seed + k, rho=rho0, alpha=1.0, a_exp=1.0) for k, n in enumerate(sizes)]
         b = P.Batch(gs)
         b.configure(info["tile"], 1)
-        if chosen.get("contact_tile"):
+        if per_edge:
+            b.configure_contact(0, 0, 0)
+        elif chosen.get("contact_tile"):
             b.configure_contact(chosen["contact_tile"], chosen["blocks_per_cu"], chosen["barrier"])
         if pol is not None:
             b.set_rho_policy(pol)
@@ -409,6 +414,8 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--batch-iters", type=int, default=1000)
     ap.add_argument("--batch-cap", type=int, default=20000)
+    ap.add_argument("--batch-per-edge", action="store_true",
+                    help="the batch on the per-edge kernel instead of the contact state")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-fixed time to tolerance")
     ap.add_argument("--host-loop-iters", type=int, default=0,
                     help="also time this many iterations of the legacy per-pass host loop")
@@ -500,7 +507,8 @@ def main():
     if host_loop:
         line["host_loop_pcie_inclusive"] = host_loop
     if args.batch > 0:
-        line["batch"] = batch_bench(P, args.batch, args.batch_seed, args.batch_iters, args.batch_cap)
+        line["batch"] = batch_bench(P, args.batch, args.batch_seed, args.batch_iters, args.batch_cap,
+                                    per_edge=args.batch_per_edge)
     if not args.no_learned:
         line["time_to_tol_learned_vs_fixed"] = learned_bench(P, ((5000, 3), (2000, 3), (2000, 4), (1000, 3), (1000, 4)),
                                                              30000)
