@@ -333,6 +333,41 @@ def batch_bench(P, count, seed, iters, cap, per_edge=False):
     }
 
 
+TRIANGLE_EDGE_UPDATE_BYTES = 52  # 4 (5 d + 3) at d = 2: the dim-2 counterpart of the 72 B unit
+
+
+def triangle_bench(P, sizes, seed, iters):
+    """The reference demo's own workload (packing/host.c:256-343, N = 2500 as
+    its main; packing/gpu.cu runs N = 5000): dim 2, four-edge pair factors,
+    three walls per circle, rho 10 (5 on the radius edges), alpha 0.1, on
+    the generic layout with the row-tiled pair pass.  Per size: time of
+    `iters` iterations, per-kernel means, and the roofline on the 52 B per
+    edge-update unit (alpha != 1: every edge carries its own U)."""
+    out = []
+    for n in sizes:
+        t0 = time.time()
+        g = P.packing_triangle(n, seed)
+        E = g.nedge
+        g.iterate(3)
+        setup = time.time() - t0
+        ms = g.time_iterations(iters) / iters
+        kernels = g.profile_kernels(2)
+        alg = TRIANGLE_EDGE_UPDATE_BYTES * E
+        r = {"n": n, "edges": E, "setup_s": setup, "iters": iters, "ms_per_iter": ms,
+             "edge_updates_per_s": E / (ms * 1e-3), "layout": "generic" if g.layout() == 1 else g.layout(),
+             "kernels_ms": {k: v[0] for k, v in kernels.items()},
+             "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_edge_update": TRIANGLE_EDGE_UPDATE_BYTES,
+                          "note": "whole iteration (pair X pass, wall and radius X passes, fused Z pass) against the "
+                                  "52 B per edge-update unit"}}
+        print(f"# triangle N={n}: E={E}, setup {setup:.1f} s, {ms:.4f} ms/iter, {r['edge_updates_per_s']:.4e} "
+              f"edge-updates/s, {r['roofline']['frac']:.2f} of HBM by 52 B/edge-update; kernels {kernels}",
+              file=sys.stderr, flush=True)
+        out.append(r)
+        g.close()
+    return out
+
+
 def learned_bench(P, cases, cap):
     """Time to tolerance (primal, dual, overlap, container <= 1e-4) on one
     GPU: plain fixed rho = 1 (the configured rho), plain fixed rho = 0.4 N
@@ -417,6 +452,8 @@ def main():
     ap.add_argument("--batch-per-edge", action="store_true",
                     help="the batch on the per-edge kernel instead of the contact state")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-fixed time to tolerance")
+    ap.add_argument("--triangle", type=str, default="2500,5000",
+                    help="circles of the reference demo's triangle workload legs (empty: skip)")
     ap.add_argument("--host-loop-iters", type=int, default=0,
                     help="also time this many iterations of the legacy per-pass host loop")
     args = ap.parse_args()
@@ -506,6 +543,8 @@ def main():
     os.environ.pop("PARADMM_CONTACT", None)
     if host_loop:
         line["host_loop_pcie_inclusive"] = host_loop
+    if args.triangle:
+        line["triangle_demo"] = triangle_bench(P, [int(x) for x in args.triangle.split(",")], args.seed, 20)
     if args.batch > 0:
         line["batch"] = batch_bench(P, args.batch, args.batch_seed, args.batch_iters, args.batch_cap,
                                     per_edge=args.batch_per_edge)

@@ -299,6 +299,100 @@ __global__ __launch_bounds__(256) void k_zpass_fused(ZArgs a)
 }
 
 // Legacy Z pass (core/paradmm.c:637-672): Z from the stored M.
+// Most edges per thread that k_zpass_fused_wg keeps in registers (X and U
+// of each: 2 C D floats, about 96 registers); the launch takes the
+// smallest of C = 8, 16, .. that covers the longest variable.
+template <int D>
+struct ZWG {
+    static constexpr int C = D == 1 ? 32 : D == 2 ? 24 : 16;
+};
+
+// The fused Z pass with one workgroup per variable, its edges' X and U held
+// in registers between the two sweeps (the Z sums, then the U update with
+// the new Z): X and U are read once (32 B per edge at D = 2, against 48 B
+// for k_zpass_fused, which reads them again).  For variables of at most
+// 256 C edges.  The sums run per thread over edges b + tid + 256 c, then
+// per wavefront and over the four wavefronts in a fixed order: runs are
+// bitwise reproducible.
+template <int D, int C>
+__global__ __launch_bounds__(256) void k_zpass_fused_wg(ZArgs a)
+{
+    __shared__ float red[4][D + 2];
+    __shared__ double redd[4];
+    const int v = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int b = a.var_off[v], end = a.var_off[v + 1];
+    const long S = a.stride;
+    float xs[C][D], us[C][D];
+    float sr = 0.0f, sr2 = 0.0f, sm[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) sm[k] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int p = b + tid + 256 * c;
+#pragma unroll
+        for (int k = 0; k < D; ++k) xs[c][k] = us[c][k] = 0.0f;
+        if (p < end) {
+            const float r = a.rho[p];
+            sr += r;
+            sr2 += r * r;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                xs[c][k] = a.X[k * S + p];
+                us[c][k] = a.U[k * S + p];
+                sm[k] += r * (xs[c][k] + us[c][k]);
+            }
+        }
+    }
+    sr = wave_sum(sr);
+    sr2 = wave_sum(sr2);
+#pragma unroll
+    for (int k = 0; k < D; ++k) sm[k] = wave_sum(sm[k]);
+    if (lane == 0) {
+        red[wv][0] = sr;
+        red[wv][1] = sr2;
+#pragma unroll
+        for (int k = 0; k < D; ++k) red[wv][2 + k] = sm[k];
+    }
+    __syncthreads();
+    const float tr = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+    const float tr2 = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
+    float z[D], zold[D];
+    double dz2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        z[k] = ((red[0][2 + k] + red[1][2 + k]) + (red[2][2 + k] + red[3][2 + k])) / tr;
+        zold[k] = a.Z[v * D + k];
+        const double t = (double)(z[k] - zold[k]);
+        dz2 += t * t;
+    }
+    double pr = 0.0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int p = b + tid + 256 * c;
+        if (p < end) {
+            const float al = a.alpha[p];
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float t = xs[c][k] - z[k];
+                a.U[k * S + p] = us[c][k] + al * t;
+                pr += (double)t * (double)t;
+            }
+        }
+    }
+    pr = wave_sum(pr);
+    if (lane == 0) redd[wv] = pr;
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            a.Z[v * D + k] = z[k];
+            a.Zprev[v * D + k] = zold[k];
+        }
+        a.res_var[2 * v] = (redd[0] + redd[1]) + (redd[2] + redd[3]);
+        a.res_var[2 * v + 1] = (double)tr2 * dz2;
+    }
+}
+
 template <int D>
 __global__ __launch_bounds__(256) void k_zpass_legacy(ZArgs a)
 {
@@ -496,6 +590,58 @@ int dmalloc(DevGraph* s, void** p, size_t bytes)
 #define RC(x) do { int r_ = (x); if (r_) return r_; } while (0)
 
 // Builds the generic CSR layout and the per-kind factor tables.
+// The row structure of the dim-2 all-pairs packing (DevGraph::rows2d_n):
+// variables 2c (position) and 2c + 1 (radius) of circles c < nc, one
+// PAIR_2D factor (2i, 2i + 1, 2j, 2j + 1) for every i < j, and each of
+// these edges at its row position.  The reference demo builds exactly this
+// (packing/host.c:256-343: radius factors, then the walls, then the pairs
+// i < j in order), so its variables list their pair edges in the order of
+// the other circle.  Anything else keeps the per-factor pass.
+// PARADMM_PAIR_ROWS=0 turns it off.
+static int detect_pair2d_rows(DevGraph* s, graph* G, const std::vector<int>& fkind, const std::vector<int>& pos)
+{
+    if (const char* e = getenv("PARADMM_PAIR_ROWS"))
+        if (e[0] == '0') return PARADMM_OK;
+    if (G->num_dims != 2 || G->num_vars % 2) return PARADMM_OK;
+    const int nc = G->num_vars / 2;
+    if (nc < 2) return PARADMM_OK;
+    long npairs = 0;
+    for (int f = 0; f < G->num_opts; ++f) npairs += fkind[f] == PARADMM_KIND_PAIR_2D;
+    if (npairs != (long)nc * (nc - 1) / 2) return PARADMM_OK;
+    // a pair listed twice would need two edges at one position: the position
+    // map is one-to-one, so the row checks below also rule that out
+    std::vector<long> rowP(nc, -1), rowR(nc, -1);
+    auto set_row = [](std::vector<long>& row, int c, long v) {
+        if (row[c] < 0) row[c] = v;
+        return row[c] == v;
+    };
+    for (int f = 0; f < G->num_opts; ++f) {
+        if (fkind[f] != PARADMM_KIND_PAIR_2D) continue;
+        if (G->operatorix_to_numedges[f] != 4) return PARADMM_OK;
+        const int* el = G->operatorix_to_edgelistix[f];
+        const int v0 = G->edgeix_to_varlistix[el[0]], v1 = G->edgeix_to_varlistix[el[1]];
+        const int v2 = G->edgeix_to_varlistix[el[2]], v3 = G->edgeix_to_varlistix[el[3]];
+        if (v0 % 2 || v1 != v0 + 1 || v2 % 2 || v3 != v2 + 1) return PARADMM_OK;
+        const int i = v0 / 2, j = v2 / 2;
+        if (i >= j) return PARADMM_OK;
+        if (!set_row(rowP, i, (long)pos[el[0]] - (j - 1)) || !set_row(rowR, i, (long)pos[el[1]] - (j - 1)) ||
+            !set_row(rowP, j, (long)pos[el[2]] - i) || !set_row(rowR, j, (long)pos[el[3]] - i))
+            return PARADMM_OK;
+    }
+    std::vector<int> rp(nc), rr(nc);
+    for (int c = 0; c < nc; ++c) {
+        if (rowP[c] < 0 || rowR[c] < 0) return PARADMM_OK;
+        rp[c] = (int)rowP[c];
+        rr[c] = (int)rowR[c];
+    }
+    RC(dmalloc(s, (void**)&s->d_rowP, sizeof(int) * (size_t)nc));
+    RC(dmalloc(s, (void**)&s->d_rowR, sizeof(int) * (size_t)nc));
+    CK(hipMemcpy(s->d_rowP, rp.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->d_rowR, rr.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice));
+    s->rows2d_n = nc;
+    return PARADMM_OK;
+}
+
 static int finalize_generic(DevGraph* s, graph* G, const std::vector<int>& fkind)
 {
     const int E = G->num_edges, V = G->num_vars, F = G->num_opts, D = G->num_dims;
@@ -515,6 +661,16 @@ static int finalize_generic(DevGraph* s, graph* G, const std::vector<int>& fkind
         }
     }
     var_off[V] = p;
+    int maxdeg = 0;
+    for (int v = 0; v < V; ++v) maxdeg = std::max(maxdeg, var_off[v + 1] - var_off[v]);
+    // edges per thread of k_zpass_fused_wg (0: the wavefront-per-variable
+    // pass): the smallest of 8, 16, 24 .. ZWG<D>::C that covers maxdeg
+    const int cmax = D == 1 ? ZWG<1>::C : D == 2 ? ZWG<2>::C : D == 3 ? ZWG<3>::C : 0;
+    s->zpass_wg = 0;
+    for (int c = 8; c <= cmax && !s->zpass_wg; c += 8)
+        if (maxdeg <= 256 * c) s->zpass_wg = c;
+    if (const char* ze = getenv("PARADMM_ZPASS_WG"))
+        if (ze[0] == '0') s->zpass_wg = 0;
     if (p != E) {
         pdm_set_error(PARADMM_ERR_STATE, "graph inconsistent: %d of %d edges listed by variables", p, E);
         return PARADMM_ERR_STATE;
@@ -572,6 +728,7 @@ static int finalize_generic(DevGraph* s, graph* G, const std::vector<int>& fkind
         }
         s->kinds.push_back(kt);
     }
+    RC(detect_pair2d_rows(s, G, fkind, pos));
     // factors whose callback runs on the host
     HostProx hp;
     std::vector<int> hpos;
@@ -834,6 +991,121 @@ static int download(DevGraph* s, int which)
 
 // ------------------------------------------------------------------ passes
 
+// The pair X pass of the dim-2 all-pairs packing whose edges lie in rows
+// (DevGraph::rows2d_n): one workgroup per tile of T x T circle pairs
+// (i in block I, j in block J, i < j).  The first circle's edges of the
+// tile's pairs are row segments (rowP[i] + j - 1, rowR[i] + j - 1:
+// consecutive j, read and written with lanes along j); the second's are
+// the transposed block (rowP[j] + i, rowR[j] + i: consecutive i), read with
+// lanes along i into LDS and written back the same way.  The prox and its
+// inputs are those of k_xpass<PAIR_2D>, so the results are the same bits.
+struct XRowArgs {
+    int nc, nb;
+    const int *rowP, *rowR;
+    const float* Z;     // [V][2]: circle c's pos at 4c, rad at 4c + 2
+    const float* U;
+    const float* Nbuf;
+    const float* rho;
+    float* X;
+    long stride;
+    int from_n;
+};
+
+template <int T>
+__global__ __launch_bounds__(256) void k_xpass_pair2d_rows(XRowArgs a)
+{
+    // second circle's N (pos 2, rad 2) and rho (pos, rad), [.][jl][il]; each
+    // thread overwrites its own entry's N with the second circle's X
+    __shared__ float bin[6][T][T + 1];
+    const int tid = threadIdx.x;
+    // tile t of the upper triangle of nb x nb blocks
+    const int t = blockIdx.x;
+    int I = (int)((2.0f * a.nb + 1.0f - sqrtf((2.0f * a.nb + 1.0f) * (2.0f * a.nb + 1.0f) - 8.0f * (float)t)) * 0.5f);
+    I = max(0, min(I, a.nb - 1));
+    while (I > 0 && I * a.nb - I * (I - 1) / 2 > t) --I;
+    while (I + 1 < a.nb && (I + 1) * a.nb - (I + 1) * I / 2 <= t) ++I;
+    const int J = I + (t - (I * a.nb - I * (I - 1) / 2));
+    const long S = a.stride;
+    // the second circles' inputs, lanes along i
+    for (int idx = tid; idx < T * T; idx += 256) {
+        const int jl = idx / T, il = idx % T;
+        const int i = I * T + il, j = J * T + jl;
+        if (i >= a.nc || j >= a.nc || i >= j) continue;
+        const long p = a.rowP[j] + i, q = a.rowR[j] + i;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            bin[k][jl][il] = a.from_n ? a.Nbuf[k * S + p] : a.Z[4L * j + k] - a.U[k * S + p];
+            bin[2 + k][jl][il] = a.from_n ? a.Nbuf[k * S + q] : a.Z[4L * j + 2 + k] - a.U[k * S + q];
+        }
+        bin[4][jl][il] = a.rho[p];
+        bin[5][jl][il] = a.rho[q];
+    }
+    __syncthreads();
+    // the first circles' edges, lanes along j; the prox
+    for (int idx = tid; idx < T * T; idx += 256) {
+        const int il = idx / T, jl = idx % T;
+        const int i = I * T + il, j = J * T + jl;
+        if (i >= a.nc || j >= a.nc || i >= j) continue;
+        const long p = a.rowP[i] + j - 1, q = a.rowR[i] + j - 1;
+        float n[4][2], x[4][2], rho[4];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            n[0][k] = a.from_n ? a.Nbuf[k * S + p] : a.Z[4L * i + k] - a.U[k * S + p];
+            n[1][k] = a.from_n ? a.Nbuf[k * S + q] : a.Z[4L * i + 2 + k] - a.U[k * S + q];
+            n[2][k] = bin[k][jl][il];
+            n[3][k] = bin[2 + k][jl][il];
+        }
+        rho[0] = a.rho[p];
+        rho[1] = a.rho[q];
+        rho[2] = bin[4][jl][il];
+        rho[3] = bin[5][jl][il];
+        prox_pair2(x, n, rho);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            a.X[k * S + p] = x[0][k];
+            a.X[k * S + q] = x[1][k];
+            bin[k][jl][il] = x[2][k];
+            bin[2 + k][jl][il] = x[3][k];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < T * T; idx += 256) {
+        const int jl = idx / T, il = idx % T;
+        const int i = I * T + il, j = J * T + jl;
+        if (i >= a.nc || j >= a.nc || i >= j) continue;
+        const long p = a.rowP[j] + i, q = a.rowR[j] + i;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            a.X[k * S + p] = bin[k][jl][il];
+            a.X[k * S + q] = bin[2 + k][jl][il];
+        }
+    }
+}
+
+constexpr int XROW_T = 32;
+
+static int launch_x_pair2d_rows(DevGraph* s, int from_n)
+{
+    XRowArgs a;
+    a.nc = s->rows2d_n;
+    a.nb = (a.nc + XROW_T - 1) / XROW_T;
+    a.rowP = s->d_rowP;
+    a.rowR = s->d_rowR;
+    a.Z = s->d_Z;
+    a.U = s->d_U;
+    a.Nbuf = s->d_N;
+    a.rho = s->d_rho;
+    a.X = s->d_X;
+    a.stride = s->stride;
+    a.from_n = from_n;
+    const unsigned nt = (unsigned)(a.nb * (a.nb + 1) / 2);
+    prof_begin(s, "disk_collision_2d_rows");
+    k_xpass_pair2d_rows<XROW_T><<<nt, 256, 0, s->stream>>>(a);
+    prof_end(s);
+    CK(hipGetLastError());
+    return PARADMM_OK;
+}
+
 template <int K>
 static int launch_x_kind(DevGraph* s, const KindTable& kt, int from_n)
 {
@@ -937,7 +1209,10 @@ static int launch_xpass_generic(DevGraph* s, int from_n)
 {
     for (const KindTable& kt : s->kinds) {
         switch (kt.kind) {
-        case PARADMM_KIND_PAIR_2D: RC(launch_x_kind<PARADMM_KIND_PAIR_2D>(s, kt, from_n)); break;
+        case PARADMM_KIND_PAIR_2D:
+            if (s->rows2d_n) RC(launch_x_pair2d_rows(s, from_n));
+            else RC(launch_x_kind<PARADMM_KIND_PAIR_2D>(s, kt, from_n));
+            break;
         case PARADMM_KIND_WALL_2D: RC(launch_x_kind<PARADMM_KIND_WALL_2D>(s, kt, from_n)); break;
         case PARADMM_KIND_RADIUS_2D: RC(launch_x_kind<PARADMM_KIND_RADIUS_2D>(s, kt, from_n)); break;
         case PARADMM_KIND_PAIR_3D: RC(launch_x_kind<PARADMM_KIND_PAIR_3D>(s, kt, from_n)); break;
@@ -1062,7 +1337,25 @@ static int fused_iteration(DevGraph* s, int from_n, int mphase = DENSE_PLAIN)
     RC(launch_xpass_generic(s, from_n));
     ZArgs a = zargs(s);
     prof_begin(s, "zpass_fused");
-    DISPATCH_D(s->D, k_zpass_fused, nblk(s->V, 4), a);
+    if (s->zpass_wg) {
+        const unsigned g = (unsigned)s->V;
+        const int c = s->zpass_wg;
+        if (s->D == 2) {
+            if (c <= 8) k_zpass_fused_wg<2, 8><<<g, 256, 0, s->stream>>>(a);
+            else if (c <= 16) k_zpass_fused_wg<2, 16><<<g, 256, 0, s->stream>>>(a);
+            else k_zpass_fused_wg<2, ZWG<2>::C><<<g, 256, 0, s->stream>>>(a);
+        } else if (s->D == 3) {
+            if (c <= 8) k_zpass_fused_wg<3, 8><<<g, 256, 0, s->stream>>>(a);
+            else k_zpass_fused_wg<3, ZWG<3>::C><<<g, 256, 0, s->stream>>>(a);
+        } else {
+            if (c <= 8) k_zpass_fused_wg<1, 8><<<g, 256, 0, s->stream>>>(a);
+            else if (c <= 16) k_zpass_fused_wg<1, 16><<<g, 256, 0, s->stream>>>(a);
+            else k_zpass_fused_wg<1, ZWG<1>::C><<<g, 256, 0, s->stream>>>(a);
+        }
+        CK(hipGetLastError());
+    } else {
+        DISPATCH_D(s->D, k_zpass_fused, nblk(s->V, 4), a);
+    }
     prof_end(s);
     return PARADMM_OK;
 }

@@ -111,6 +111,16 @@ struct DevGraph {
 
     DenseState* dense = nullptr;
     HostProx* host = nullptr;  // generic layout only
+    // generic layout, dim-2 all-pairs packing of the reference demo: every
+    // pair factor (pos_i, rad_i, pos_j, rad_j), i < j, has its edges in
+    // rows -- circle c's pos and rad edges of its pairs at rowP[c] + col
+    // and rowR[c] + col, col = the other circle's index, less one past c --
+    // so the pair X pass runs as coalesced tiles (k_xpass_pair2d_rows)
+    int zpass_wg = 0;           // generic layout: edges per thread of k_zpass_fused_wg (0: every variable's
+                                // edges do not fit its registers)
+    int rows2d_n = 0;           // circles (0: no row structure)
+    int* d_rowP = nullptr;      // [rows2d_n]
+    int* d_rowR = nullptr;
 
     // rho policy applied by paradmm_gpu_solve every policy_every iterations;
     // iter_frac = iterations done / max_iters.  Set by the learned-rho module.

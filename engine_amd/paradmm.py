@@ -584,6 +584,30 @@ class Batch:
             pass
 
 
+def packing_triangle(ncircles, seed, lo=-1.0, hi=2.0):
+    """The reference demo's packing (packing/host.c:256-343): circles of a
+    position and a radius variable (dim 2) in the triangle of walls
+    y >= 0, y <= sqrt(3) x and y <= sqrt(3) (1 - x); one radius factor per
+    circle (exponent 2), three wall factors per circle, one pair factor per
+    pair, created in host.c's order with the library's dim-2 operators;
+    X, N, Z, M, U seeded uniform in [lo, hi] (paradmm_init_rand_seeded: the
+    reference draws them from rand()), rho = 10 and alpha = 0.1 on every
+    edge, rho = 5 on the radius factors' edges."""
+    s = np.sin(np.pi / 3)
+    walls = np.array([[0., -1., 0., 0.], [-s, 0.5, 0., 0.], [s, 0.5, 1., 0.]])
+    g = Graph(2)
+    c = np.arange(ncircles, dtype=np.int32)
+    g.add_factors(prox_address("paradmm_prox_max_radius_2d"), (2 * c + 1).reshape(-1, 1), np.array([2.0]))
+    for k in range(3):
+        g.add_factors(prox_address("paradmm_prox_disk_plane_2d"), np.stack([2 * c, 2 * c + 1], 1), walls[k])
+    i, j = np.triu_indices(ncircles, 1)
+    g.add_factors(prox_address("paradmm_prox_disk_collision_2d"), np.stack([2 * i, 2 * i + 1, 2 * j, 2 * j + 1], 1))
+    g.init_rand_seeded(seed, (lo, hi), (lo, hi), (lo, hi), (lo, hi), (lo, hi))
+    g.init_rho_alpha(10.0, 0.1)
+    g.array("RHOS")[:ncircles] = 5.0
+    return g
+
+
 def batch_sizes(count, seed, lo=16, hi=512):
     """Instance sizes N_i ~ U{lo..hi} drawn from the batch seed (splitmix64)."""
     out = []
