@@ -1018,8 +1018,13 @@ __global__ __launch_bounds__(256) void k_xpass_pair2d_rows(XRowArgs a)
     // thread overwrites its own entry's N with the second circle's X
     __shared__ float bin[6][T][T + 1];
     const int tid = threadIdx.x;
-    // tile t of the upper triangle of nb x nb blocks
-    const int t = blockIdx.x;
+    // tile t of the upper triangle of nb x nb blocks.  Workgroups are dealt
+    // to the 8 XCDs in turn (blockIdx % 8), each with its own L2; each XCD
+    // takes a contiguous run of tiles instead, so the tiles (I, J), (I, J + 1)
+    // that share the cache lines where their row segments meet share an L2
+    const int nt = a.nb * (a.nb + 1) / 2, chunk = (nt + 7) / 8;
+    const int t = (int)(blockIdx.x % 8) * chunk + (int)(blockIdx.x / 8);
+    if (t >= nt) return;
     int I = (int)((2.0f * a.nb + 1.0f - sqrtf((2.0f * a.nb + 1.0f) * (2.0f * a.nb + 1.0f) - 8.0f * (float)t)) * 0.5f);
     I = max(0, min(I, a.nb - 1));
     while (I > 0 && I * a.nb - I * (I - 1) / 2 > t) --I;
@@ -1100,7 +1105,7 @@ static int launch_x_pair2d_rows(DevGraph* s, int from_n)
     a.from_n = from_n;
     const unsigned nt = (unsigned)(a.nb * (a.nb + 1) / 2);
     prof_begin(s, "disk_collision_2d_rows");
-    k_xpass_pair2d_rows<XROW_T><<<nt, 256, 0, s->stream>>>(a);
+    k_xpass_pair2d_rows<XROW_T><<<8 * ((nt + 7) / 8), 256, 0, s->stream>>>(a);
     prof_end(s);
     CK(hipGetLastError());
     return PARADMM_OK;
