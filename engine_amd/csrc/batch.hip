@@ -2014,20 +2014,45 @@ extern "C" int paradmm_batch_autotune(paradmm_batch* b, int iters, paradmm_batch
         if (b->T != 32) {
             RC(paradmm_batch_configure(b, 32, 1, b->barrier));
         }
+        // every candidate starts from the uploaded state: a device copy of
+        // it (X, U, N, Z; rho and alpha do not change without a policy)
+        // restored before each run instead of a host upload
+        const size_t S = (size_t)b->slots, nz = 3 * (size_t)b->ncirc;
+        float* snap = nullptr;
+        CK(hipMalloc((void**)&snap, sizeof(float) * (9 * S + nz)));
+        auto restore = [&](bool save) -> int {
+            float* parts[4] = {b->X, b->U, b->Nb, b->Z};
+            const size_t len[4] = {3 * S, 3 * S, 3 * S, nz};
+            size_t off = 0;
+            for (int k = 0; k < 4; ++k) {
+                float* dst = save ? snap + off : parts[k];
+                const float* src = save ? parts[k] : snap + off;
+                CK(hipMemcpyAsync(dst, src, sizeof(float) * len[k], hipMemcpyDeviceToDevice, b->stream));
+                off += len[k];
+            }
+            if (!save) {
+                CK(hipMemcpyAsync(b->Zprev, b->Z, sizeof(float) * nz, hipMemcpyDeviceToDevice, b->stream));
+                RC(batch_reset_flags(b));
+            }
+            CK(hipStreamSynchronize(b->stream));
+            return PARADMM_OK;
+        };
+        int rc = restore(true);
         const int sides[2] = {32, 64};
-        for (int si = 0; si < 2; ++si) {
+        for (int si = 0; si < 2 && !rc; ++si) {
             const int CS = sides[si];
             int per_cu = 0;
-            RC(contact_occupancy(b, CS, &per_cu));
-            for (int bpc = 0; bpc <= per_cu && bpc <= 4; ++bpc)
-                for (int bar = 0; bar < 2; ++bar) {
+            rc = contact_occupancy(b, CS, &per_cu);
+            for (int bpc = 0; !rc && bpc <= per_cu && bpc <= 4; ++bpc)
+                for (int bar = 0; !rc && bar < 2; ++bar) {
                     if (bpc == 0 && bar == 1) continue;  // no grid barrier in the launch-per-iteration form
-                    RC(paradmm_batch_configure_contact(b, CS, bpc, bar));
                     float ms = 0.0f;
-                    RC(batch_upload(b));
-                    RC(batch_contact_solve(b, &o, &ms));  // warm
-                    RC(batch_upload(b));
-                    RC(batch_contact_solve(b, &o, &ms));
+                    rc = paradmm_batch_configure_contact(b, CS, bpc, bar);
+                    if (!rc) rc = restore(false);
+                    if (!rc) rc = batch_contact_solve(b, &o, &ms);  // warm
+                    if (!rc) rc = restore(false);
+                    if (!rc) rc = batch_contact_solve(b, &o, &ms);
+                    if (rc) break;
                     const double per = b->last_ms[2] / (iters - 1);
                     const paradmm_batch_config c{b->T, bpc, bar, per, CS};
                     if (tried && n < max_tried) tried[n] = c;
@@ -2035,6 +2060,8 @@ extern "C" int paradmm_batch_autotune(paradmm_batch* b, int iters, paradmm_batch
                     if (best.contact_tile == 0 || per < best.ms_per_iter) best = c;
                 }
         }
+        (void)hipFree(snap);
+        RC(rc);
         if (ntried) *ntried = n;
         RC(paradmm_batch_configure_contact(b, best.contact_tile, best.blocks_per_cu, best.barrier));
         if (chosen) *chosen = best;
