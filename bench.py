@@ -10,14 +10,16 @@ launched through the engine's captured hipGraph and bracketed by HIP events
 on the engine's stream (the library synchronises before and after).  With
 one rho and alpha = 1 the engine runs the dense layout's contact state (only
 pairs whose prox is active keep per-edge values; every edge is still updated
-every iteration).
+every iteration), from N = 4500 in its spatial form (circles in Morton
+order, contact tiles only where two blocks of circles can touch).
 
 Prints one JSON line: value = edge-updates/s, plus the roofline of the
 dominant kernel (its algorithmic bytes per launch / mean launch time from HIP
 events recorded by the dispatch itself, the HBM traffic of the committed
 rocprofv3 counter profile, and for the contact state its VALU issue rate).
-Further legs: the per-edge path on the same workload (PARADMM_CONTACT=0, HBM
-bound), N = 1000 circles, the 256-instance batch, learned vs fixed rho to
+Further legs: the contact state's id-order form and the per-edge path on
+the same workload (PARADMM_CONTACT=0, HBM bound), N = 1000 circles, both
+contact forms at N = 10000, the 256-instance batch, learned vs fixed rho to
 tolerance, and the CPU baseline: the FP64 restatement of the reference's
 OpenMP path compiled on this host and timed on all its usable cores and on
 one, with the GPU's first iterations checked against the FP64 oracle.
@@ -58,9 +60,9 @@ BYTES_PER_EDGE = {
 UNIFORM_BYTES_PER_EDGE = {"pairtile_fused": 48, "pairtile_m": 24}
 # kernels that carry a whole iteration's edge work (their launches count
 # E edge-updates each)
-FULL_ITERATION_KERNELS = ("pairtile_fused", "pairtile_m", "contact_tile")
+FULL_ITERATION_KERNELS = ("pairtile_fused", "pairtile_m", "contact_tile", "scs_rows")
 TRAFFIC_PROFILES = {k: os.path.join(ROOT, "profiles", f"{k}_n5000_hbm_traffic.md") for k in FULL_ITERATION_KERNELS}
-VALU_PROFILES = {"contact_tile": os.path.join(ROOT, "profiles", "contact_tile_n5000_valu.md")}
+VALU_PROFILES = {k: os.path.join(ROOT, "profiles", f"{k}_n5000_valu.md") for k in ("contact_tile", "scs_rows")}
 # MI355X vector issue: 256 CUs x 4 SIMDs, one wave64 VALU instruction per
 # SIMD every 4 cycles at 2.4 GHz
 VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
@@ -133,6 +135,49 @@ def roofline_contact(dom_ms, alg_bytes, traffic, valu, copy_gbs, n, edges, liste
                                      "pairs, so this is a throughput equivalent, not a bandwidth"},
         "limiter": "latency: all tile wavefronts are resident at once and wait on their first loads "
                    "(Z of the tile's circles, list entries); HBM and VALU are both well below peak",
+    }
+    if valu:
+        rate = valu / t
+        r["valu"] = {"insts_per_launch": valu, "achieved": rate, "peak": VALU_PEAK_WAVE_INSTS,
+                     "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK_WAVE_INSTS}
+    return r
+
+
+def spatial_alg_bytes(n, listed):
+    """Bytes the spatial form's row kernel (k_scs_rows) must move at the
+    least: the contact lists (both edges' M, 32 B per listed pair, read and
+    as many written), per circle its (Z_k, c) read once (32 B) and its three
+    sums added (24 B), and the box and radius edges' X and U read and
+    written (96 B).  The per-circle kernel before it (k_scs_circles: Z and
+    the pending sums read, Z and (Z_k, c) stored, the sum buffer cleared)
+    is timed and reported on its own."""
+    return 64 * listed + n * (32 + 24 + 96)
+
+
+def roofline_spatial(dom_ms, alg_bytes, traffic, valu, copy_gbs, edges, listed, circles_ms):
+    """Roofline of the spatial form's row kernel: its own algorithmic bytes
+    per launch against HBM peak, the measured traffic of the committed
+    counter profile, and its VALU issue rate."""
+    t = dom_ms * 1e-3
+    achieved = alg_bytes / t / 1e9
+    r = {
+        "bound": "hbm", "kernel": "scs_rows", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "algorithmic_bytes_per_launch": alg_bytes, "listed_pairs": listed,
+        "traffic_gbs": (traffic / t / 1e9) if traffic else None,
+        "traffic_frac": (traffic / t / 1e9 / HBM_PEAK_GBS) if traffic else None,
+        "traffic_frac_of_copy": (traffic / t / 1e9 / copy_gbs) if traffic else None,
+        "mean_launch_ms": dom_ms, "copy_kernel_gbs": copy_gbs,
+        "circles_kernel_ms": circles_ms,
+        "edge_update_unit": {"bytes_per_edge_update": EDGE_UPDATE_BYTES,
+                             "bytes_per_launch": EDGE_UPDATE_BYTES * edges,
+                             "equivalent_gbs": EDGE_UPDATE_BYTES * edges / t / 1e9,
+                             "note": "the per-edge state an X pass and a Z pass would move for the same "
+                                     "edge-updates; the contact state stores per-edge values only for listed "
+                                     "pairs, so this is a throughput equivalent, not a bandwidth"},
+        "limiter": "latency: each row workgroup lists its running tiles, then its wavefronts take them in "
+                   "turn; a tile's loads (its circles' Z, list entries) and its prox rounds are one dependent "
+                   "chain; HBM and VALU are both well below peak",
     }
     if valu:
         rate = valu / t
@@ -402,12 +447,17 @@ def learned_bench(P, cases, cap):
     return out
 
 
-def time_leg(P, n, seed, steps, warmup, profile_iters, contact):
+def time_leg(P, n, seed, steps, warmup, profile_iters, contact, sparse=None):
     """One single-instance leg: N circles, rho = 1, alpha = 1; the contact
-    state (the default for one rho and alpha = 1) or the per-edge path
+    state (the default for one rho and alpha = 1; sparse: its spatial form
+    forced on or off, None: the size rule) or the per-edge path
     (PARADMM_CONTACT=0).  Returns timing, kernel profile and what the
     roofline needs."""
     os.environ["PARADMM_CONTACT"] = "1" if contact else "0"
+    if sparse is None:
+        os.environ.pop("PARADMM_CONTACT_SPARSE", None)
+    else:
+        os.environ["PARADMM_CONTACT_SPARSE"] = "1" if sparse else "0"
     t0 = time.time()
     g = P.packing_square(n, seed, rho=1.0, alpha=1.0, a_exp=1.0)
     g.prepare()
@@ -422,9 +472,12 @@ def time_leg(P, n, seed, steps, warmup, profile_iters, contact):
     g.close()
     per_iter = {k: v[0] * v[1] / profile_iters for k, v in prof.items()}
     dom = max(per_iter, key=per_iter.get)
+    form = ("spatial" if "scs_rows" in prof else "id-order") if in_c else None
+    os.environ.pop("PARADMM_CONTACT_SPARSE", None)
     leg = {"n": n, "edges": E, "setup_s": t_setup, "steps": steps, "ms": ms, "ms_per_step": ms / steps,
            "edge_updates_per_s": E * steps / (ms * 1e-3), "layout": layout,
-           "state": "contact" if in_c else "per-edge", "contact_tile_side": cs, "listed_pairs": listed,
+           "state": "contact" if in_c else "per-edge", "contact_form": form,
+           "contact_tile_side": cs, "listed_pairs": listed,
            "kernels_ms": {k: v[0] for k, v in prof.items()}, "dominant": dom, "dominant_ms": prof[dom][0],
            "residuals_last": {"primal2": p2, "dual2": d2}}
     print(f"# N={n} {leg['state']}: setup {t_setup:.1f} s, {steps} iterations {ms / steps:.4f} ms/iter, "
@@ -445,6 +498,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-edge", action="store_true", help="skip the per-edge (PARADMM_CONTACT=0) leg")
     ap.add_argument("--no-n1000", action="store_true", help="skip the N = 1000 leg")
+    ap.add_argument("--no-id-order", action="store_true",
+                    help="skip the id-order contact form leg on the headline workload")
+    ap.add_argument("--large-n", type=int, default=10000,
+                    help="circles of the leg timing both contact forms at a larger packing (0: skip)")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batched workload (0: skip)")
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--batch-iters", type=int, default=1000)
@@ -474,7 +531,11 @@ def main():
     dom, dom_ms = head["dominant"], head["dominant_ms"]
     traffic = measured_traffic(dom, args.n, args.seed)
     valu = measured_valu(dom, args.n, args.seed)
-    if head["state"] == "contact":
+    if head["state"] == "contact" and dom == "scs_rows":
+        alg = spatial_alg_bytes(args.n, head["listed_pairs"])
+        roof = roofline_spatial(dom_ms, alg, traffic, valu, copy_gbs, E, head["listed_pairs"],
+                                head["kernels_ms"].get("scs_circles"))
+    elif head["state"] == "contact":
         alg = contact_alg_bytes(args.n, head["contact_tile_side"], head["listed_pairs"])
         roof = roofline_contact(dom_ms, alg, traffic, valu, copy_gbs, args.n, E, head["listed_pairs"])
     else:
@@ -527,6 +588,7 @@ def main():
             "workload": f"plain ADMM unit-square circle packing, N={args.n} circles, E={E} edges of dim 3, "
                         f"fixed rho=1, alpha=1, radius exponent 1, seed {args.seed}",
             "circles": args.n, "edges": E, "layout": head["layout"], "state": head["state"],
+            "contact_form": head["contact_form"],
         },
         "roofline": roof,
         "kernels_ms": head["kernels_ms"],
@@ -538,8 +600,21 @@ def main():
         pe["roofline"] = roofline_per_edge(pe["dominant"], pe["dominant_ms"], kb,
                                            measured_traffic(pe["dominant"], args.n, args.seed), copy_gbs, pe["edges"])
         line["per_edge_path"] = pe
+    if head["contact_form"] == "spatial" and not args.no_id_order:
+        # the contact state's id-order form on the same workload
+        io = time_leg(P, args.n, args.seed, args.steps, args.warmup, args.profile_iters, contact=True, sparse=False)
+        io["roofline"] = roofline_contact(io["dominant_ms"], contact_alg_bytes(args.n, io["contact_tile_side"],
+                                                                              io["listed_pairs"]),
+                                          measured_traffic(io["dominant"], args.n, args.seed),
+                                          measured_valu(io["dominant"], args.n, args.seed), copy_gbs, args.n,
+                                          io["edges"], io["listed_pairs"])
+        line["id_order_contact"] = io
     if not args.no_n1000 and args.n != 1000:
         line["n1000"] = time_leg(P, 1000, args.seed, 10000, 200, args.profile_iters, contact=True)
+    if args.large_n:
+        # the spatial form's O(N) iteration against the id-order form's O(N^2) at a larger packing
+        line["large_n"] = [time_leg(P, args.large_n, args.seed, args.steps, args.warmup, args.profile_iters,
+                                    contact=True, sparse=sp) for sp in (True, False)]
     os.environ.pop("PARADMM_CONTACT", None)
     if host_loop:
         line["host_loop_pcie_inclusive"] = host_loop

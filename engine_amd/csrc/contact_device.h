@@ -82,7 +82,17 @@ struct CArgs {
     double* Rd;
     unsigned* flag;        // set when a circle sum could not take a value
     float* Xu;             // [CUNARY][N] the box and radius edges' state inside the contact state
+    // spatial form (contact_sparse.h): circles in contact order, a tile only
+    // where its two blocks' boxes meet; null in the id-order form
+    const int* perm = nullptr;   // [N] contact index -> circle
+    const int* inv = nullptr;    // [N] circle -> contact index
+    float4* zcv = nullptr;       // [2][N] per contact circle: (Z_k, c) of this launch, written before the tiles
+    float4* bb = nullptr;        // [nbs] per block: box of its circles' common N (x lo, x hi, y lo, y hi)
 };
+
+// words of a contact tile row's pair mask
+template <int CS>
+constexpr int cwords() { return CS >= 32 ? CS / 32 : 1; }
 
 // v rounded to the nearest multiple of 2^-31, in units of 2^-31:
 // v = k + f with k = rint(v) and f = v - k exact (|f| <= 1/2), so
@@ -196,15 +206,15 @@ __device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOM
 // ring capacity: more than 63 pending items plus the 2 RL 64 pairs one test
 // step can add (RL = CS / 8): 1024 for CS = 32, 2048 for CS = 64
 template <int CS>
-constexpr int contact_ring() { return CS == 32 ? 1024 : 2048; }
+constexpr int contact_ring() { return CS == 16 ? 512 : CS == 32 ? 1024 : 2048; }
 
 template <int CS, int Q>
 struct ContactWave {
     unsigned long long acc[2 * CS][Q];  // fixed-point circle sums: rows (block P), then columns (block Q)
     float zc[2 * CS][6];                // Z_k and c = Z_k - Z_{k-1} of the tile's 2 CS circles
     unsigned short ring[contact_ring<CS>()];  // colliding pairs not in the previous list (slot)
-    unsigned mask[CS][CS / 32];         // pairs in the previous list, per row
-    unsigned nmask[CS][CS / 32];        // pairs in the new list
+    unsigned mask[CS][cwords<CS>()];    // pairs in the previous list, per row
+    unsigned nmask[CS][cwords<CS>()];   // pairs in the new list
     unsigned bad;                       // a circle sum could not take a value
 };
 
@@ -312,7 +322,7 @@ __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArg
         const int slot = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
         Ln[2 * slot] = make_float4(mA[0], mA[1], mA[2], __int_as_float(idx));
         Ln[2 * slot + 1] = make_float4(mB[0], mB[1], mB[2], 0.0f);
-        atomicOr(&sh.nmask[idx / CS][(idx % CS) >> 5], 1u << (idx & 31));
+        atomicOr(&sh.nmask[idx / CS][(idx % CS) >> 5], 1u << ((idx % CS) & 31));
     }
     base += (int)__popcll(bal);
 }
@@ -425,11 +435,14 @@ __device__ __forceinline__ void contact_unary(const CArgs& a, int P)
 
 // Contact tile t of the iteration (one wavefront, lane = threadIdx.x % 64,
 // sh its own scratch).
-template <int CS, bool RES, bool RE>
+template <int CS, bool RES, bool RE, bool SPATIAL = false>
 __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, ContactWave<CS, RES ? CQ : 3>& sh)
 {
     constexpr int Q = RES ? CQ : 3;
-    constexpr int CSS = CS * CS, RL = CS / 8, W = CS / 32;
+    constexpr int CSS = CS * CS, RL = CS / 8, W = cwords<CS>();
+    // spatial form: the circles' (Z_k, c) come from the launch before, and a
+    // list of length 0 may have a stale mask (its tile was not launched)
+    constexpr bool SP = SPATIAL;
     const int lane = threadIdx.x & 63;
     const int N = a.N, nbs = a.nbs;
     int P, Qb;
@@ -445,7 +458,13 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
     for (int e = lane; e < 2 * CS; e += 64) {
         const int cc = min((e < CS ? P * CS + e : Qb * CS + (e - CS)), N - 1);
         float z[3], c[3];
-        contact_zc<RE>(a, cc, z, c);
+        if constexpr (SP) {
+            const float4 u = a.zcv[2 * cc], v = a.zcv[2 * cc + 1];
+            z[0] = u.x, z[1] = u.y, z[2] = u.z;
+            c[0] = u.w, c[1] = v.x, c[2] = v.y;
+        } else {
+            contact_zc<RE>(a, cc, z, c);
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             sh.zc[e][k] = z[k];
@@ -453,15 +472,17 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
         }
     }
     for (int e = lane; e < CS * W; e += 64) {
-        (&sh.mask[0][0])[e] = a.Lmask[tp * CS * W + e];
+        (&sh.mask[0][0])[e] = (!SP || cp) ? a.Lmask[tp * CS * W + e] : 0u;
         (&sh.nmask[0][0])[e] = 0u;
     }
     for (int k = lane; k < 2 * CS * Q; k += 64) (&sh.acc[0][0])[k] = 0ull;
     if (lane == 0) sh.bad = 0u;
     wave_lds_fence();
     // the first 64 entries of the previous list, in flight during the tests
+    // (spatial form: loaded before the list's length is known, within the
+    // list's capacity; entries past it are not used)
     float4 pre0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pre1 = pre0;
-    if (lane < cp) {
+    if (SP || lane < cp) {
         pre0 = Lp[2 * lane];
         pre1 = Lp[2 * lane + 1];
     }

@@ -42,6 +42,7 @@
 #include "pairtile.h"
 #include "contact_device.h"
 #include "contact_policy.h"
+#include "contact_sparse.h"
 
 #define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hip_fail(e_, #x, __FILE__, __LINE__); } while (0)
 #define RC(x) do { int r_ = (x); if (r_) return r_; } while (0)
@@ -86,6 +87,26 @@ struct DenseState {
     float* d_Xu = nullptr;     // [CUNARY][N] box and radius edges inside the contact state
     float* d_Zpp = nullptr;    // [N][3] Z two iterations back
     unsigned* d_flag = nullptr; // a circle sum lost a value (non-finite or out of range) since the state was entered
+
+    // spatial form of the contact state (contact_sparse.h): circles in the
+    // Morton order of their centres at the entry, tiles only where two
+    // blocks' boxes meet; d_L, d_cnt, d_Lmask, cs, nbs, nts above are the
+    // current form's lists, the other form's wait in c_alt
+    struct ContactLists {
+        float4* L = nullptr;
+        int* cnt = nullptr;
+        unsigned* Lmask = nullptr;
+        int cs = 0, nbs = 0, nts = 0;
+    };
+    ContactLists c_alt;
+    bool c_sparse = false;
+    int* d_perm = nullptr;     // [N] contact index -> circle
+    int* d_inv = nullptr;      // [N] circle -> contact index
+    unsigned long long* d_key = nullptr;  // [N] Morton keys (entry)
+    float4* d_zcv = nullptr;   // [2][N] (Z_k, c) per contact circle
+    float4* d_bb = nullptr;    // [nbs] block boxes
+    int row_waves = 4;         // wavefronts per row workgroup (2, 4, 8 or 16)
+    int row_split = 1;         // workgroups per block row
 
     // learned rho applied inside the contact state (dense_contact_policy)
     bool c_base_ok = false;    // d_base holds every circle's unlisted pair edges' rho
@@ -268,6 +289,271 @@ __global__ __launch_bounds__(256) void k_contact_leave(CArgs a)
 {
     __shared__ ContactLeaveShared<CS> sh;
     contact_leave_tile<CS>(a, (int)blockIdx.x, sh);
+}
+
+// ------------------------------------------------------------------ spatial form of the contact state
+// (contact_sparse.h): three launches per iteration, circles, tile
+// selection, the selected tiles.
+
+template <int CS, bool RES>
+__global__ __launch_bounds__(256) void k_scs_circles(CArgs a)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    float n[3] = {0.0f, 0.0f, 0.0f};
+    const bool valid = c < a.N;
+    if (valid) scs_circle<RES>(a, c, n);
+    scs_block_box<CS>(a, c, valid, n);
+}
+
+// split workgroups of SW wavefronts per block row P: part h lists the
+// row's running tiles (P, Q), Q >= P, Q = P + h mod split, in Q order in
+// LDS (a workgroup prefix over the candidates, SW 64 at a time); part 0
+// also runs block P's box and radius edges; wavefront w takes the listed
+// tiles w, w + SW, ...
+constexpr int SCS_MAXB = 4096;  // blocks of a row (N <= 65,536 circles at tiles of 16)
+
+template <int CS, bool RES, int SCS_WAVES>
+__global__ __launch_bounds__(64 * SCS_WAVES) void k_scs_rows(CArgs a, int split)
+{
+    __shared__ ContactWave<CS, RES ? CQ : 3> sh[SCS_WAVES];
+    __shared__ unsigned short run[SCS_MAXB];
+    __shared__ int wsum[SCS_WAVES];
+    // split workgroups share a row: part h takes the candidates Q = P + h mod split
+    const int P = (int)blockIdx.x / split, h = (int)blockIdx.x % split, nbs = a.nbs;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const float4 bp = a.bb[P];
+    int total = 0;
+    for (int q0 = P + h; q0 < nbs; q0 += 64 * SCS_WAVES * split) {
+        const int Qb = q0 + tid * split;
+        const bool r = Qb < nbs && scs_tile_runs(a, bp, P, Qb);
+        const unsigned long long b = __ballot(r);
+        if (lane == 0) wsum[w] = (int)__popcll(b);
+        __syncthreads();
+        int base = total, all = 0;
+#pragma unroll
+        for (int k = 0; k < SCS_WAVES; ++k) {
+            base += k < w ? wsum[k] : 0;
+            all += wsum[k];
+        }
+        if (r) run[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = (unsigned short)Qb;
+        total += all;
+        __syncthreads();
+    }
+    // the box and radius edges on a wavefront with one tile fewer than the rest, if any
+    if (h == 0 && w == (total % SCS_WAVES)) scs_unary<CS, RES>(a, P);
+    for (int k = w; k < total; k += SCS_WAVES) {
+        const int Qb = run[k];
+        contact_tile_body<CS, RES, false, true>(a, P * nbs - P * (P - 1) / 2 + (Qb - P), sh[w]);
+        wave_lds_fence();  // the next tile reuses the scratch
+    }
+}
+
+template <bool RES>
+__global__ __launch_bounds__(CZ_THREADS) void k_scs_zreduce(CArgs a)
+{
+    const int c = blockIdx.x * CZ_THREADS + threadIdx.x;
+    if (c < a.N) scs_zreduce_circle<RES>(a, c);
+}
+
+// Morton key of each circle's centre, ties broken by the circle index
+__global__ __launch_bounds__(256) void k_scs_keys(CArgs a, unsigned long long* key)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < a.N) key[i] = ((unsigned long long)scs_morton(a.Z[3 * i], a.Z[3 * i + 1]) << 32) | (unsigned)i;
+}
+
+// contact order: rank of each key among all N (keys are distinct), by
+// counting the smaller ones (O(N^2) compares, once per entry)
+__global__ __launch_bounds__(256) void k_scs_rank(int N, const unsigned long long* __restrict__ key, int* perm,
+                                                  int* inv)
+{
+    __shared__ unsigned long long ks[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const unsigned long long my = i < N ? key[i] : ~0ull;
+    int r = 0;
+    for (int b = 0; b < N; b += 256) {
+        __syncthreads();
+        ks[threadIdx.x] = b + (int)threadIdx.x < N ? key[b + threadIdx.x] : ~0ull;
+        __syncthreads();
+        const int m = min(256, N - b);
+        for (int q = 0; q < m; ++q) r += ks[q] < my ? 1 : 0;
+    }
+    if (i < N) {
+        perm[r] = i;
+        inv[i] = r;
+    }
+}
+
+// Entry from the steady X, U state: every pair with M = X + U != Z_prev on
+// either edge (the test of the id-order entry) goes to list 0 of its
+// contact tile, appended by atomics (the order within a list changes no
+// value); one workgroup per dense tile, a wavefront along a dense row.
+template <int CS>
+__global__ __launch_bounds__(256) void k_scs_enter(CArgs a)
+{
+    constexpr int W = cwords<CS>();
+    const int T = a.T, N = a.N;
+    int I, J;
+    tri_tile((int)blockIdx.x, a.nb, I, J);
+    const long S = a.stride, TT = (long)T * T, t0 = (long)blockIdx.x * 2 * TT;
+    const int jl = (int)threadIdx.x % T, j = J * T + jl;
+    if (j >= N) return;
+    const int jc = a.inv[j];
+    float zpj[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zpj[k] = a.Zp[3 * j + k];
+    for (int il = (int)threadIdx.x / T; il < T; il += 256 / T) {
+        const int i = I * T + il;
+        if (i >= N || (I == J && il >= jl)) continue;
+        const long pa = t0 + (long)il * T + jl, pb = pa + TT;
+        float mA[3], mB[3];
+        bool ex = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            mA[k] = a.X[k * S + pa] + a.U[k * S + pa];
+            mB[k] = a.X[k * S + pb] + a.U[k * S + pb];
+            ex = ex || mA[k] != a.Zp[3 * i + k] || mB[k] != zpj[k];
+        }
+        if (!ex) continue;
+        const int ic = a.inv[i];
+        const bool f = ic < jc;  // circle i is the tile's row circle
+        int ct, idx;
+        scs_tile_of(f ? ic : jc, f ? jc : ic, CS, a.nbs, ct, idx);
+        const int slot = atomicAdd(&a.cnt[ct], 1);
+        float4* L0 = a.L + (long)ct * (2 * CS * CS);
+        L0[2 * slot] = f ? make_float4(mA[0], mA[1], mA[2], __int_as_float(idx))
+                         : make_float4(mB[0], mB[1], mB[2], __int_as_float(idx));
+        L0[2 * slot + 1] = f ? make_float4(mB[0], mB[1], mB[2], 0.0f) : make_float4(mA[0], mA[1], mA[2], 0.0f);
+        atomicOr(&a.Lmask[(long)ct * CS * W + (idx / CS) * W + ((idx % CS) >> 5)], 1u << ((idx % CS) & 31));
+    }
+}
+
+// Entry: the box and radius edges into the compact array, in contact order
+__global__ __launch_bounds__(256) void k_scs_enter_unary(CArgs a)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= a.N) return;
+    const long N = a.N, S = a.stride, TT = (long)a.T * a.T;
+    const long q0 = dense_unary_slot(a.perm[c], a.nb, a.T);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const long q = q0 + side * TT;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            a.Xu[(6 * side + k) * N + c] = a.X[k * S + q];
+            a.Xu[(6 * side + 3 + k) * N + c] = a.U[k * S + q];
+        }
+        a.Xu[(12 + side) * N + c] = a.rho[q];
+    }
+}
+
+// Leave, first pass: every pair edge as if in no list (U_k = Z_{k-1} - Z_k,
+// X_k = Z_k - U_k, the id-order leave's values for such a pair) and the box
+// and radius edges back to their slots; one workgroup per dense tile.
+__global__ __launch_bounds__(256) void k_scs_leave_dense(CArgs a)
+{
+    const int T = a.T, N = a.N;
+    int I, J;
+    tri_tile((int)blockIdx.x, a.nb, I, J);
+    const long S = a.stride, TT = (long)T * T, t0 = (long)blockIdx.x * 2 * TT;
+    const int jl = (int)threadIdx.x % T, j = J * T + jl;
+    if (I == J && (int)threadIdx.x < T) {
+        const int c = I * T + (int)threadIdx.x;
+        if (c < N) {
+            const int cc = a.inv[c];
+            const long q0 = dense_unary_slot(c, a.nb, T);
+#pragma unroll
+            for (int side = 0; side < 2; ++side)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    a.X[k * S + q0 + side * TT] = a.Xu[(6 * side + k) * (long)N + cc];
+                    a.U[k * S + q0 + side * TT] = a.Xu[(6 * side + 3 + k) * (long)N + cc];
+                }
+        }
+    }
+    if (j >= N) return;
+    float uj[3], xj[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float zk = a.Zp[3 * j + k];
+        uj[k] = a.Zpp[3 * j + k] - zk;
+        xj[k] = zk - uj[k];
+    }
+    for (int il = (int)threadIdx.x / T; il < T; il += 256 / T) {
+        const int i = I * T + il;
+        if (i >= N || (I == J && il >= jl)) continue;
+        const long pa = t0 + (long)il * T + jl, pb = pa + TT;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float zk = a.Zp[3 * i + k];
+            const float u = a.Zpp[3 * i + k] - zk;
+            a.U[k * S + pa] = u;
+            a.X[k * S + pa] = zk - u;
+            a.U[k * S + pb] = uj[k];
+            a.X[k * S + pb] = xj[k];
+        }
+    }
+}
+
+// Leave, second pass: the pairs of the last two lists (U_k = M_{k-1} - Z_k
+// from the older, X_k = M_k - U_k from the newer) over the first pass's
+// values; one workgroup per contact tile.
+template <int CS>
+__global__ __launch_bounds__(256) void k_scs_leave_lists(CArgs a)
+{
+    constexpr int CSS = CS * CS, W = cwords<CS>();
+    __shared__ unsigned short map[2][CSS];
+    __shared__ unsigned mask[2][CS][W];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const long lt[2] = {(long)a.par * a.nts + t, (long)(a.par ^ 1) * a.nts + t};
+    const int cn[2] = {min(a.cnt[lt[0]], CSS), min(a.cnt[lt[1]], CSS)};
+    if (cn[0] == 0 && cn[1] == 0) return;  // the same for the whole workgroup
+    const float4* Ls[2] = {a.L + lt[0] * (2 * CSS), a.L + lt[1] * (2 * CSS)};
+    for (int e = tid; e < 2 * CS * W; e += 256) (&mask[0][0][0])[e] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+        for (int e = tid; e < cn[b]; e += 256) {
+            const int idx = __float_as_int(Ls[b][2 * e].w) & (CSS - 1);
+            map[b][idx] = (unsigned short)e;
+            atomicOr(&mask[b][idx / CS][(idx % CS) >> 5], 1u << ((idx % CS) & 31));
+        }
+    __syncthreads();
+    int P, Qb;
+    tri_tile(t, a.nbs, P, Qb);
+    const long S = a.stride;
+    for (int idx = tid; idx < CSS; idx += 256) {
+        const int il = idx / CS, jl = idx % CS;
+        const bool in_k = (mask[0][il][jl >> 5] >> (jl & 31)) & 1u;
+        const bool in_k1 = (mask[1][il][jl >> 5] >> (jl & 31)) & 1u;
+        if (!in_k && !in_k1) continue;
+        const int gi = a.perm[P * CS + il], gj = a.perm[Qb * CS + jl];
+        long qi, qj;
+        scs_pair_slots(gi, gj, a.nb, a.T, qi, qj);
+        float4 ek[2] = {}, ek1[2] = {};
+        if (in_k) {
+            ek[0] = Ls[0][2 * map[0][idx]];
+            ek[1] = Ls[0][2 * map[0][idx] + 1];
+        }
+        if (in_k1) {
+            ek1[0] = Ls[1][2 * map[1][idx]];
+            ek1[1] = Ls[1][2 * map[1][idx] + 1];
+        }
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int v = side == 0 ? gi : gj;
+            const long q = side == 0 ? qi : qj;
+            const float mk[3] = {ek[side].x, ek[side].y, ek[side].z};
+            const float mk1[3] = {ek1[side].x, ek1[side].y, ek1[side].z};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float zk = a.Zp[3 * v + k];
+                const float u = (in_k1 ? mk1[k] : a.Zpp[3 * v + k]) - zk;
+                a.U[k * S + q] = u;
+                a.X[k * S + q] = (in_k ? mk[k] : zk) - u;
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ learned rho in the contact state
@@ -584,6 +870,12 @@ static CArgs cargs(DevGraph* s)
     a.Rd = d->d_Rd;
     a.flag = d->d_flag;
     a.Xu = d->d_Xu;
+    if (d->c_sparse) {
+        a.perm = d->d_perm;
+        a.inv = d->d_inv;
+        a.zcv = d->d_zcv;
+        a.bb = d->d_bb;
+    }
     return a;
 }
 
@@ -614,19 +906,88 @@ static int contact_tile_side(int N)
     return N >= 2500 ? 64 : 32;
 }
 
+// The spatial form: from N = 4500 circles (below, the id-order form's one
+// launch per iteration is faster than two), with one rho, alpha = 1 and
+// no rho policy (its application runs in the id-order form).
+// PARADMM_CONTACT_SPARSE = 0 or 1 overrides the size rule.
+static bool contact_sparse_wanted(DevGraph* s, bool rho_edge)
+{
+    const DenseState* d = s->dense;
+    if (rho_edge || s->rho_policy) return false;
+    if ((d->N + 15) / 16 > SCS_MAXB) return false;  // a row's blocks in LDS
+    if (const char* e = getenv("PARADMM_CONTACT_SPARSE")) return e[0] == '1';
+    return d->N >= 4500;
+}
+
+// Spatial tile side: 16 (PARADMM_SPARSE_TILE = 32 overrides): a block of
+// 16 circles covers a patch small enough that a circle meets few blocks.
+static int sparse_tile_side()
+{
+    if (const char* e = getenv("PARADMM_SPARSE_TILE"))
+        if (atoi(e) == 32) return 32;
+    return 16;
+}
+
+// the lists of the form wanted from here (the other form's are kept)
+static void contact_select_form(DenseState* d, bool sparse)
+{
+    if (sparse == d->c_sparse) return;
+    DenseState::ContactLists cur;
+    cur.L = d->d_L;
+    cur.cnt = d->d_cnt;
+    cur.Lmask = d->d_Lmask;
+    cur.cs = d->cs;
+    cur.nbs = d->nbs;
+    cur.nts = d->nts;
+    d->d_L = d->c_alt.L;
+    d->d_cnt = d->c_alt.cnt;
+    d->d_Lmask = d->c_alt.Lmask;
+    d->cs = d->c_alt.cs;
+    d->nbs = d->c_alt.nbs;
+    d->nts = d->c_alt.nts;
+    d->c_alt = cur;
+    d->c_sparse = sparse;
+}
+
 static int contact_alloc(DevGraph* s)
 {
     DenseState* d = s->dense;
+    if (d->c_sparse && !d->d_perm) {
+        const size_t N = (size_t)d->N;
+        RC(dmalloc(s, (void**)&d->d_perm, sizeof(int) * N));
+        RC(dmalloc(s, (void**)&d->d_inv, sizeof(int) * N));
+        RC(dmalloc(s, (void**)&d->d_key, sizeof(unsigned long long) * N));
+        RC(dmalloc(s, (void**)&d->d_zcv, sizeof(float4) * 2 * N));
+    }
     if (d->d_L) return PARADMM_OK;
-    d->cs = contact_tile_side(d->N);
+    d->cs = d->c_sparse ? sparse_tile_side() : contact_tile_side(d->N);
     d->nbs = (d->N + d->cs - 1) / d->cs;
     d->nts = d->nbs * (d->nbs + 1) / 2;
-    const size_t nts = (size_t)d->nts, css = (size_t)d->cs * d->cs, words = (size_t)d->cs * (d->cs / 32);
+    if (d->c_sparse) {
+        RC(dmalloc(s, (void**)&d->d_bb, sizeof(float4) * (size_t)d->nbs));
+        // Workgroups of 4 wavefronts, each block row split over
+        // max(3, 1024 / nbs) of them (at most 8): the uneven rows are
+        // shared out better than by one workgroup per row (ms per
+        // iteration, N = 5000: one row of 8 wavefronts per workgroup
+        // 0.0190, split 3 ways 0.0163; N = 8000: 0.0337 unsplit, 0.0234
+        // split 3 ways; N = 10000: 0.0377, 0.0269; N = 3000: split 5 ways
+        // 0.0134, 3 ways 0.0159).  PARADMM_SPARSE_WAVES (2, 4, 8, 16) and
+        // PARADMM_SPARSE_SPLIT (1-8) override.
+        const char* e = getenv("PARADMM_SPARSE_WAVES");
+        const int wv = e ? atoi(e) : 4;
+        d->row_waves = wv == 2 || wv == 8 || wv == 16 ? wv : 4;
+        e = getenv("PARADMM_SPARSE_SPLIT");
+        const int sp = e ? atoi(e) : (1024 / d->nbs > 3 ? 1024 / d->nbs : 3);
+        d->row_split = sp < 1 ? 1 : sp > 8 ? 8 : sp;
+    }
+    const size_t nts = (size_t)d->nts, css = (size_t)d->cs * d->cs,
+                 words = (size_t)d->cs * (d->cs >= 32 ? d->cs / 32 : 1);
     RC(dmalloc(s, (void**)&d->d_L, sizeof(float4) * 2 * 2 * nts * css));
     RC(dmalloc(s, (void**)&d->d_cnt, sizeof(int) * 2 * nts));
     RC(dmalloc(s, (void**)&d->d_Lmask, sizeof(unsigned) * 2 * nts * words));
     CK(hipMemsetAsync(d->d_cnt, 0, sizeof(int) * 2 * nts, s->stream));
     CK(hipMemsetAsync(d->d_Lmask, 0, sizeof(unsigned) * 2 * nts * words, s->stream));
+    if (d->d_Ac) return PARADMM_OK;  // the per-circle arrays are shared by both forms
     RC(dmalloc(s, (void**)&d->d_Ac, sizeof(unsigned long long) * 3 * CQ * (size_t)d->N));
     CK(hipMemsetAsync(d->d_Ac, 0, sizeof(unsigned long long) * 3 * CQ * (size_t)d->N, s->stream));
     RC(dmalloc(s, (void**)&d->d_Zb, sizeof(float) * 2 * 3 * (size_t)d->N));
@@ -640,15 +1001,15 @@ static int contact_alloc(DevGraph* s)
 }
 
 // res: also the residual sums (the last iteration of a run)
-template <typename K>
-static void contact_launch(DevGraph* s, const char* name, K kernel, unsigned grid, unsigned block, const CArgs& a)
+template <typename K, typename... Args>
+static void contact_launch(DevGraph* s, const char* name, K kernel, unsigned grid, unsigned block, const Args&... a)
 {
     hipEvent_t e0, e1;
     prof_dispatch(s, name, e0, e1);
     if (e0)  // timed, eager
-        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, e0, e1, 0, a);
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, e0, e1, 0, a...);
     else
-        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, a);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, a...);
 }
 
 // Launch j of a run (j = c_run launches since the last run end) applies the
@@ -658,9 +1019,67 @@ static void contact_launch(DevGraph* s, const char* name, K kernel, unsigned gri
 // single iteration): also the residual sums, then k_contact_zreduce ends the
 // run into the canonical Z arrays, so outside a run the state is as it
 // always was.
+// The spatial form's launch j of a run: the circles (Z_k from the sums of
+// launch j - 1 in buffer (j - 1) % 2, stored to ring slot j % 2; buffer
+// j % 2 cleared), then the rows (the box and radius edges and the running
+// tiles add to buffer j % 2); res: then k_scs_zreduce ends the run into
+// the canonical arrays and clears both buffers.
+static int sparse_iteration(DevGraph* s, bool res)
+{
+    DenseState* d = s->dense;
+    const int j = d->c_run;
+    const size_t AQ = (size_t)CQ * d->N, ZN = 3 * (size_t)d->N;
+    unsigned long long* A = d->d_Ac;
+    CArgs a = cargs(s);
+    a.Ac = A + (j % 2) * AQ;
+    a.Ap = j ? A + ((j - 1) % 2) * AQ : nullptr;
+    a.Zin = j ? d->d_Zb + ((j - 1) % 2) * ZN : nullptr;
+    a.Zo = d->d_Zb + (j % 2) * ZN;
+    const unsigned cg = (unsigned)((d->N + 255) / 256), rg = (unsigned)(d->nbs * d->row_split);
+    const char* cn = res ? "scs_circles_res" : "scs_circles";
+    const char* rn = res ? "scs_rows_res" : "scs_rows";
+#define SCS_LAUNCH(CS_, W_)                                                                           \
+    do {                                                                                              \
+        res ? contact_launch(s, cn, k_scs_circles<CS_, true>, cg, 256, a)                             \
+            : contact_launch(s, cn, k_scs_circles<CS_, false>, cg, 256, a);                           \
+        CK(hipGetLastError());                                                                        \
+        res ? contact_launch(s, rn, k_scs_rows<CS_, true, W_>, rg, 64 * (W_), a, d->row_split)        \
+            : contact_launch(s, rn, k_scs_rows<CS_, false, W_>, rg, 64 * (W_), a, d->row_split);      \
+    } while (0)
+    if (d->cs == 16) {
+        if (d->row_waves == 16) SCS_LAUNCH(16, 16);
+        else if (d->row_waves == 8) SCS_LAUNCH(16, 8);
+        else if (d->row_waves == 2) SCS_LAUNCH(16, 2);
+        else SCS_LAUNCH(16, 4);
+    } else {
+        if (d->row_waves == 16) SCS_LAUNCH(32, 16);
+        else if (d->row_waves == 8) SCS_LAUNCH(32, 8);
+        else if (d->row_waves == 2) SCS_LAUNCH(32, 2);
+        else SCS_LAUNCH(32, 4);
+    }
+#undef SCS_LAUNCH
+    CK(hipGetLastError());
+    d->c_par ^= 1;
+    if (!res) {
+        d->c_run = j + 1;
+        return PARADMM_OK;
+    }
+    CArgs f = cargs(s);
+    f.Ac = A + (j % 2) * AQ;
+    f.Acl = A + ((j + 1) % 2) * AQ;
+    f.Zin = d->d_Zb + (j % 2) * ZN;
+    f.Zpin = j ? d->d_Zb + ((j - 1) % 2) * ZN : nullptr;
+    const unsigned zgrid = (unsigned)((d->N + CZ_THREADS - 1) / CZ_THREADS);
+    contact_launch(s, "scs_zreduce_res", k_scs_zreduce<true>, zgrid, CZ_THREADS, f);
+    CK(hipGetLastError());
+    d->c_run = 0;
+    return PARADMM_OK;
+}
+
 static int contact_iteration(DevGraph* s, bool res)
 {
     DenseState* d = s->dense;
+    if (d->c_sparse) return sparse_iteration(s, res);
     const int j = d->c_run;
     const size_t AQ = (size_t)CQ * d->N, ZN = 3 * (size_t)d->N;
     unsigned long long* A = d->d_Ac;
@@ -794,6 +1213,9 @@ int dense_contact_enter(DevGraph* s)
                                          "and alpha = 1");
         return PARADMM_ERR_STATE;
     }
+    // one rho everywhere (a policy's raised edges make it per edge)
+    const bool rho_edge = !(s->rho_uni && s->policy_raised == 0);
+    contact_select_form(d, contact_sparse_wanted(s, rho_edge));
     RC(contact_alloc(s));
     d->c_par = 0;  // the entry writes list 0
     d->c_run = 0;
@@ -801,15 +1223,35 @@ int dense_contact_enter(DevGraph* s)
     // as the entry may run inside a captured stream)
     k_fill_u32<<<1, 64, 0, s->stream>>>(d->d_flag, 1, 0u);
     CK(hipGetLastError());
-    // one rho everywhere (a policy's raised edges make it per edge)
-    d->c_rhoe = !(s->rho_uni && s->policy_raised == 0);
-    // with one rho it is every circle's base rho (learned rho in the state)
-    d->c_base_ok = !d->c_rhoe;
+    d->c_rhoe = rho_edge;
+    // with one rho it is every circle's base rho (learned rho in the
+    // id-order state; the spatial form takes no policy application)
+    d->c_base_ok = !d->c_rhoe && !d->c_sparse;
     if (d->c_base_ok) {
         k_fill_f32<<<(unsigned)((d->N + 255) / 256), 256, 0, s->stream>>>(d->d_base, d->N, d->d_uni);
         CK(hipGetLastError());
     }
     const CArgs a = cargs(s);
+    if (d->c_sparse) {
+        const unsigned cg = (unsigned)((d->N + 255) / 256);
+        const size_t words = (size_t)d->nts * d->cs * (d->cs >= 32 ? d->cs / 32 : 1);
+        prof_begin(s, "scs_enter");
+        k_scs_keys<<<cg, 256, 0, s->stream>>>(a, d->d_key);
+        k_scs_rank<<<cg, 256, 0, s->stream>>>(d->N, d->d_key, d->d_perm, d->d_inv);
+        // both lists empty (list 0 is filled below), list 0's masks clear
+        k_fill_u32<<<(unsigned)((2 * (size_t)d->nts + 255) / 256), 256, 0, s->stream>>>(
+            (unsigned*)d->d_cnt, 2 * d->nts, 0u);
+        k_fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, s->stream>>>(d->d_Lmask, (int)words, 0u);
+        CK(hipGetLastError());
+        const CArgs b = cargs(s);  // with the contact order
+        if (d->cs == 16) k_scs_enter<16><<<(unsigned)d->nt, 256, 0, s->stream>>>(b);
+        else k_scs_enter<32><<<(unsigned)d->nt, 256, 0, s->stream>>>(b);
+        k_scs_enter_unary<<<cg, 256, 0, s->stream>>>(b);
+        prof_end(s);
+        CK(hipGetLastError());
+        d->c_state = true;
+        return contact_iteration(s, true);
+    }
     if (d->c_rhoe) {
         prof_begin(s, "contact_weights");
         k_contact_weights<<<(unsigned)d->N, 64, 0, s->stream>>>(a);
@@ -831,6 +1273,15 @@ int dense_contact_leave(DevGraph* s)
     if (!d || !d->c_state) return PARADMM_OK;
     const CArgs a = cargs(s);
     prof_begin(s, "contact_leave");
+    if (d->c_sparse) {
+        k_scs_leave_dense<<<(unsigned)d->nt, 256, 0, s->stream>>>(a);
+        if (d->cs == 16) k_scs_leave_lists<16><<<(unsigned)d->nts, 256, 0, s->stream>>>(a);
+        else k_scs_leave_lists<32><<<(unsigned)d->nts, 256, 0, s->stream>>>(a);
+        prof_end(s);
+        CK(hipGetLastError());
+        d->c_state = false;
+        return PARADMM_OK;
+    }
     if (d->cs == 64) k_contact_leave<64><<<(unsigned)d->nts, 256, 0, s->stream>>>(a);
     else k_contact_leave<32><<<(unsigned)d->nts, 256, 0, s->stream>>>(a);
     prof_end(s);

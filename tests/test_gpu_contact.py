@@ -19,18 +19,31 @@ pytestmark = pytest.mark.gpu
 STATE = ("X", "M", "N", "U", "Z")
 
 
-@pytest.fixture(autouse=True, params=["32", "64"])
+@pytest.fixture(autouse=True, params=["32", "64", "spatial16"])
 def contact_on(monkeypatch, request):
-    """Every test on both contact tile sides (the default picks 32 below
-    N = 2500)."""
+    """Every test on both contact tile sides of the id-order form (the
+    default picks 32 below N = 2500) and on the spatial form (circles in
+    Morton order, tiles of 16 only where blocks meet; the default from
+    N = 4500 when no rho policy is attached)."""
     monkeypatch.delenv("PARADMM_LAYOUT", raising=False)
     monkeypatch.delenv("PARADMM_CONTACT", raising=False)
     monkeypatch.delenv("PARADMM_DENSE_TILE", raising=False)
-    monkeypatch.setenv("PARADMM_CONTACT_TILE", request.param)
+    if request.param.startswith("spatial"):
+        monkeypatch.delenv("PARADMM_CONTACT_TILE", raising=False)
+        monkeypatch.setenv("PARADMM_CONTACT_SPARSE", "1")
+        monkeypatch.setenv("PARADMM_SPARSE_TILE", request.param[len("spatial"):])
+    else:
+        monkeypatch.setenv("PARADMM_CONTACT_SPARSE", "0")
+        monkeypatch.setenv("PARADMM_CONTACT_TILE", request.param)
 
 
 def _kernels(g, iters):
-    return set(g.profile_kernels(iters))
+    """Kernel names of iters eager iterations; the spatial form's tile
+    kernel counts as the contact tile kernel."""
+    k = set(g.profile_kernels(iters))
+    if any(n.startswith("scs_rows") for n in k):
+        k.add("contact_tile")
+    return k
 
 
 @pytest.mark.parametrize("fixture", ["square20_seed0.npz", "square20_seed0_rho100.npz"])
