@@ -312,8 +312,14 @@ __global__ __launch_bounds__(256) void k_scs_circles(CArgs a)
 // tiles w, w + SW, ...
 constexpr int SCS_MAXB = 4096;  // blocks of a row (N <= 65,536 circles at tiles of 16)
 
+// minimum waves per SIMD requested for the row kernel (register budget)
+#ifndef PDM_SCS_WAVES_PER_EU
+#define PDM_SCS_WAVES_PER_EU 4
+#endif
+
 template <int CS, bool RES, int SCS_WAVES>
-__global__ __launch_bounds__(64 * SCS_WAVES) void k_scs_rows(CArgs a, int split)
+__global__ __launch_bounds__(64 * SCS_WAVES) __attribute__((amdgpu_waves_per_eu(PDM_SCS_WAVES_PER_EU))) void k_scs_rows(
+    CArgs a, int split)
 {
     __shared__ ContactWave<CS, RES ? CQ : 3> sh[SCS_WAVES];
     __shared__ unsigned short run[SCS_MAXB];
