@@ -284,12 +284,12 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
 }
 
 // One round of the tile's pair stream: items d .. d + 63 (item < cp: entry
-// of the previous list, else ring slot item - cp); the active pairs are
-// appended to the new list in lane order.
+// of the previous list, (pa, pb) loaded by the caller, else ring slot
+// item - cp); the active pairs are appended to the new list in lane order.
 template <int CS, int Q, bool RE>
 __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArgs& a, int P, int Qb, int d, int avail,
-                                              int cp, const float4* Lp, float4 pre0, float4 pre1, float4* Ln,
-                                              int& base, bool diag, float rho)
+                                              int cp, float4 pa, float4 pb, float4* Ln, int& base, bool diag,
+                                              float rho)
 {
     constexpr int CSS = CS * CS;
     const int lane = threadIdx.x & 63;
@@ -299,12 +299,7 @@ __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArg
     float mA[3], mB[3];
     if (item < avail) {
         const bool ex = item < cp;
-        float4 pa = pre0, pb = pre1;
         if (ex) {
-            if (d > 0) {
-                pa = Lp[2 * item];
-                pb = Lp[2 * item + 1];
-            }
             idx = __float_as_int(pa.w) & (CSS - 1);
         } else {
             idx = sh.ring[(item - cp) & (contact_ring<CS>() - 1)];
@@ -497,10 +492,24 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
     int base = 0;           // entries of the new list so far
     int done = 0;           // stream items taken through the prox
     int avail = cp;         // stream items so far (previous list + ring)
-    while (avail - done >= 64) {
-        contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
+    // the previous list's entries of the round at `done` (cur) and of the
+    // one after it (nxt, in flight while the round runs)
+    float4 cur0 = pre0, cur1 = pre1, nxt0 = pre0, nxt1 = pre1;
+    auto next_entries = [&]() {
+        const int it = done + 64 + lane;
+        if (done + 64 < cp && it < cp) {
+            nxt0 = Lp[2 * it];
+            nxt1 = Lp[2 * it + 1];
+        }
+    };
+    auto take_round = [&]() {
+        next_entries();
+        contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, cur0, cur1, Ln, base, diag, rho);
         done += 64;
-    }
+        cur0 = nxt0;
+        cur1 = nxt1;
+    };
+    while (avail - done >= 64) take_round();
 #pragma unroll 1
     for (int h = 0; h < RL / 2; ++h) {
         // the tests of rows r0 + 2h, r0 + 2h + 1
@@ -547,12 +556,9 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
             sh.ring[(off++) & (contact_ring<CS>() - 1)] = (unsigned short)((r0 + 2 * h + b / RL) * CS + c0 + b % RL);
         }
         wave_lds_fence();
-        while (avail - done >= 64) {
-            contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
-            done += 64;
-        }
+        while (avail - done >= 64) take_round();
     }
-    if (avail > done) contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, Lp, pre0, pre1, Ln, base, diag, rho);
+    if (avail > done) contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, cur0, cur1, Ln, base, diag, rho);
     if (lane == 0) a.cnt[tn] = base;
 
     wave_lds_fence();
