@@ -644,13 +644,13 @@ __device__ __forceinline__ void contact_zreduce_circle(const CArgs& a, const int
 template <int CS>
 struct ContactEnterShared {
     int wsum[4];
-    unsigned emask[CS][CS / 32];
+    unsigned emask[CS][cwords<CS>()];
 };
 
 template <int CS>
 __device__ __forceinline__ void contact_enter_tile(const CArgs& a, const int t, ContactEnterShared<CS>& sh)
 {
-    constexpr int CSS = CS * CS, W = CS / 32, RS = 256 / CS, CPT = CSS / 256;
+    constexpr int CSS = CS * CS, W = cwords<CS>(), RS = 256 / CS, CPT = CSS / 256;
     int(&wsum)[4] = sh.wsum;
     unsigned(&emask)[CS][W] = sh.emask;
     const int tid = threadIdx.x;
@@ -733,13 +733,13 @@ __device__ __forceinline__ void contact_enter_tile(const CArgs& a, const int t, 
 template <int CS>
 struct ContactLeaveShared {
     unsigned short map[2][CS * CS];
-    unsigned mask[2][CS][CS / 32];
+    unsigned mask[2][CS][cwords<CS>()];
 };
 
 template <int CS>
 __device__ __forceinline__ void contact_leave_tile(const CArgs& a, const int t, ContactLeaveShared<CS>& sh)
 {
-    constexpr int CSS = CS * CS, W = CS / 32, RS = 256 / CS, CPT = CSS / 256;
+    constexpr int CSS = CS * CS, W = cwords<CS>(), RS = 256 / CS, CPT = CSS / 256;
     unsigned short(&map)[2][CSS] = sh.map;
     unsigned(&mask)[2][CS][W] = sh.mask;
     const int tid = threadIdx.x;
@@ -757,7 +757,7 @@ __device__ __forceinline__ void contact_leave_tile(const CArgs& a, const int t, 
         for (int e = tid; e < cn[b]; e += 256) {
             const int idx = __float_as_int(Ls[b][2 * e].w) & (CSS - 1);
             map[b][idx] = (unsigned short)e;
-            atomicOr(&mask[b][idx / CS][(idx % CS) >> 5], 1u << (idx & 31));
+            atomicOr(&mask[b][idx / CS][(idx % CS) >> 5], 1u << ((idx % CS) & 31));
         }
     __syncthreads();
     const int jl = tid % CS, r0 = tid / CS;

@@ -115,7 +115,7 @@ struct CPolShared {
     float yv[4][64];
     float zc[2 * CS][9];             // Z, Z_prev, Z_prev2 of the tile's circles
     unsigned short omap[CS * CS];    // other list: pair -> entry
-    unsigned omask[CS][CS / 32], pmask[CS][CS / 32], rmask[CS][CS / 32];
+    unsigned omask[CS][cwords<CS>()], pmask[CS][cwords<CS>()], rmask[CS][cwords<CS>()];
     unsigned short q[CS * CS];       // the tile's pairs to evaluate: kind << 14 | entry or pair
     int wsum[4];
 };
@@ -134,7 +134,7 @@ struct CPolShared {
 template <int CS, bool APPLY>
 __device__ __forceinline__ void cpol_tile_body(const CPolArgs& p, const int t, CPolShared<CS>& sh)
 {
-    constexpr int CSS = CS * CS, W = CS / 32;
+    constexpr int CSS = CS * CS, W = cwords<CS>();
     MlpShared& w = sh.w;
     TileScratch(&ts)[4] = sh.ts;
     float(&yv)[4][64] = sh.yv;
@@ -175,7 +175,7 @@ __device__ __forceinline__ void cpol_tile_body(const CPolArgs& p, const int t, C
     for (int e = tid; e < co; e += 256) omap[__float_as_int(Lo[2 * e].w) & (CSS - 1)] = (unsigned short)e;
     for (int e = tid; e < cp; e += 256) q[e] = (unsigned short)e;
     __syncthreads();
-    auto inmask = [&](const unsigned (&m)[CS][W], int idx) { return (m[idx / CS][(idx % CS) >> 5] >> (idx & 31)) & 1u; };
+    auto inmask = [&](const unsigned (&m)[CS][W], int idx) { return (m[idx / CS][(idx % CS) >> 5] >> ((idx % CS) & 31)) & 1u; };
     // the queue: par entries, other-only entries, off-base unlisted pairs
     int n = cp;
     for (int e0 = 0; e0 < co; e0 += 256) {
@@ -201,7 +201,7 @@ __device__ __forceinline__ void cpol_tile_body(const CPolArgs& p, const int t, C
             const unsigned code = q[e];
             const int k2 = (int)(code >> 14), en = (int)(code & 0x3fffu);
             const int idx = k2 == 2 ? en : __float_as_int((k2 == 0 ? Lp : Lo)[2 * en].w) & (CSS - 1);
-            atomicOr(&rmask[idx / CS][(idx % CS) >> 5], 1u << (idx & 31));
+            atomicOr(&rmask[idx / CS][(idx % CS) >> 5], 1u << ((idx % CS) & 31));
         }
         __syncthreads();
     }
@@ -277,7 +277,7 @@ __device__ __forceinline__ void cpol_tile_body(const CPolArgs& p, const int t, C
                 if (side == 0) atomicOr(a.flag, 1u);
             } else if (app) {
                 Lp[2 * at + side] = make_float4(mn[0], mn[1], mn[2], side ? 0.0f : __int_as_float(idx));
-                if (side == 0) atomicOr(&a.Lmask[tp * CS * W + (idx / CS) * W + ((idx % CS) >> 5)], 1u << (idx & 31));
+                if (side == 0) atomicOr(&a.Lmask[tp * CS * W + (idx / CS) * W + ((idx % CS) >> 5)], 1u << ((idx % CS) & 31));
             }
         }
     }
