@@ -284,12 +284,13 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
 }
 
 // One round of the tile's pair stream: items d .. d + 63 (item < cp: entry
-// of the previous list, (pa, pb) loaded by the caller, else ring slot
-// item - cp); the active pairs are appended to the new list in lane order.
+// of the previous list, (pa, pb) loaded by the caller or, with Lp, here;
+// else ring slot item - cp); the active pairs are appended to the new list
+// in lane order.
 template <int CS, int Q, bool RE>
 __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArgs& a, int P, int Qb, int d, int avail,
-                                              int cp, float4 pa, float4 pb, float4* Ln, int& base, bool diag,
-                                              float rho)
+                                              int cp, const float4* Lp, float4 pa, float4 pb, float4* Ln, int& base,
+                                              bool diag, float rho)
 {
     constexpr int CSS = CS * CS;
     const int lane = threadIdx.x & 63;
@@ -300,6 +301,10 @@ __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArg
     if (item < avail) {
         const bool ex = item < cp;
         if (ex) {
+            if (Lp) {  // not loaded by the caller
+                pa = Lp[2 * item];
+                pb = Lp[2 * item + 1];
+            }
             idx = __float_as_int(pa.w) & (CSS - 1);
         } else {
             idx = sh.ring[(item - cp) & (contact_ring<CS>() - 1)];
@@ -492,8 +497,11 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
     int base = 0;           // entries of the new list so far
     int done = 0;           // stream items taken through the prox
     int avail = cp;         // stream items so far (previous list + ring)
-    // the previous list's entries of the round at `done` (cur) and of the
-    // one after it (nxt, in flight while the round runs)
+    // the previous list's entries of the round at `done` (cur) and, in the
+    // id-order form, of the one after it (nxt, in flight while the round
+    // runs: 0.0180 -> 0.0176 ms per iteration at N = 5000, 0.0063 -> 0.0061
+    // at N = 1000); the spatial form's later rounds load their own (loaded
+    // a round ahead, its iteration took 0.0165 -> 0.0171 ms at N = 5000)
     float4 cur0 = pre0, cur1 = pre1, nxt0 = pre0, nxt1 = pre1;
     auto next_entries = [&]() {
         const int it = done + 64 + lane;
@@ -503,8 +511,9 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
         }
     };
     auto take_round = [&]() {
-        next_entries();
-        contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, cur0, cur1, Ln, base, diag, rho);
+        if constexpr (!SP) next_entries();
+        contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, SP && done > 0 ? Lp : nullptr, cur0, cur1, Ln,
+                                 base, diag, rho);
         done += 64;
         cur0 = nxt0;
         cur1 = nxt1;
@@ -558,7 +567,9 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
         wave_lds_fence();
         while (avail - done >= 64) take_round();
     }
-    if (avail > done) contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, cur0, cur1, Ln, base, diag, rho);
+    if (avail > done)
+        contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, SP && done > 0 ? Lp : nullptr, cur0, cur1, Ln, base,
+                                 diag, rho);
     if (lane == 0) a.cnt[tn] = base;
 
     wave_lds_fence();
