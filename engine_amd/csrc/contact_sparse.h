@@ -13,11 +13,11 @@
 //                  sums, (Z_k, c) for the tiles, the launch's sum buffer
 //                  cleared, and each block's box of the circles' common N,
 //                  widened by a rounding margin;
-//   k_scs_rows     one workgroup per block row P: the tiles (P, Q >= P) that
-//                  run (the two boxes meet, or the previous list holds
-//                  pairs), in Q order in LDS; block P's box and radius
-//                  edges; then its wavefronts take the row's running tiles
-//                  in turn (the id-order form's tile body).
+//   k_scs_rows     a few workgroups per block row P, each listing its share
+//                  of the tiles (P, Q >= P) that run (the two boxes meet, or
+//                  the previous list holds pairs) in Q order in LDS; block
+//                  P's box and radius edges; then its wavefronts take the
+//                  listed tiles in turn (the id-order form's tile body).
 // A pair whose blocks' boxes do not meet cannot pass the prox's collision
 // test (the margin covers the test's roundings), and an inactive pair that
 // is in no list adds nothing to any sum, so the skipped tiles change no

@@ -80,7 +80,8 @@ struct DenseState {
     bool c_rhoe = false;       // rho differs between edges (fixed while in the state)
     float* d_W = nullptr;      // [2][N] sum rho, sum rho^2 per circle (per-edge rho)
     unsigned* d_Lmask = nullptr; // [2][nts][cs][cs/32] the pairs of each list, per tile row
-    unsigned long long* d_Ac = nullptr;  // [3][CQ][N] circle sums of the last three launches of a run
+    unsigned long long* d_Ac = nullptr;  // [3][CQ][N] circle sums of the last three launches of a run (the
+                                         // spatial form uses the first two)
     float* d_Zb = nullptr;               // [2][N][3] Z inside a run (its canonical arrays hold the run's start)
     int c_run = 0;                       // contact launches since the last run end
     double* d_Rpc = nullptr;   // [N] primal residual per circle
@@ -292,8 +293,8 @@ __global__ __launch_bounds__(256) void k_contact_leave(CArgs a)
 }
 
 // ------------------------------------------------------------------ spatial form of the contact state
-// (contact_sparse.h): three launches per iteration, circles, tile
-// selection, the selected tiles.
+// (contact_sparse.h): two launches per iteration, the circles, then the
+// block rows' tile selection and tiles.
 
 template <int CS, bool RES>
 __global__ __launch_bounds__(256) void k_scs_circles(CArgs a)
