@@ -913,7 +913,7 @@ static int contact_tile_side(int N)
     return N >= 2500 ? 64 : 32;
 }
 
-// The spatial form: from N = 4500 circles (below, the id-order form's one
+// The spatial form: from N = 4000 circles (below, the id-order form's one
 // launch per iteration is faster than two), with one rho, alpha = 1 and
 // no rho policy (its application runs in the id-order form).
 // PARADMM_CONTACT_SPARSE = 0 or 1 overrides the size rule.
@@ -923,7 +923,7 @@ static bool contact_sparse_wanted(DevGraph* s, bool rho_edge)
     if (rho_edge || s->rho_policy) return false;
     if ((d->N + 15) / 16 > SCS_MAXB) return false;  // a row's blocks in LDS
     if (const char* e = getenv("PARADMM_CONTACT_SPARSE")) return e[0] == '1';
-    return d->N >= 4500;
+    return d->N >= 4000;
 }
 
 // Spatial tile side: 16 (PARADMM_SPARSE_TILE = 32 overrides): a block of
@@ -972,19 +972,22 @@ static int contact_alloc(DevGraph* s)
     d->nts = d->nbs * (d->nbs + 1) / 2;
     if (d->c_sparse) {
         RC(dmalloc(s, (void**)&d->d_bb, sizeof(float4) * (size_t)d->nbs));
-        // Workgroups of 4 wavefronts, each block row split over
-        // max(3, 1024 / nbs) of them (at most 8): the uneven rows are
-        // shared out better than by one workgroup per row (ms per
-        // iteration, N = 5000: one row of 8 wavefronts per workgroup
-        // 0.0190, split 3 ways 0.0163; N = 8000: 0.0337 unsplit, 0.0234
-        // split 3 ways; N = 10000: 0.0377, 0.0269; N = 3000: split 5 ways
-        // 0.0134, 3 ways 0.0159).  PARADMM_SPARSE_WAVES (2, 4, 8, 16) and
+        // Workgroups of 4 wavefronts, each block row split over several of
+        // them: the uneven rows are shared out better than by one
+        // workgroup per row, and with about 11 running tiles per row most
+        // wavefronts take one tile.  Up to 512 blocks max(5, 1024 / nbs)
+        // parts (at most 8), above 3 (ms per iteration, 2, 3, 4, 5, 6
+        // and 8 parts: N = 5000 0.0203, 0.0169, 0.0163, 0.0160, 0.0172,
+        // 0.0199; N = 8000 0.0262, 0.0229, 0.0233, 0.0225, 0.0227,
+        // 0.0276; N = 10000 0.0280, 0.0260, 0.0259, 0.0262, 0.0271,
+        // 0.0322; N = 5000 with one row per workgroup of 8 wavefronts
+        // 0.0190).  PARADMM_SPARSE_WAVES (2, 4, 8, 16) and
         // PARADMM_SPARSE_SPLIT (1-8) override.
         const char* e = getenv("PARADMM_SPARSE_WAVES");
         const int wv = e ? atoi(e) : 4;
         d->row_waves = wv == 2 || wv == 8 || wv == 16 ? wv : 4;
         e = getenv("PARADMM_SPARSE_SPLIT");
-        const int sp = e ? atoi(e) : (1024 / d->nbs > 3 ? 1024 / d->nbs : 3);
+        const int sp = e ? atoi(e) : d->nbs <= 512 ? (1024 / d->nbs > 5 ? 1024 / d->nbs : 5) : 3;
         d->row_split = sp < 1 ? 1 : sp > 8 ? 8 : sp;
     }
     const size_t nts = (size_t)d->nts, css = (size_t)d->cs * d->cs,
