@@ -200,6 +200,23 @@ __device__ __forceinline__ int block_offset(int n, int (&wsum)[4], int& total)
 // the wavefront's list stores to reach memory.
 __device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
+// Exclusive prefix over the wavefront's lanes of n (0 <= n < 2^BITS), and
+// the total, by one ballot per bit of n: no LDS round trips (a shuffle
+// scan takes six dependent ds_bpermute).
+template <int BITS>
+__device__ __forceinline__ int wave_prefix_small(int n, int& total)
+{
+    int pre = 0, all = 0;
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+        const unsigned long long m = __ballot((n >> b) & 1);
+        pre += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u)) << b;
+        all += (int)__popcll(m) << b;
+    }
+    total = all;
+    return pre;
+}
+
 // Shared scratch of one contact tile (one wavefront).  The pairs to project
 // form one stream per tile: the previous list's pairs, then the colliding
 // pairs found by the tests, which pass through a ring in LDS.
@@ -284,27 +301,22 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
 }
 
 // One round of the tile's pair stream: items d .. d + 63 (item < cp: entry
-// of the previous list, (pa, pb) loaded by the caller or, with Lp, here;
-// else ring slot item - cp); the active pairs are appended to the new list
-// in lane order.
+// of the previous list, (pa, pb) loaded by the caller; else ring slot
+// item - cp); the active pairs are appended to the new list in lane order.
 template <int CS, int Q, bool RE>
 __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArgs& a, int P, int Qb, int d, int avail,
-                                              int cp, const float4* Lp, float4 pa, float4 pb, float4* Ln, int& base,
-                                              bool diag, float rho)
+                                              int cp, float4 pa, float4 pb, float4* Ln, int& base, bool diag,
+                                              float rho)
 {
     constexpr int CSS = CS * CS;
     const int lane = threadIdx.x & 63;
     const int item = d + lane;
     bool act = false;
     int idx = 0;
-    float mA[3], mB[3];
+    float mA[3] = {0.0f, 0.0f, 0.0f}, mB[3] = {0.0f, 0.0f, 0.0f};
     if (item < avail) {
         const bool ex = item < cp;
         if (ex) {
-            if (Lp) {  // not loaded by the caller
-                pa = Lp[2 * item];
-                pb = Lp[2 * item + 1];
-            }
             idx = __float_as_int(pa.w) & (CSS - 1);
         } else {
             idx = sh.ring[(item - cp) & (contact_ring<CS>() - 1)];
@@ -318,12 +330,21 @@ __device__ __forceinline__ void contact_round(ContactWave<CS, Q>& sh, const CArg
         act = contact_pair<CS, Q, RE>(sh, idx, ex, pa, pb, diag, rA, rB, mA, mB);
     }
     const unsigned long long bal = __ballot(act);
-    if (act) {
-        const int slot = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
-        Ln[2 * slot] = make_float4(mA[0], mA[1], mA[2], __int_as_float(idx));
-        Ln[2 * slot + 1] = make_float4(mB[0], mB[1], mB[2], 0.0f);
-        atomicOr(&sh.nmask[idx / CS][(idx % CS) >> 5], 1u << ((idx % CS) & 31));
-    }
+    // the active pairs' entries in lane order after the list's `base`; the
+    // stores are issued by every lane (an inactive lane's offset lies past
+    // the list, where a raw buffer store is dropped), so no branch sits
+    // around them and waiting for a later load does not wait for them (the
+    // raw descriptor: stride 0, the list's CSS * 32 bytes, word 3 the
+    // 32-bit data format)
+    const __amdgpu_buffer_rsrc_t lr = __builtin_amdgcn_make_buffer_rsrc(Ln, 0, CSS * 32, 0x00020000);
+    const int slot = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
+    const int off = act ? slot * 32 : CSS * 32;
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const u4 ea = {__float_as_uint(mA[0]), __float_as_uint(mA[1]), __float_as_uint(mA[2]), (unsigned)idx};
+    const u4 eb = {__float_as_uint(mB[0]), __float_as_uint(mB[1]), __float_as_uint(mB[2]), 0u};
+    __builtin_amdgcn_raw_buffer_store_b128(ea, lr, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(eb, lr, off + 16, 0, 0);
+    if (act) atomicOr(&sh.nmask[idx / CS][(idx % CS) >> 5], 1u << ((idx % CS) & 31));
     base += (int)__popcll(bal);
 }
 
@@ -497,11 +518,11 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
     int base = 0;           // entries of the new list so far
     int done = 0;           // stream items taken through the prox
     int avail = cp;         // stream items so far (previous list + ring)
-    // the previous list's entries of the round at `done` (cur) and, in the
-    // id-order form, of the one after it (nxt, in flight while the round
-    // runs: 0.0180 -> 0.0176 ms per iteration at N = 5000, 0.0063 -> 0.0061
-    // at N = 1000); the spatial form's later rounds load their own (loaded
-    // a round ahead, its iteration took 0.0165 -> 0.0171 ms at N = 5000)
+    // the previous list's entries of the round at `done` (cur) and of the
+    // one after it (nxt, in flight while the round runs; the round's list
+    // stores are issued after those loads and are not waited for: id-order
+    // form 0.0180 -> 0.0176 ms per iteration at N = 5000, spatial form
+    // 0.0253 -> 0.0244 at N = 10000)
     float4 cur0 = pre0, cur1 = pre1, nxt0 = pre0, nxt1 = pre1;
     auto next_entries = [&]() {
         const int it = done + 64 + lane;
@@ -511,9 +532,8 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
         }
     };
     auto take_round = [&]() {
-        if constexpr (!SP) next_entries();
-        contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, SP && done > 0 ? Lp : nullptr, cur0, cur1, Ln,
-                                 base, diag, rho);
+        next_entries();
+        contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, cur0, cur1, Ln, base, diag, rho);
         done += 64;
         cur0 = nxt0;
         cur1 = nxt1;
@@ -549,15 +569,10 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
                 }
             qm &= vmask;
         }
-        const int n = __popc(qm);
-        int s = n;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(s, o, 64);
-            s += lane >= o ? y : 0;
-        }
-        int off = avail - cp + s - n;
-        avail += __shfl(s, 63, 64);
+        // n <= 2 RL: 3, 4 or 5 bits for tiles of 16, 32, 64
+        int tot;
+        int off = avail - cp + wave_prefix_small<RL == 2 ? 3 : RL == 4 ? 4 : 5>(__popc(qm), tot);
+        avail += tot;
         unsigned m = qm;
         while (m) {
             const int b = __ffs(m) - 1;
@@ -567,9 +582,7 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
         wave_lds_fence();
         while (avail - done >= 64) take_round();
     }
-    if (avail > done)
-        contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, SP && done > 0 ? Lp : nullptr, cur0, cur1, Ln, base,
-                                 diag, rho);
+    if (avail > done) contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, cur0, cur1, Ln, base, diag, rho);
     if (lane == 0) a.cnt[tn] = base;
 
     wave_lds_fence();
