@@ -97,15 +97,14 @@ constexpr int cwords() { return CS >= 32 ? CS / 32 : 1; }
 // v rounded to the nearest multiple of 2^-31, in units of 2^-31:
 // v = k + f with k = rint(v) and f = v - k exact (|f| <= 1/2), so
 // v 2^31 = k 2^31 + rint(f 2^31) needs no FP64.  A value outside
-// |v| < 2^30 (NaN included) gives 0 and sets bad.
+// |v| < 2^30 (NaN included) gives 0 and sets bad (a select, no branch).
 __device__ __forceinline__ long long cfix(float v, unsigned& bad)
 {
-    if (!(fabsf(v) < 1073741824.0f)) {
-        bad = 1u;
-        return 0;
-    }
-    const float k = rintf(v);
-    const float f = v - k;
+    const bool ok = fabsf(v) < 1073741824.0f;
+    bad |= ok ? 0u : 1u;
+    const float w = ok ? v : 0.0f;
+    const float k = rintf(w);
+    const float f = w - k;
     return (long long)(int)k * 2147483648LL + (long long)(int)rintf(f * 2147483648.0f);
 }
 
@@ -113,12 +112,9 @@ __device__ __forceinline__ long long cfix(float v, unsigned& bad)
 // 2^-31 below v, lo = rint((v - hi 2^-31) 2^62) with the remainder exact
 __device__ __forceinline__ void cfix2(float v, long long& hi, long long& lo, unsigned& bad)
 {
-    if (!(fabsf(v) < 1073741824.0f)) {
-        bad = 1u;
-        hi = lo = 0;
-        return;
-    }
-    const float t = v * 2147483648.0f;  // exact
+    const bool ok = fabsf(v) < 1073741824.0f;
+    bad |= ok ? 0u : 1u;
+    const float t = (ok ? v : 0.0f) * 2147483648.0f;  // exact
     const float k = floorf(t);
     hi = (long long)k;
     lo = (long long)rintf((t - k) * 2147483648.0f);  // t - k exact, in [0, 1)
@@ -127,17 +123,15 @@ __device__ __forceinline__ void cfix2(float v, long long& hi, long long& lo, uns
 // a circle sum (units of 2^-31) as fp32: one rounding, no FP64
 __device__ __forceinline__ float csum_f32(unsigned long long v) { return (float)(long long)v * 4.656612873077393e-10f; }
 
-template <typename SH>
-__device__ __forceinline__ void cadd(SH& sh, unsigned long long* acc, float v)
+__device__ __forceinline__ void cadd(unsigned& bad, unsigned long long* acc, float v)
 {
-    atomicAdd(acc, (unsigned long long)cfix(v, sh.bad));
+    atomicAdd(acc, (unsigned long long)cfix(v, bad));
 }
 
-template <typename SH>
-__device__ __forceinline__ void cadd2(SH& sh, unsigned long long* acc, float v)
+__device__ __forceinline__ void cadd2(unsigned& bad, unsigned long long* acc, float v)
 {
     long long hi, lo;
-    cfix2(v, hi, lo, sh.bad);
+    cfix2(v, hi, lo, bad);
     atomicAdd(acc, (unsigned long long)hi);
     atomicAdd(acc + 1, (unsigned long long)lo);
 }
@@ -244,6 +238,7 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
 {
     const int rl = idx / CS, cl = idx % CS;
     const int cs = diag ? cl : CS + cl;
+    unsigned bad = 0u;  // in a register: one LDS write per pair, not a branch per sum
     float zA[3], zB[3], cA[3], cB[3], uA[3], uB[3], nA[3], nB[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -279,13 +274,13 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
         for (int k = 0; k < 3; ++k) {
             const float gA = (xA[k] - zA[k]) - cA[k];
             const float gB = (xB[k] - zB[k]) - cB[k];
-            cadd(sh, &sh.acc[rl][3 + k], gA);
-            cadd(sh, &sh.acc[cs][3 + k], gB);
+            cadd(bad, &sh.acc[rl][3 + k], gA);
+            cadd(bad, &sh.acc[cs][3 + k], gB);
             gA2 += gA * gA;
             gB2 += gB * gB;
         }
-        cadd2(sh, &sh.acc[rl][6], gA2);
-        cadd2(sh, &sh.acc[cs][6], gB2);
+        cadd2(bad, &sh.acc[rl][6], gA2);
+        cadd2(bad, &sh.acc[cs][6], gB2);
     }
     if (act) {
 #pragma unroll
@@ -293,10 +288,11 @@ __device__ __forceinline__ bool contact_pair(ContactWave<CS, Q>& sh, int idx, bo
             mA[k] = xA[k] + uA[k];
             mB[k] = xB[k] + uB[k];
             // per-edge rho: the consensus weighs each deviation with its edge's rho
-            cadd(sh, &sh.acc[rl][k], RE ? rA * (mA[k] - zA[k]) : mA[k] - zA[k]);
-            cadd(sh, &sh.acc[cs][k], RE ? rB * (mB[k] - zB[k]) : mB[k] - zB[k]);
+            cadd(bad, &sh.acc[rl][k], RE ? rA * (mA[k] - zA[k]) : mA[k] - zA[k]);
+            cadd(bad, &sh.acc[cs][k], RE ? rB * (mB[k] - zB[k]) : mB[k] - zB[k]);
         }
     }
+    if (bad) sh.bad = 1u;
     return act;
 }
 
