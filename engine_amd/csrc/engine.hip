@@ -326,21 +326,30 @@ __global__ __launch_bounds__(256) void k_zpass_fused_wg(ZArgs a)
     float sr = 0.0f, sr2 = 0.0f, sm[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) sm[k] = 0.0f;
+    // every slot loads (a slot past the variable's edges reads edge 0, E >= 1,
+    // and takes 0 for its values), so no branch sits around the loads and
+    // the C slots' loads are in flight together; the sums add +0 for such a
+    // slot, which leaves them unchanged (a sum started at +0 never holds -0)
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const int p = b + tid + 256 * c;
+        const bool in = p < end;
+        const int pc = in ? p : 0;
+        const float rl = a.rho[pc];
+        float xl[D], ul[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) xs[c][k] = us[c][k] = 0.0f;
-        if (p < end) {
-            const float r = a.rho[p];
-            sr += r;
-            sr2 += r * r;
+        for (int k = 0; k < D; ++k) {
+            xl[k] = a.X[k * S + pc];
+            ul[k] = a.U[k * S + pc];
+        }
+        const float r = in ? rl : 0.0f;
+        sr += r;
+        sr2 += r * r;
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                xs[c][k] = a.X[k * S + p];
-                us[c][k] = a.U[k * S + p];
-                sm[k] += r * (xs[c][k] + us[c][k]);
-            }
+        for (int k = 0; k < D; ++k) {
+            xs[c][k] = in ? xl[k] : 0.0f;
+            us[c][k] = in ? ul[k] : 0.0f;
+            sm[k] += r * (xs[c][k] + us[c][k]);
         }
     }
     sr = wave_sum(sr);
@@ -366,15 +375,20 @@ __global__ __launch_bounds__(256) void k_zpass_fused_wg(ZArgs a)
         dz2 += t * t;
     }
     double pr = 0.0;
+    float al[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int p = b + tid + 256 * c;
+        al[c] = a.alpha[p < end ? p : 0];
+    }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const int p = b + tid + 256 * c;
         if (p < end) {
-            const float al = a.alpha[p];
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 const float t = xs[c][k] - z[k];
-                a.U[k * S + p] = us[c][k] + al * t;
+                a.U[k * S + p] = us[c][k] + al[c] * t;
                 pr += (double)t * (double)t;
             }
         }
@@ -1011,13 +1025,16 @@ struct XRowArgs {
     int from_n;
 };
 
-template <int T>
+template <int T, bool FROM_N>
 __global__ __launch_bounds__(256) void k_xpass_pair2d_rows(XRowArgs a)
 {
     // second circle's N (pos 2, rad 2) and rho (pos, rad), [.][jl][il]; each
     // thread overwrites its own entry's N with the second circle's X
     __shared__ float bin[6][T][T + 1];
+    constexpr int R = T * T / 256;  // pairs per thread in each phase
+    static_assert(T * T % 256 == 0 && 256 % T == 0, "whole rows of the tile per pass of the workgroup");
     const int tid = threadIdx.x;
+    if (a.nc < 2) return;  // no pairs (the stand-in pair below is (0, 1))
     // tile t of the upper triangle of nb x nb blocks.  Workgroups are dealt
     // to the 8 XCDs in turn (blockIdx % 8), each with its own L2; each XCD
     // takes a contiguous run of tiles instead, so the tiles (I, J), (I, J + 1)
@@ -1031,50 +1048,87 @@ __global__ __launch_bounds__(256) void k_xpass_pair2d_rows(XRowArgs a)
     while (I + 1 < a.nb && (I + 1) * a.nb - (I + 1) * I / 2 <= t) ++I;
     const int J = I + (t - (I * a.nb - I * (I - 1) / 2));
     const long S = a.stride;
-    // the second circles' inputs, lanes along i
-    for (int idx = tid; idx < T * T; idx += 256) {
+    // A thread's R pairs of a phase are loaded before any is used, and a
+    // pair outside the tile's triangle or past nc reads the stand-in pair
+    // (0, 1) instead (its values are not used): no branch around a load, so
+    // the R pairs' loads are in flight together.
+    // The second circles' inputs, lanes along i.
+    float in[R][6];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+        const int idx = tid + 256 * m;
         const int jl = idx / T, il = idx % T;
         const int i = I * T + il, j = J * T + jl;
-        if (i >= a.nc || j >= a.nc || i >= j) continue;
-        const long p = a.rowP[j] + i, q = a.rowR[j] + i;
+        const bool ok = i < a.nc && j < a.nc && i < j;
+        const int ie = ok ? i : 0, je = ok ? j : 1;
+        const long p = a.rowP[je] + ie, q = a.rowR[je] + ie;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            bin[k][jl][il] = a.from_n ? a.Nbuf[k * S + p] : a.Z[4L * j + k] - a.U[k * S + p];
-            bin[2 + k][jl][il] = a.from_n ? a.Nbuf[k * S + q] : a.Z[4L * j + 2 + k] - a.U[k * S + q];
+            in[m][k] = FROM_N ? a.Nbuf[k * S + p] : a.Z[4L * je + k] - a.U[k * S + p];
+            in[m][2 + k] = FROM_N ? a.Nbuf[k * S + q] : a.Z[4L * je + 2 + k] - a.U[k * S + q];
         }
-        bin[4][jl][il] = a.rho[p];
-        bin[5][jl][il] = a.rho[q];
+        in[m][4] = a.rho[p];
+        in[m][5] = a.rho[q];
+    }
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+        const int idx = tid + 256 * m;
+        const int jl = idx / T, il = idx % T;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) bin[f][jl][il] = in[m][f];
     }
     __syncthreads();
     // the first circles' edges, lanes along j; the prox
-    for (int idx = tid; idx < T * T; idx += 256) {
+    float fn[R][4], fr[R][2];
+    long fp[R], fq[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+        const int idx = tid + 256 * m;
+        const int il = idx / T, jl = idx % T;
+        const int i = I * T + il, j = J * T + jl;
+        const bool ok = i < a.nc && j < a.nc && i < j;
+        const int ie = ok ? i : 0, je = ok ? j : 1;
+        fp[m] = a.rowP[ie] + je - 1;
+        fq[m] = a.rowR[ie] + je - 1;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            fn[m][k] = FROM_N ? a.Nbuf[k * S + fp[m]] : a.Z[4L * ie + k] - a.U[k * S + fp[m]];
+            fn[m][2 + k] = FROM_N ? a.Nbuf[k * S + fq[m]] : a.Z[4L * ie + 2 + k] - a.U[k * S + fq[m]];
+        }
+        fr[m][0] = a.rho[fp[m]];
+        fr[m][1] = a.rho[fq[m]];
+    }
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+        const int idx = tid + 256 * m;
         const int il = idx / T, jl = idx % T;
         const int i = I * T + il, j = J * T + jl;
         if (i >= a.nc || j >= a.nc || i >= j) continue;
-        const long p = a.rowP[i] + j - 1, q = a.rowR[i] + j - 1;
         float n[4][2], x[4][2], rho[4];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            n[0][k] = a.from_n ? a.Nbuf[k * S + p] : a.Z[4L * i + k] - a.U[k * S + p];
-            n[1][k] = a.from_n ? a.Nbuf[k * S + q] : a.Z[4L * i + 2 + k] - a.U[k * S + q];
+            n[0][k] = fn[m][k];
+            n[1][k] = fn[m][2 + k];
             n[2][k] = bin[k][jl][il];
             n[3][k] = bin[2 + k][jl][il];
         }
-        rho[0] = a.rho[p];
-        rho[1] = a.rho[q];
+        rho[0] = fr[m][0];
+        rho[1] = fr[m][1];
         rho[2] = bin[4][jl][il];
         rho[3] = bin[5][jl][il];
         prox_pair2(x, n, rho);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            a.X[k * S + p] = x[0][k];
-            a.X[k * S + q] = x[1][k];
+            a.X[k * S + fp[m]] = x[0][k];
+            a.X[k * S + fq[m]] = x[1][k];
             bin[k][jl][il] = x[2][k];
             bin[2 + k][jl][il] = x[3][k];
         }
     }
     __syncthreads();
-    for (int idx = tid; idx < T * T; idx += 256) {
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+        const int idx = tid + 256 * m;
         const int jl = idx / T, il = idx % T;
         const int i = I * T + il, j = J * T + jl;
         if (i >= a.nc || j >= a.nc || i >= j) continue;
@@ -1105,7 +1159,8 @@ static int launch_x_pair2d_rows(DevGraph* s, int from_n)
     a.from_n = from_n;
     const unsigned nt = (unsigned)(a.nb * (a.nb + 1) / 2);
     prof_begin(s, "disk_collision_2d_rows");
-    k_xpass_pair2d_rows<XROW_T><<<8 * ((nt + 7) / 8), 256, 0, s->stream>>>(a);
+    if (from_n) k_xpass_pair2d_rows<XROW_T, true><<<8 * ((nt + 7) / 8), 256, 0, s->stream>>>(a);
+    else k_xpass_pair2d_rows<XROW_T, false><<<8 * ((nt + 7) / 8), 256, 0, s->stream>>>(a);
     prof_end(s);
     CK(hipGetLastError());
     return PARADMM_OK;
