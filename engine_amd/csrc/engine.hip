@@ -183,11 +183,9 @@ struct XArgs {
 // X pass: one thread per factor (per edge for unary kinds).  N = Z - U is
 // formed on the fly unless the N buffer is authoritative.
 template <int K>
-__global__ __launch_bounds__(256) void k_xpass(XArgs a)
+__device__ __forceinline__ void xpass_item(const XArgs& a, const int i)
 {
     constexpr int DEG = KT<K>::DEG, D = KT<K>::D, P = KT<K>::P;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.nitems) return;
     int e[DEG];
     float n[DEG][D], x[DEG][D], rho[DEG], prm[P > 0 ? P : 1];
 #pragma unroll
@@ -214,6 +212,13 @@ __global__ __launch_bounds__(256) void k_xpass(XArgs a)
     for (int s = 0; s < DEG; ++s)
 #pragma unroll
         for (int k = 0; k < D; ++k) a.X[k * a.stride + e[s]] = x[s][k];
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_xpass(XArgs a)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < a.nitems) xpass_item<K>(a, i);
 }
 
 __device__ __forceinline__ float wave_sum(float v)
@@ -1023,6 +1028,10 @@ struct XRowArgs {
     float* X;
     long stride;
     int from_n;
+    // the graph's wall and radius factors, taken by wblk and rblk
+    // workgroups after the pair tiles' (their edges are not the pairs')
+    XArgs wall, rad;
+    int wblk, rblk;
 };
 
 template <int T, bool FROM_N>
@@ -1034,12 +1043,22 @@ __global__ __launch_bounds__(256) void k_xpass_pair2d_rows(XRowArgs a)
     constexpr int R = T * T / 256;  // pairs per thread in each phase
     static_assert(T * T % 256 == 0 && 256 % T == 0, "whole rows of the tile per pass of the workgroup");
     const int tid = threadIdx.x;
-    if (a.nc < 2) return;  // no pairs (the stand-in pair below is (0, 1))
     // tile t of the upper triangle of nb x nb blocks.  Workgroups are dealt
     // to the 8 XCDs in turn (blockIdx % 8), each with its own L2; each XCD
     // takes a contiguous run of tiles instead, so the tiles (I, J), (I, J + 1)
     // that share the cache lines where their row segments meet share an L2
     const int nt = a.nb * (a.nb + 1) / 2, chunk = (nt + 7) / 8;
+    if ((int)blockIdx.x >= 8 * chunk) {  // the wall and radius factors, one per thread
+        const int b = (int)blockIdx.x - 8 * chunk;
+        if (b < a.wblk) {
+            if (b * 256 + tid < a.wall.nitems) xpass_item<PARADMM_KIND_WALL_2D>(a.wall, b * 256 + tid);
+        } else if (b - a.wblk < a.rblk) {
+            const int i = (b - a.wblk) * 256 + tid;
+            if (i < a.rad.nitems) xpass_item<PARADMM_KIND_RADIUS_2D>(a.rad, i);
+        }
+        return;
+    }
+    if (a.nc < 2) return;  // no pairs (the stand-in pair below is (0, 1))
     const int t = (int)(blockIdx.x % 8) * chunk + (int)(blockIdx.x / 8);
     if (t >= nt) return;
     int I = (int)((2.0f * a.nb + 1.0f - sqrtf((2.0f * a.nb + 1.0f) * (2.0f * a.nb + 1.0f) - 8.0f * (float)t)) * 0.5f);
@@ -1143,31 +1162,7 @@ __global__ __launch_bounds__(256) void k_xpass_pair2d_rows(XRowArgs a)
 
 constexpr int XROW_T = 32;
 
-static int launch_x_pair2d_rows(DevGraph* s, int from_n)
-{
-    XRowArgs a;
-    a.nc = s->rows2d_n;
-    a.nb = (a.nc + XROW_T - 1) / XROW_T;
-    a.rowP = s->d_rowP;
-    a.rowR = s->d_rowR;
-    a.Z = s->d_Z;
-    a.U = s->d_U;
-    a.Nbuf = s->d_N;
-    a.rho = s->d_rho;
-    a.X = s->d_X;
-    a.stride = s->stride;
-    a.from_n = from_n;
-    const unsigned nt = (unsigned)(a.nb * (a.nb + 1) / 2);
-    prof_begin(s, "disk_collision_2d_rows");
-    if (from_n) k_xpass_pair2d_rows<XROW_T, true><<<8 * ((nt + 7) / 8), 256, 0, s->stream>>>(a);
-    else k_xpass_pair2d_rows<XROW_T, false><<<8 * ((nt + 7) / 8), 256, 0, s->stream>>>(a);
-    prof_end(s);
-    CK(hipGetLastError());
-    return PARADMM_OK;
-}
-
-template <int K>
-static int launch_x_kind(DevGraph* s, const KindTable& kt, int from_n)
+static XArgs xargs(DevGraph* s, const KindTable& kt, int from_n)
 {
     XArgs a;
     a.nitems = kt.nitems;
@@ -1181,6 +1176,44 @@ static int launch_x_kind(DevGraph* s, const KindTable& kt, int from_n)
     a.X = s->d_X;
     a.stride = s->stride;
     a.from_n = from_n;
+    return a;
+}
+
+// The pair tiles, then (wall, rad: their kind tables, or null) the wall and
+// radius factors in the same launch: their latency hides in the pair
+// pass's tail instead of taking two launches of their own.
+static int launch_x_pair2d_rows(DevGraph* s, int from_n, const KindTable* wall, const KindTable* rad)
+{
+    XRowArgs a;
+    a.nc = s->rows2d_n;
+    a.nb = (a.nc + XROW_T - 1) / XROW_T;
+    a.rowP = s->d_rowP;
+    a.rowR = s->d_rowR;
+    a.Z = s->d_Z;
+    a.U = s->d_U;
+    a.Nbuf = s->d_N;
+    a.rho = s->d_rho;
+    a.X = s->d_X;
+    a.stride = s->stride;
+    a.from_n = from_n;
+    a.wall = wall ? xargs(s, *wall, from_n) : XArgs{};
+    a.rad = rad ? xargs(s, *rad, from_n) : XArgs{};
+    a.wblk = wall ? (int)nblk(wall->nitems) : 0;
+    a.rblk = rad ? (int)nblk(rad->nitems) : 0;
+    const unsigned nt = (unsigned)(a.nb * (a.nb + 1) / 2);
+    const unsigned grid = 8 * ((nt + 7) / 8) + (unsigned)(a.wblk + a.rblk);
+    prof_begin(s, "disk_collision_2d_rows");
+    if (from_n) k_xpass_pair2d_rows<XROW_T, true><<<grid, 256, 0, s->stream>>>(a);
+    else k_xpass_pair2d_rows<XROW_T, false><<<grid, 256, 0, s->stream>>>(a);
+    prof_end(s);
+    CK(hipGetLastError());
+    return PARADMM_OK;
+}
+
+template <int K>
+static int launch_x_kind(DevGraph* s, const KindTable& kt, int from_n)
+{
+    const XArgs a = xargs(s, kt, from_n);
     prof_begin(s, kind_info(K).name);
     k_xpass<K><<<nblk(kt.nitems), 256, 0, s->stream>>>(a);
     prof_end(s);
@@ -1267,14 +1300,25 @@ static int run_host_prox(DevGraph* s, int from_n)
 
 static int launch_xpass_generic(DevGraph* s, int from_n)
 {
+    // with the pairs in rows, the wall and radius factors go with the pair tiles
+    const KindTable *wall = nullptr, *rad = nullptr;
+    if (s->rows2d_n)
+        for (const KindTable& kt : s->kinds) {
+            if (kt.kind == PARADMM_KIND_WALL_2D) wall = &kt;
+            if (kt.kind == PARADMM_KIND_RADIUS_2D) rad = &kt;
+        }
     for (const KindTable& kt : s->kinds) {
         switch (kt.kind) {
         case PARADMM_KIND_PAIR_2D:
-            if (s->rows2d_n) RC(launch_x_pair2d_rows(s, from_n));
+            if (s->rows2d_n) RC(launch_x_pair2d_rows(s, from_n, wall, rad));
             else RC(launch_x_kind<PARADMM_KIND_PAIR_2D>(s, kt, from_n));
             break;
-        case PARADMM_KIND_WALL_2D: RC(launch_x_kind<PARADMM_KIND_WALL_2D>(s, kt, from_n)); break;
-        case PARADMM_KIND_RADIUS_2D: RC(launch_x_kind<PARADMM_KIND_RADIUS_2D>(s, kt, from_n)); break;
+        case PARADMM_KIND_WALL_2D:
+            if (!wall) RC(launch_x_kind<PARADMM_KIND_WALL_2D>(s, kt, from_n));
+            break;
+        case PARADMM_KIND_RADIUS_2D:
+            if (!rad) RC(launch_x_kind<PARADMM_KIND_RADIUS_2D>(s, kt, from_n));
+            break;
         case PARADMM_KIND_PAIR_3D: RC(launch_x_kind<PARADMM_KIND_PAIR_3D>(s, kt, from_n)); break;
         case PARADMM_KIND_BOX_3D: RC(launch_x_kind<PARADMM_KIND_BOX_3D>(s, kt, from_n)); break;
         case PARADMM_KIND_RADIUS_3D: RC(launch_x_kind<PARADMM_KIND_RADIUS_3D>(s, kt, from_n)); break;
