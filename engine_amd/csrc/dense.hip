@@ -975,19 +975,19 @@ static int contact_alloc(DevGraph* s)
         // Workgroups of 4 wavefronts, each block row split over several of
         // them: the uneven rows are shared out better than by one
         // workgroup per row, and with about 11 running tiles per row most
-        // wavefronts take one tile.  Up to 512 blocks max(5, 1024 / nbs)
-        // parts (at most 8), above 3 (ms per iteration, 2, 3, 4, 5, 6
-        // and 8 parts: N = 5000 0.0203, 0.0169, 0.0163, 0.0160, 0.0172,
-        // 0.0199; N = 8000 0.0262, 0.0229, 0.0233, 0.0225, 0.0227,
-        // 0.0276; N = 10000 0.0280, 0.0260, 0.0259, 0.0262, 0.0271,
-        // 0.0322; N = 5000 with one row per workgroup of 8 wavefronts
-        // 0.0190).  PARADMM_SPARSE_WAVES (2, 4, 8, 16) and
+        // wavefronts take one tile.  Up to 320 blocks (N = 5120)
+        // max(5, 1024 / nbs) parts (at most 8), above 3 (ms per iteration,
+        // 3, 4, 5 and 6 parts: N = 5000 0.0155, 0.0152, 0.0152, 0.0161;
+        // N = 5500 0.0166, 0.0175, 0.0166; N = 6000 0.0168, 0.0184, 0.0182;
+        // N = 8000 0.0208, 0.0216, 0.0210, 0.0214; N = 10000 0.0239,
+        // 0.0244, 0.0241, 0.0254; N = 5000 with one row per workgroup of 8
+        // wavefronts 0.0190).  PARADMM_SPARSE_WAVES (2, 4, 8, 16) and
         // PARADMM_SPARSE_SPLIT (1-8) override.
         const char* e = getenv("PARADMM_SPARSE_WAVES");
         const int wv = e ? atoi(e) : 4;
         d->row_waves = wv == 2 || wv == 8 || wv == 16 ? wv : 4;
         e = getenv("PARADMM_SPARSE_SPLIT");
-        const int sp = e ? atoi(e) : d->nbs <= 512 ? (1024 / d->nbs > 5 ? 1024 / d->nbs : 5) : 3;
+        const int sp = e ? atoi(e) : d->nbs <= 320 ? (1024 / d->nbs > 5 ? 1024 / d->nbs : 5) : 3;
         d->row_split = sp < 1 ? 1 : sp > 8 ? 8 : sp;
     }
     const size_t nts = (size_t)d->nts, css = (size_t)d->cs * d->cs,
