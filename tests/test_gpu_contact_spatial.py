@@ -98,3 +98,19 @@ def test_spatial_solve_equals_id_order_solve(monkeypatch):
     assert np.array_equal(z0, z1)
     print(f"solve: {r1['iterations']} iterations, converged {r1['converged']}, primal {r1['primal_rms']:.3e}, "
           f"same bits in both forms")
+
+
+@pytest.mark.parametrize("split,waves", [("1", "4"), ("3", "4"), ("8", "4"), ("3", "2"), ("2", "8"), ("1", "16")])
+def test_spatial_launch_shapes_give_the_same_bits(monkeypatch, split, waves):
+    """The row kernel's launch shape (workgroups per block row, wavefronts
+    per workgroup) only shares the tiles out: every shape gives the bits of
+    the default one (N = 3000, the default 5 parts of 4 wavefronts)."""
+    monkeypatch.delenv("PARADMM_SPARSE_SPLIT", raising=False)
+    monkeypatch.delenv("PARADMM_SPARSE_WAVES", raising=False)
+    ref, info_r, _ = _run(monkeypatch, "1", 3000, 4, 1.0)
+    monkeypatch.setenv("PARADMM_SPARSE_SPLIT", split)
+    monkeypatch.setenv("PARADMM_SPARSE_WAVES", waves)
+    got, info_g, kern = _run(monkeypatch, "1", 3000, 4, 1.0)
+    assert "scs_rows" in kern
+    assert info_g[2] == info_r[2]
+    _same(got, ref, f"split {split}, waves {waves}")
