@@ -179,14 +179,18 @@ __device__ __forceinline__ void scs_block_box(const CArgs& a, int c, bool valid,
 // holds pairs (their residual terms, and their prox if they still touch).
 // A tile that does not run gets an empty new list (its mask may stay
 // stale: a list of length 0 is read as empty).
+// (The box, both counts are loaded together and combined without
+// short-circuit operators: otherwise each is a load of its own behind a
+// branch, and the test waits for them one after another.)
 __device__ __forceinline__ bool scs_tile_runs(const CArgs& a, const float4 p, int P, int Qb)
 {
     const int t = P * a.nbs - P * (P - 1) / 2 + (Qb - P);
-    const float4 q = a.bb[Qb];
-    const bool meet = p.x <= q.y && q.x <= p.y && p.z <= q.w && q.z <= p.w;
     const long to = (long)a.par * a.nts + t, tn = (long)(a.par ^ 1) * a.nts + t;
-    const bool run = meet || a.cnt[to] > 0;
-    if (!run && a.cnt[tn] != 0) a.cnt[tn] = 0;
+    const float4 q = a.bb[Qb];
+    const int c_old = a.cnt[to], c_new = a.cnt[tn];
+    const bool meet = (p.x <= q.y) & (q.x <= p.y) & (p.z <= q.w) & (q.z <= p.w);
+    const bool run = meet | (c_old > 0);
+    if (!run & (c_new != 0)) a.cnt[tn] = 0;
     return run;
 }
 
