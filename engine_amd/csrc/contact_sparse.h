@@ -153,6 +153,34 @@ __device__ __forceinline__ void scs_unary(const CArgs& a, const int P)
 // |x1 - x2| <= |r1| + |r2| up to far less than that margin, and so for y.
 // Lanes of CS consecutive contact indices (CS divides 64) reduce by
 // shuffles; a lane without a circle adds an empty box.
+// v from another lane of its 16-lane row by DPP control CTRL (a VALU
+// operand modifier: no LDS round trip).  Every lane of the row is active.
+template <int CTRL>
+__device__ __forceinline__ float scs_dpp(float v)
+{
+    const int x = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false));
+}
+
+// min (MAX false) or max of v over each aligned group of G lanes (G = 16 or
+// 32), every lane of the group receiving it.  Within a 16-lane row: the
+// quad swaps [1,0,3,2] and [2,3,0,1] leave each lane its quad's value, the
+// half-row mirror (lane i of 8 <- lane 7 - i) its 8 lanes', the row mirror
+// (i <- 15 - i) its row's; min and max do not depend on the order.  Across
+// the two rows of 32, one shuffle.
+template <int G, bool MAX>
+__device__ __forceinline__ float scs_group_extreme(float v)
+{
+    static_assert(G == 16 || G == 32, "groups of 16 or 32 lanes");
+    auto op = [](float x, float y) { return MAX ? fmaxf(x, y) : fminf(x, y); };
+    v = op(v, scs_dpp<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = op(v, scs_dpp<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = op(v, scs_dpp<0x141>(v));  // row_half_mirror
+    v = op(v, scs_dpp<0x140>(v));  // row_mirror
+    if constexpr (G == 32) v = op(v, __shfl_xor(v, 16, 64));
+    return v;
+}
+
 template <int CS>
 __device__ __forceinline__ void scs_block_box(const CArgs& a, int c, bool valid, const float (&n)[3])
 {
@@ -161,14 +189,11 @@ __device__ __forceinline__ void scs_block_box(const CArgs& a, int c, bool valid,
     float lx = valid ? n[0] - ar : inf, hx = valid ? n[0] + ar : -inf;
     float ly = valid ? n[1] - ar : inf, hy = valid ? n[1] + ar : -inf;
     float mg = valid ? fabsf(n[0]) + fabsf(n[1]) + ar : 0.0f;
-#pragma unroll
-    for (int o = 1; o < CS; o <<= 1) {
-        lx = fminf(lx, __shfl_xor(lx, o, 64));
-        hx = fmaxf(hx, __shfl_xor(hx, o, 64));
-        ly = fminf(ly, __shfl_xor(ly, o, 64));
-        hy = fmaxf(hy, __shfl_xor(hy, o, 64));
-        mg = fmaxf(mg, __shfl_xor(mg, o, 64));
-    }
+    lx = scs_group_extreme<CS, false>(lx);
+    hx = scs_group_extreme<CS, true>(hx);
+    ly = scs_group_extreme<CS, false>(ly);
+    hy = scs_group_extreme<CS, true>(hy);
+    mg = scs_group_extreme<CS, true>(mg);
     if ((c % CS) == 0 && c < a.N) {
         const float d = 1e-5f * mg;
         a.bb[c / CS] = make_float4(lx - d, hx + d, ly - d, hy + d);
