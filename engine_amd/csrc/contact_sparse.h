@@ -55,8 +55,11 @@ __device__ __forceinline__ unsigned scs_morton(float x, float y)
 // c = Z_k - Z_{k-1}, to the table the tiles read (the same operations as
 // the id-order form's contact_zc, so the same bits); the launch's sum
 // buffer cleared.  Returns the circle's common N as the tiles form it.
+// A lane past the last circle (valid false) calls it with c = N - 1 and
+// stores nothing: the loads then need no branch, so they issue together
+// with the kernel argument loads they depend on.
 template <bool RES>
-__device__ __forceinline__ void scs_circle(const CArgs& a, const int c, float (&n)[3])
+__device__ __forceinline__ void scs_circle(const CArgs& a, const int c, const bool valid, float (&n)[3])
 {
     constexpr int Q = RES ? CQ : 3;
     const int N = a.N;
@@ -76,12 +79,14 @@ __device__ __forceinline__ void scs_circle(const CArgs& a, const int c, float (&
             cz[k] = z[k] - a.Zp[3 * pc + k];
         }
     }
+    if (valid) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) a.Zo[3 * c + k] = z[k];
-    a.zcv[2 * c] = make_float4(z[0], z[1], z[2], cz[0]);
-    a.zcv[2 * c + 1] = make_float4(cz[1], cz[2], 0.0f, 0.0f);
+        for (int k = 0; k < 3; ++k) a.Zo[3 * c + k] = z[k];
+        a.zcv[2 * c] = make_float4(z[0], z[1], z[2], cz[0]);
+        a.zcv[2 * c + 1] = make_float4(cz[1], cz[2], 0.0f, 0.0f);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) a.Ac[(long)q * N + c] = 0ull;
+        for (int q = 0; q < Q; ++q) a.Ac[(long)q * N + c] = 0ull;
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) n[k] = z[k] + cz[k];
 }
@@ -147,12 +152,6 @@ __device__ __forceinline__ void scs_unary(const CArgs& a, const int P)
     if (bad) atomicOr(a.flag, 1u);
 }
 
-// The box of a block's common N, x and y each widened by |r|, then by
-// 1e-5 (|x| + |y| + |r|)max: the collision test d2 < (r1 + r2)^2 rounds a
-// few times by 2^-24 relative, so a pair that passes it has
-// |x1 - x2| <= |r1| + |r2| up to far less than that margin, and so for y.
-// Lanes of CS consecutive contact indices (CS divides 64) reduce by
-// shuffles; a lane without a circle adds an empty box.
 // v from another lane of its 16-lane row by DPP control CTRL (a VALU
 // operand modifier: no LDS round trip).  Every lane of the row is active.
 template <int CTRL>
@@ -181,6 +180,12 @@ __device__ __forceinline__ float scs_group_extreme(float v)
     return v;
 }
 
+// The box of a block's common N, x and y each widened by |r|, then by
+// 1e-5 (|x| + |y| + |r|)max: the collision test d2 < (r1 + r2)^2 rounds a
+// few times by 2^-24 relative, so a pair that passes it has
+// |x1 - x2| <= |r1| + |r2| up to far less than that margin, and so for y.
+// Lanes of CS consecutive contact indices (CS divides 64) reduce by
+// scs_group_extreme; a lane without a circle adds an empty box.
 template <int CS>
 __device__ __forceinline__ void scs_block_box(const CArgs& a, int c, bool valid, const float (&n)[3])
 {

@@ -297,12 +297,12 @@ __global__ __launch_bounds__(256) void k_contact_leave(CArgs a)
 // block rows' tile selection and tiles.
 
 template <int CS, bool RES>
-__global__ __launch_bounds__(256) void k_scs_circles(CArgs a)
+__global__ __launch_bounds__(64) void k_scs_circles(CArgs a)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    float n[3] = {0.0f, 0.0f, 0.0f};
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    float n[3];
     const bool valid = c < a.N;
-    if (valid) scs_circle<RES>(a, c, n);
+    scs_circle<RES>(a, valid ? c : a.N - 1, valid, n);
     scs_block_box<CS>(a, c, valid, n);
 }
 
@@ -1045,13 +1045,13 @@ static int sparse_iteration(DevGraph* s, bool res)
     a.Ap = j ? A + ((j - 1) % 2) * AQ : nullptr;
     a.Zin = j ? d->d_Zb + ((j - 1) % 2) * ZN : nullptr;
     a.Zo = d->d_Zb + (j % 2) * ZN;
-    const unsigned cg = (unsigned)((d->N + 255) / 256), rg = (unsigned)(d->nbs * d->row_split);
+    const unsigned cg = (unsigned)((d->N + 63) / 64), rg = (unsigned)(d->nbs * d->row_split);
     const char* cn = res ? "scs_circles_res" : "scs_circles";
     const char* rn = res ? "scs_rows_res" : "scs_rows";
 #define SCS_LAUNCH(CS_, W_)                                                                           \
     do {                                                                                              \
-        res ? contact_launch(s, cn, k_scs_circles<CS_, true>, cg, 256, a)                             \
-            : contact_launch(s, cn, k_scs_circles<CS_, false>, cg, 256, a);                           \
+        res ? contact_launch(s, cn, k_scs_circles<CS_, true>, cg, 64, a)                              \
+            : contact_launch(s, cn, k_scs_circles<CS_, false>, cg, 64, a);                            \
         CK(hipGetLastError());                                                                        \
         res ? contact_launch(s, rn, k_scs_rows<CS_, true, W_>, rg, 64 * (W_), a, d->row_split)        \
             : contact_launch(s, rn, k_scs_rows<CS_, false, W_>, rg, 64 * (W_), a, d->row_split);      \
