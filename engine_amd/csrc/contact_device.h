@@ -450,10 +450,11 @@ __device__ __forceinline__ void contact_unary(const CArgs& a, int P)
 }
 
 
-// Contact tile t of the iteration (one wavefront, lane = threadIdx.x % 64,
-// sh its own scratch).
+// Contact tile t = (P, Qb) of the iteration (one wavefront, lane =
+// threadIdx.x % 64, sh its own scratch).
 template <int CS, bool RES, bool RE, bool SPATIAL = false>
-__device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, ContactWave<CS, RES ? CQ : 3>& sh)
+__device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, const int P, const int Qb,
+                                                  ContactWave<CS, RES ? CQ : 3>& sh)
 {
     constexpr int Q = RES ? CQ : 3;
     constexpr int CSS = CS * CS, RL = CS / 8, W = cwords<CS>();
@@ -461,9 +462,7 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
     // list of length 0 may have a stale mask (its tile was not launched)
     constexpr bool SP = SPATIAL;
     const int lane = threadIdx.x & 63;
-    const int N = a.N, nbs = a.nbs;
-    int P, Qb;
-    tri_tile(t, nbs, P, Qb);
+    const int N = a.N;
     const bool diag = P == Qb;
     const long tp = (long)a.par * a.nts + t, tn = (long)(a.par ^ 1) * a.nts + t;
     const float4* Lp = a.L + tp * (2 * CSS);
@@ -471,35 +470,60 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
     const int cp = min(a.cnt[tp], CSS);
     const float rho = a.uni[0];
 
-    // circle table: entries 0..CS-1 the rows' circles, CS..2CS-1 the columns'
-    for (int e = lane; e < 2 * CS; e += 64) {
+    // The tile's first loads are issued together, at clamped indices with
+    // no branch, before any is waited for (each behind a branch of its own
+    // was a round trip in turn): the circle table's (Z_k, c), entries
+    // 0..CS-1 the rows' circles, CS..2CS-1 the columns' (lane e + 64 i:
+    // entry e + 64 i); the previous list's mask (every tile's mask is inside
+    // the buffer; a stale one, spatial form, is replaced by 0); spatial form,
+    // the previous list's first 64 entries (within the list's capacity,
+    // before its length is known; entries past it are not used).
+    constexpr int NZ = (2 * CS + 63) / 64, NM = (CS * W + 63) / 64;
+    float zt[NZ][3], ct[NZ][3];
+#pragma unroll
+    for (int i = 0; i < NZ; ++i) {
+        const int e = min(lane + 64 * i, 2 * CS - 1);
         const int cc = min((e < CS ? P * CS + e : Qb * CS + (e - CS)), N - 1);
-        float z[3], c[3];
         if constexpr (SP) {
             const float4 u = a.zcv[2 * cc], v = a.zcv[2 * cc + 1];
-            z[0] = u.x, z[1] = u.y, z[2] = u.z;
-            c[0] = u.w, c[1] = v.x, c[2] = v.y;
+            zt[i][0] = u.x, zt[i][1] = u.y, zt[i][2] = u.z;
+            ct[i][0] = u.w, ct[i][1] = v.x, ct[i][2] = v.y;
         } else {
-            contact_zc<RE>(a, cc, z, c);
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            sh.zc[e][k] = z[k];
-            sh.zc[e][3 + k] = c[k];
+            contact_zc<RE>(a, cc, zt[i], ct[i]);
         }
     }
-    for (int e = lane; e < CS * W; e += 64) {
-        (&sh.mask[0][0])[e] = (!SP || cp) ? a.Lmask[tp * CS * W + e] : 0u;
+    unsigned mt[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) mt[i] = a.Lmask[tp * CS * W + min(lane + 64 * i, CS * W - 1)];
+    // the first 64 entries of the previous list, in flight during the tests
+    // (id-order form: only those of the list, after the table is written)
+    float4 pre0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pre1 = pre0;
+    if constexpr (SP) {
+        pre0 = Lp[2 * lane];
+        pre1 = Lp[2 * lane + 1];
+    }
+    // the writes at the same clamped indices, with no branch (a branch
+    // would take the loads with it): lanes past the last entry write the
+    // last entry's values again
+#pragma unroll
+    for (int i = 0; i < NZ; ++i) {
+        const int e = min(lane + 64 * i, 2 * CS - 1);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sh.zc[e][k] = zt[i][k];
+            sh.zc[e][3 + k] = ct[i][k];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        const int e = min(lane + 64 * i, CS * W - 1);
+        (&sh.mask[0][0])[e] = (!SP || cp) ? mt[i] : 0u;
         (&sh.nmask[0][0])[e] = 0u;
     }
     for (int k = lane; k < 2 * CS * Q; k += 64) (&sh.acc[0][0])[k] = 0ull;
     if (lane == 0) sh.bad = 0u;
     wave_lds_fence();
-    // the first 64 entries of the previous list, in flight during the tests
-    // (spatial form: loaded before the list's length is known, within the
-    // list's capacity; entries past it are not used)
-    float4 pre0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pre1 = pre0;
-    if (SP || lane < cp) {
+    if (!SP && lane < cp) {
         pre0 = Lp[2 * lane];
         pre1 = Lp[2 * lane + 1];
     }
@@ -579,6 +603,15 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
         while (avail - done >= 64) take_round();
     }
     if (avail > done) contact_round<CS, Q, RE>(sh, a, P, Qb, done, avail, cp, cur0, cur1, Ln, base, diag, rho);
+    // spatial form, where a wavefront takes tile after tile: entries loaded
+    // ahead and not used (no round took them) are waited for here, before
+    // the tile's stores and atomics; still in flight at the next tile, the
+    // first reuse of their registers would wait for those stores and
+    // atomics too (the memory counter is in order).  (A wavefront of the
+    // id-order form has no next tile.)
+    if constexpr (SP)
+        asm volatile("" ::"v"(cur0.x), "v"(cur0.y), "v"(cur0.z), "v"(cur0.w), "v"(cur1.x), "v"(cur1.y), "v"(cur1.z),
+                     "v"(cur1.w));
     if (lane == 0) a.cnt[tn] = base;
 
     wave_lds_fence();
@@ -595,6 +628,15 @@ __device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, C
             }
     }
     if (lane == 0 && sh.bad) atomicOr(a.flag, 1u);
+}
+
+// Contact tile t of the iteration, (P, Qb) found from t.
+template <int CS, bool RES, bool RE, bool SPATIAL = false>
+__device__ __forceinline__ void contact_tile_body(const CArgs& a, const int t, ContactWave<CS, RES ? CQ : 3>& sh)
+{
+    int P, Qb;
+    tri_tile(t, a.nbs, P, Qb);
+    contact_tile_body<CS, RES, RE, SPATIAL>(a, t, P, Qb, sh);
 }
 
 // The end of a run: Z_{k+1} = Z_k + sum a / (N + 1) from the last launch's

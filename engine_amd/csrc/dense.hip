@@ -348,9 +348,11 @@ __global__ __launch_bounds__(64 * SCS_WAVES) __attribute__((amdgpu_waves_per_eu(
     }
     // the box and radius edges on a wavefront with one tile fewer than the rest, if any
     if (h == 0 && w == (total % SCS_WAVES)) scs_unary<CS, RES>(a, P);
+    // the tile index is the wavefront's own (one LDS word for all lanes):
+    // read as uniform, the tile's addresses and list resource stay scalar
     for (int k = w; k < total; k += SCS_WAVES) {
-        const int Qb = run[k];
-        contact_tile_body<CS, RES, false, true>(a, P * nbs - P * (P - 1) / 2 + (Qb - P), sh[w]);
+        const int Qb = __builtin_amdgcn_readfirstlane((int)run[k]);
+        contact_tile_body<CS, RES, false, true>(a, P * nbs - P * (P - 1) / 2 + (Qb - P), P, Qb, sh[w]);
         wave_lds_fence();  // the next tile reuses the scratch
     }
 }
