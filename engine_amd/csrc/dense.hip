@@ -977,19 +977,24 @@ static int contact_alloc(DevGraph* s)
         // Workgroups of 4 wavefronts, each block row split over several of
         // them: the uneven rows are shared out better than by one
         // workgroup per row, and with about 11 running tiles per row most
-        // wavefronts take one tile.  Up to 320 blocks (N = 5120)
-        // max(5, 1024 / nbs) parts (at most 8), above 3 (ms per iteration,
-        // 3, 4, 5 and 6 parts: N = 5000 0.0155, 0.0152, 0.0152, 0.0161;
-        // N = 5500 0.0166, 0.0175, 0.0166; N = 6000 0.0168, 0.0184, 0.0182;
-        // N = 8000 0.0208, 0.0216, 0.0210, 0.0214; N = 10000 0.0239,
-        // 0.0244, 0.0241, 0.0254; N = 5000 with one row per workgroup of 8
-        // wavefronts 0.0190).  PARADMM_SPARSE_WAVES (2, 4, 8, 16) and
-        // PARADMM_SPARSE_SPLIT (1-8) override.
+        // wavefronts take one tile.  max(5, 1024 / nbs) parts (at most 8)
+        // while the nbs x 5 workgroups fit on the chip at once (6 per CU at
+        // the row kernel's 77 VGPRs: 1536 on 256 CUs), 4 up to 320 blocks
+        // (N = 5120), above 3 (us per iteration, 3, 4 and 5 parts: N = 4000
+        // 11.7, 11.6, 10.7; N = 4500 12.0, 12.1, 11.6; N = 4750 12.5, 12.6,
+        // 12.1; N = 5000 12.3, 12.3, 12.8; N = 5500 13.2, 14.5, 13.7;
+        // N = 6000 13.7, 14.7, 14.3; N = 8000 16.3, 17.3, 16.6; N = 10000
+        // 18.8, 20.0, 19.2; 8 wavefronts per workgroup or 2 slower at each
+        // N).  PARADMM_SPARSE_WAVES (2, 4, 8, 16) and PARADMM_SPARSE_SPLIT
+        // (1-8) override.
         const char* e = getenv("PARADMM_SPARSE_WAVES");
         const int wv = e ? atoi(e) : 4;
         d->row_waves = wv == 2 || wv == 8 || wv == 16 ? wv : 4;
         e = getenv("PARADMM_SPARSE_SPLIT");
-        const int sp = e ? atoi(e) : d->nbs <= 320 ? (1024 / d->nbs > 5 ? 1024 / d->nbs : 5) : 3;
+        const int sp = e ? atoi(e)
+                         : d->nbs * 5 <= 1536 ? (1024 / d->nbs > 5 ? 1024 / d->nbs : 5)
+                         : d->nbs <= 320      ? 4
+                                              : 3;
         d->row_split = sp < 1 ? 1 : sp > 8 ? 8 : sp;
     }
     const size_t nts = (size_t)d->nts, css = (size_t)d->cs * d->cs,
