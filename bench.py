@@ -10,7 +10,7 @@ launched through the engine's captured hipGraph and bracketed by HIP events
 on the engine's stream (the library synchronises before and after).  With
 one rho and alpha = 1 the engine runs the dense layout's contact state (only
 pairs whose prox is active keep per-edge values; every edge is still updated
-every iteration), from N = 4000 in its spatial form (circles in Morton
+every iteration), from N = 3200 in its spatial form (circles in Morton
 order, contact tiles only where two blocks of circles can touch).
 
 Prints one JSON line: value = edge-updates/s, plus the roofline of the

@@ -915,8 +915,10 @@ static int contact_tile_side(int N)
     return N >= 2500 ? 64 : 32;
 }
 
-// The spatial form: from N = 4000 circles (below, the id-order form's one
-// launch per iteration is faster than two), with one rho, alpha = 1 and
+// The spatial form: from N = 3200 circles (below, the id-order form's one
+// launch per iteration is as fast or faster: us per iteration, id-order and
+// spatial, N = 2500 9.6, 10.1; N = 3000 10.3, 10.3; N = 3250 11.0, 10.5;
+// N = 3500 11.9, 10.7; N = 4000 13.1, 10.8), with one rho, alpha = 1 and
 // no rho policy (its application runs in the id-order form).
 // PARADMM_CONTACT_SPARSE = 0 or 1 overrides the size rule.
 static bool contact_sparse_wanted(DevGraph* s, bool rho_edge)
@@ -925,7 +927,7 @@ static bool contact_sparse_wanted(DevGraph* s, bool rho_edge)
     if (rho_edge || s->rho_policy) return false;
     if ((d->N + 15) / 16 > SCS_MAXB) return false;  // a row's blocks in LDS
     if (const char* e = getenv("PARADMM_CONTACT_SPARSE")) return e[0] == '1';
-    return d->N >= 4000;
+    return d->N >= 3200;
 }
 
 // Spatial tile side: 16 (PARADMM_SPARSE_TILE = 32 overrides): a block of

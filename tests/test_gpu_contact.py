@@ -24,7 +24,7 @@ def contact_on(monkeypatch, request):
     """Every test on both contact tile sides of the id-order form (the
     default picks 32 below N = 2500) and on the spatial form (circles in
     Morton order, tiles of 16 only where blocks meet; the default from
-    N = 4000 when no rho policy is attached)."""
+    N = 3200 when no rho policy is attached)."""
     monkeypatch.delenv("PARADMM_LAYOUT", raising=False)
     monkeypatch.delenv("PARADMM_CONTACT", raising=False)
     monkeypatch.delenv("PARADMM_DENSE_TILE", raising=False)

@@ -69,10 +69,10 @@ def test_spatial_form_equals_id_order(monkeypatch, nc, seed, rho, tile):
     print(f"N={nc}: spatial form equals the id-order form bit for bit, {info_g[2]} listed pairs")
 
 
-def test_spatial_form_is_the_default_from_4000(monkeypatch):
+def test_spatial_form_is_the_default_from_3200(monkeypatch):
     monkeypatch.delenv("PARADMM_CONTACT_SPARSE", raising=False)
     monkeypatch.delenv("PARADMM_CONTACT", raising=False)
-    for nc, kern in ((3999, "contact_tile"), (4000, "scs_rows")):
+    for nc, kern in ((3199, "contact_tile"), (3200, "scs_rows")):
         g = P.packing_square(nc, 0, 1.0, 1.0, 1.0)
         g.iterate(3)
         assert kern in set(g.profile_kernels(2))
